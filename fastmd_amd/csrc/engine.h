@@ -1,0 +1,112 @@
+// engine.h -- the device-resident multi-replica MD engine (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "fmd/fmd.h"
+#include "fmd_device.h"
+#include "kernels.h"
+
+namespace fmd {
+
+constexpr int kMaxReplicas = 4096;
+constexpr int kNumClasses = 3;
+constexpr int kGraphSteps = 20;
+
+int size_class(int natoms);
+
+struct Replica {
+    int id = -1;
+    int natoms = 0, nslots = 0, ntiles = 0, cls = 0;
+    int ensemble = 0;
+    RepDesc desc{};         // host copy of the device descriptor
+    RepState* st = nullptr; // device
+    std::vector<void*> allocs;
+    uint64_t bytes = 0;
+    int64_t steps_done = 0; // host mirror of st->step
+    int64_t thermo_read = 0;
+    int thermo_cap = 0;
+};
+
+struct Batch {
+    std::vector<int> ids;   // replica ids, ordered by size class
+    Tile* d_tiles = nullptr;
+    int* d_reps = nullptr;
+    int ntl = 0, nr = 0;
+    int cls_t0[kNumClasses] = {0, 0, 0}, cls_nt[kNumClasses] = {0, 0, 0};
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+    int tune_version = -1;
+};
+
+class Engine {
+  public:
+    Engine(int device, uint64_t budget);
+    ~Engine();
+    bool ok() const { return init_ok_; }
+    const char* last_error() const { return err_.c_str(); }
+    int32_t fail(int32_t code, const std::string& msg);
+
+    int32_t create(const fmd_system_desc* d, int32_t* sys);
+    int32_t destroy(int32_t sys);
+    uint64_t bytes_in_use() const { return in_use_; }
+    uint64_t budget() const { return budget_; }
+
+    int32_t set_force_cap(int32_t sys, float cap);
+    int32_t set_langevin(int32_t sys, float T0, float T1, float Tdamp);
+    int32_t build_neighbors(int32_t sys);
+    int32_t get_pairs(int32_t sys, int64_t* n, int32_t* ij, int64_t cap);
+    int32_t get_cell_ids(int32_t sys, int32_t* out, int32_t* nx);
+    int32_t get_morton(int32_t sys, uint32_t* out);
+    int32_t compute_forces(int32_t sys, uint32_t terms, float* f4, double* vir, double* pe);
+
+    int32_t step(const int32_t* ids, int32_t n, int64_t nsteps, bool use_graph);
+    // Non-blocking pieces used by the server: set run lengths, launch k steps.
+    int32_t begin_run(const std::vector<int>& ids, int64_t nsteps);
+    int32_t launch_steps(const std::vector<int>& ids, int64_t k, bool use_graph);
+    int32_t sync();
+    int32_t check_errors(const std::vector<int>& ids, std::vector<int>* failed);
+    int32_t replica_error(int32_t sys);
+    // Frees every cached batch except the one for `keep` (sorted or not).
+    void trim_batches(const std::vector<int>& keep);
+
+    int64_t step_count(int32_t sys);
+    int64_t rebuilds(int32_t sys);
+    int32_t get_state(int32_t sys, float* pos4, float* vel3, int32_t* img3);
+    int32_t get_box(int32_t sys, double* L);
+    int32_t get_thermo(int32_t sys, fmd_thermo* out);
+    int32_t drain_thermo(int32_t sys, fmd_thermo* out, int32_t cap, int32_t* n, int64_t* lost);
+
+    int32_t autotune(const int32_t* ids, int32_t n, int reps);
+    int32_t time_force(const int32_t* ids, int32_t n, uint32_t terms, int reps, double* ms,
+                       double* bytes, int64_t* pairs);
+    int tuned(int kernel, int cls) const { return tuned_[kernel][cls]; }
+    hipStream_t stream() const { return stream_; }
+    Replica* rep(int32_t sys);
+
+  private:
+    int32_t hip_check(hipError_t e, const char* what);
+    void* dalloc(Replica& r, size_t bytes, bool* ok);
+    Batch* batch_for(const std::vector<int>& ids);
+    void drop_batches_with(int id);
+    void launch_step_seq(Batch& b);
+    Launch launch_of(const Batch& b) const;
+    int32_t read_state(Replica& r, RepState& s);
+
+    int device_ = 0;
+    bool init_ok_ = false;
+    std::string err_;
+    hipStream_t stream_ = nullptr;
+    uint64_t budget_ = 0, in_use_ = 0;
+    std::vector<Replica*> reps_;
+    RepDesc* d_descs_ = nullptr;
+    std::map<std::vector<int>, Batch*> batches_;
+    int tuned_[2][kNumClasses];
+    int tune_version_ = 0;
+};
+
+}  // namespace fmd

@@ -1,0 +1,975 @@
+// engine.hip -- host side of the device-resident multi-replica MD engine.
+//
+// Each replica owns its device buffers (sized at creation, never resized) and
+// one RepDesc entry in a device table of kMaxReplicas.  A Batch is a set of
+// replicas with its device tile list; its K-step launch sequence is captured
+// once into a hipGraph and replayed.  Nothing in the step loop reads back to
+// the host: rebuild decisions, thermostat/barostat scalars and thermo records
+// all stay on the device.
+#include "engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "kernels.h"
+
+namespace fmd {
+
+int size_class(int natoms) { return natoms < 8192 ? 0 : (natoms < 32768 ? 1 : 2); }
+
+static inline int ceil64(int n) { return (n + 63) / 64 * 64; }
+
+Engine::Engine(int device, uint64_t budget) : device_(device) {
+    for (int k = 0; k < 2; k++)
+        for (int c = 0; c < kNumClasses; c++) tuned_[k][c] = 256;
+    if (hip_check(hipSetDevice(device), "hipSetDevice")) return;
+    if (hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate")) return;
+    size_t fr = 0, tot = 0;
+    if (hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo")) return;
+    budget_ = budget ? budget : (uint64_t)(0.9 * (double)fr);
+    if (hip_check(hipMalloc(&d_descs_, sizeof(RepDesc) * kMaxReplicas), "hipMalloc(descs)")) return;
+    reps_.assign(kMaxReplicas, nullptr);
+    init_ok_ = true;
+}
+
+Engine::~Engine() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (auto& kv : batches_) {
+        Batch* b = kv.second;
+        if (b->graph) (void)hipGraphExecDestroy(b->graph);
+        (void)hipFree(b->d_tiles);
+        (void)hipFree(b->d_reps);
+        delete b;
+    }
+    for (Replica* r : reps_)
+        if (r) {
+            for (void* p : r->allocs) (void)hipFree(p);
+            delete r;
+        }
+    if (d_descs_) (void)hipFree(d_descs_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+int32_t Engine::fail(int32_t code, const std::string& msg) {
+    err_ = msg;
+    return code;
+}
+
+int32_t Engine::hip_check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return FMD_OK;
+    err_ = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? FMD_ERR_NOMEM : FMD_ERR_HIP;
+}
+
+Replica* Engine::rep(int32_t sys) {
+    if (sys < 0 || sys >= kMaxReplicas) return nullptr;
+    return reps_[sys];
+}
+
+void* Engine::dalloc(Replica& r, size_t bytes, bool* ok) {
+    void* p = nullptr;
+    if (bytes == 0) bytes = 16;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        *ok = false;
+        return nullptr;
+    }
+    r.allocs.push_back(p);
+    r.bytes += bytes;
+    return p;
+}
+
+// ---------------------------------------------------------------------------
+// sizing
+// ---------------------------------------------------------------------------
+struct Sizing {
+    int natoms, nslots, ntiles, kcap, bcap, acap, ncell_cap, thermo_cap;
+    uint64_t bytes;
+};
+
+static int cbits_for(int nx) {
+    int cb = 0;
+    while ((1 << cb) < nx) cb++;
+    return cb;
+}
+
+static Sizing sizing(const fmd_system_desc* d, int bcap, int acap) {
+    Sizing s;
+    s.natoms = d->natoms;
+    s.nslots = ceil64(d->natoms);
+    s.ntiles = s.nslots / 64;
+    const double rcs = (double)d->rc + (double)d->skin;
+    const double rho = (double)d->natoms / ((double)d->L * d->L * d->L);
+    // capacity rule of src/neighbor/verlet_list.cu:4-7: max(256, 3 rho 4/3 pi rcs^3);
+    // only the filled rows are streamed, so the reserve costs memory, not bandwidth
+    const double expect = rho * 4.0 / 3.0 * M_PI * rcs * rcs * rcs;
+    int kc = std::max(256, (int)(3.0 * expect));
+    kc = (kc + 7) / 8 * 8;
+    if (kc > s.nslots) kc = s.nslots;  // cannot have more than N-1 neighbors
+    if (kc < 8) kc = 8;
+    s.kcap = kc;
+    s.bcap = std::max(1, bcap);
+    s.acap = std::max(1, acap);
+    float rcsf = d->rc + d->skin;
+    double growth = (d->ensemble == FMD_NPT_NH) ? 1.5 : 1.0;
+    int nxmax = std::max(1, (int)(growth * (double)d->L / (double)rcsf) + 1);
+    int cb = cbits_for(nxmax);
+    s.ncell_cap = 1 << (3 * cb);
+    s.thermo_cap = d->thermo_freq > 0 ? 4096 : 0;
+    uint64_t b = 0;
+    b += (uint64_t)s.nslots * (8 * 16 + 7 * 4);
+    b += (uint64_t)s.natoms * 4;
+    b += (uint64_t)s.ncell_cap * 8 + 4;
+    b += (uint64_t)s.ntiles * 64 * ((uint64_t)s.kcap * 4 + (uint64_t)s.bcap * 4 + (uint64_t)s.acap * 8);
+    b += (uint64_t)s.ntiles * kNumPart * 8;
+    b += (uint64_t)(d->natoms + 1) * 8 + (uint64_t)d->nbonds * 8 + (uint64_t)d->nangles * 48;
+    b += sizeof(RepState) + (uint64_t)s.thermo_cap * sizeof(ThermoRec) + 4096;
+    s.bytes = b;
+    return s;
+}
+
+// Per-atom bonded incidence counts (max over atoms), needed for the table caps.
+static void incidence_caps(const fmd_system_desc* d, int* bcap, int* acap) {
+    std::vector<int> nb(d->natoms, 0), na(d->natoms, 0);
+    for (int b = 0; b < d->nbonds; b++) { nb[d->bonds[2 * b]]++; nb[d->bonds[2 * b + 1]]++; }
+    for (int a = 0; a < d->nangles; a++)
+        for (int q = 0; q < 3; q++) na[d->angles[4 * a + q]]++;
+    *bcap = 0; *acap = 0;
+    for (int i = 0; i < d->natoms; i++) { *bcap = std::max(*bcap, nb[i]); *acap = std::max(*acap, na[i]); }
+}
+
+static int32_t validate(const fmd_system_desc* d, std::string& msg) {
+    if (!d) { msg = "null descriptor"; return FMD_ERR_ARG; }
+    if (d->natoms <= 0 || d->natoms > kSlotMask) { msg = "natoms out of range"; return FMD_ERR_ARG; }
+    if (d->ntypes < 1 || d->ntypes > kMaxTypes) { msg = "ntypes must be in [1,16]"; return FMD_ERR_ARG; }
+    if (!d->pos || !d->lj) { msg = "pos and lj are required"; return FMD_ERR_ARG; }
+    if (!(d->L > 0.0f) || !(d->rc > 0.0f) || d->skin < 0.0f || !(d->dt > 0.0f)) {
+        msg = "need L > 0, rc > 0, skin >= 0, dt > 0";
+        return FMD_ERR_ARG;
+    }
+    if (d->ensemble < 0 || d->ensemble > 2) { msg = "bad ensemble"; return FMD_ERR_ARG; }
+    if (!(d->Tdamp > 0.0f)) { msg = "Tdamp must be > 0"; return FMD_ERR_ARG; }
+    if (d->ensemble == FMD_NPT_NH && !(d->Pdamp > 0.0f)) { msg = "Pdamp must be > 0"; return FMD_ERR_ARG; }
+    if (d->ensemble != FMD_LANGEVIN && (d->chain_length < 1)) { msg = "chain_length must be >= 1"; return FMD_ERR_ARG; }
+    for (int i = 0; i < d->natoms; i++) {
+        int t = type_of(d->pos[4 * i + 3]);
+        if (t < 0 || t >= d->ntypes) { msg = "atom type out of range"; return FMD_ERR_ARG; }
+    }
+    if (d->nbonds < 0 || d->nangles < 0) { msg = "negative counts"; return FMD_ERR_ARG; }
+    if (d->nbonds > 0 && (!d->bonds || !d->bond_types || !d->bond_params)) { msg = "bonds need bonds, bond_types, bond_params"; return FMD_ERR_ARG; }
+    if (d->nangles > 0 && (!d->angles || !d->angle_params)) { msg = "angles need angles and angle_params"; return FMD_ERR_ARG; }
+    for (int b = 0; b < d->nbonds; b++) {
+        int i = d->bonds[2 * b], j = d->bonds[2 * b + 1], t = d->bond_types[b];
+        if (i < 0 || j < 0 || i >= d->natoms || j >= d->natoms || i == j) { msg = "bond atom index out of range"; return FMD_ERR_ARG; }
+        if (t < 0 || t >= d->nbond_types || t > 63) { msg = "bond type out of range"; return FMD_ERR_ARG; }
+    }
+    for (int a = 0; a < d->nangles; a++) {
+        for (int q = 0; q < 3; q++) {
+            int i = d->angles[4 * a + q];
+            if (i < 0 || i >= d->natoms) { msg = "angle atom index out of range"; return FMD_ERR_ARG; }
+        }
+        int t = d->angles[4 * a + 3];
+        if (t < 0 || t >= d->nangle_types || t > 63) { msg = "angle type out of range"; return FMD_ERR_ARG; }
+    }
+    return FMD_OK;
+}
+
+uint64_t system_bytes(const fmd_system_desc* d) {
+    std::string m;
+    if (validate(d, m)) return 0;
+    int bc, ac;
+    incidence_caps(d, &bc, &ac);
+    return sizing(d, bc, ac).bytes;
+}
+
+// ---------------------------------------------------------------------------
+// create / destroy
+// ---------------------------------------------------------------------------
+int32_t Engine::create(const fmd_system_desc* d, int32_t* sys) {
+    std::string msg;
+    if (int32_t rc = validate(d, msg)) return fail(rc, msg);
+    int bcap, acap;
+    incidence_caps(d, &bcap, &acap);
+    if (bcap > 255 || acap > 255) return fail(FMD_ERR_ARG, "more than 255 bonds or angles on one atom");
+    Sizing S = sizing(d, bcap, acap);
+    if (in_use_ + S.bytes > budget_)
+        return fail(FMD_ERR_BUDGET, "system needs " + std::to_string(S.bytes) +
+                                        " bytes; budget left " + std::to_string(budget_ - in_use_));
+    int id = -1;
+    for (int k = 0; k < kMaxReplicas; k++)
+        if (!reps_[k]) { id = k; break; }
+    if (id < 0) return fail(FMD_ERR_BUDGET, "replica table full");
+    Replica* R = new Replica();
+    Replica& r = *R;
+    r.id = id;
+    r.natoms = S.natoms; r.nslots = S.nslots; r.ntiles = S.ntiles;
+    r.cls = size_class(S.natoms);
+    r.ensemble = d->ensemble;
+    r.thermo_cap = S.thermo_cap;
+    RepDesc& D = r.desc;
+    std::memset(&D, 0, sizeof(D));
+    D.natoms = S.natoms; D.nslots = S.nslots; D.ntiles = S.ntiles; D.ntypes = d->ntypes;
+    D.kcap = S.kcap; D.bcap = S.bcap; D.acap = S.acap; D.ncell_cap = S.ncell_cap;
+    D.ensemble = d->ensemble;
+    D.rc2 = d->rc * d->rc;
+    D.rcs = d->rc + d->skin;
+    D.rcs2 = D.rcs * D.rcs;
+    float hs = d->skin * 0.5f;
+    D.half_skin2 = hs * hs;
+    D.dt = d->dt;
+    D.half_dt = 0.5f * d->dt;
+    D.c1 = std::exp(-(1.0f / d->Tdamp) * d->dt);  // src/integrate/langevin.cu:17-19
+    D.c2 = std::sqrt(1.0f - D.c1 * D.c1);
+    D.fcap = d->force_cap > 0.0f ? d->force_cap : 0.0f;
+    D.clamp_lj = 1;
+    D.seed_lo = (uint32_t)d->seed;
+    D.seed_hi = (uint32_t)(d->seed >> 32);
+
+    bool ok = true;
+    const size_t ns = S.nslots, nt = S.ntiles, na = S.natoms;
+    D.pos = (float4*)dalloc(r, ns * 16, &ok);
+    D.vel = (float4*)dalloc(r, ns * 16, &ok);
+    D.force = (float4*)dalloc(r, ns * 16, &ok);
+    D.pos_ref = (float4*)dalloc(r, ns * 16, &ok);
+    D.img = (int4*)dalloc(r, ns * 16, &ok);
+    D.tmp_pos = (float4*)dalloc(r, ns * 16, &ok);
+    D.tmp_vel = (float4*)dalloc(r, ns * 16, &ok);
+    D.tmp_img = (int4*)dalloc(r, ns * 16, &ok);
+    D.cell_of = (int*)dalloc(r, ns * 4, &ok);
+    D.rank = (int*)dalloc(r, ns * 4, &ok);
+    D.order = (int*)dalloc(r, ns * 4, &ok);
+    D.order2 = (int*)dalloc(r, ns * 4, &ok);
+    D.slot_of = (int*)dalloc(r, na * 4, &ok);
+    D.cell_cnt = (int*)dalloc(r, (size_t)S.ncell_cap * 4, &ok);
+    D.cell_start = (int*)dalloc(r, ((size_t)S.ncell_cap + 1) * 4, &ok);
+    D.nnbr = (int*)dalloc(r, ns * 4, &ok);
+    D.nbr = (int*)dalloc(r, nt * S.kcap * 64 * 4, &ok);
+    D.bhdr = (int*)dalloc(r, ns * 4, &ok);
+    D.bnd = (int*)dalloc(r, nt * S.bcap * 64 * 4, &ok);
+    D.ang = (int2*)dalloc(r, nt * S.acap * 64 * 8, &ok);
+    D.wpart = (double*)dalloc(r, nt * kNumPart * 8, &ok);
+    r.st = (RepState*)dalloc(r, sizeof(RepState), &ok);
+    D.st = r.st;
+    D.thermo = (ThermoRec*)dalloc(r, (size_t)std::max(1, S.thermo_cap) * sizeof(ThermoRec), &ok);
+
+    // topology by global id
+    std::vector<int> boff(na + 1, 0), aoff(na + 1, 0);
+    for (int b = 0; b < d->nbonds; b++) { boff[d->bonds[2 * b] + 1]++; boff[d->bonds[2 * b + 1] + 1]++; }
+    for (int a = 0; a < d->nangles; a++)
+        for (int q = 0; q < 3; q++) aoff[d->angles[4 * a + q] + 1]++;
+    for (size_t i = 0; i < na; i++) { boff[i + 1] += boff[i]; aoff[i + 1] += aoff[i]; }
+    std::vector<int> blst(std::max(1, boff[na]));
+    std::vector<int4> alst(std::max(1, aoff[na]));
+    {
+        std::vector<int> fb(boff.begin(), boff.end() - 1), fa(aoff.begin(), aoff.end() - 1);
+        for (int b = 0; b < d->nbonds; b++) {
+            int i = d->bonds[2 * b], j = d->bonds[2 * b + 1], t = d->bond_types[b];
+            blst[fb[i]++] = j | (t << kSlotBits);
+            blst[fb[j]++] = i | (t << kSlotBits);
+        }
+        for (int a = 0; a < d->nangles; a++) {
+            int4 A = make_int4(d->angles[4 * a], d->angles[4 * a + 1], d->angles[4 * a + 2], d->angles[4 * a + 3]);
+            alst[fa[A.x]++] = A;
+            if (A.y != A.x) alst[fa[A.y]++] = A;
+            if (A.z != A.x && A.z != A.y) alst[fa[A.z]++] = A;
+        }
+        // an atom listed twice in one angle gets one entry; fix the offsets
+        std::vector<int> cnt(na, 0);
+        for (int a = 0; a < d->nangles; a++) {
+            int x = d->angles[4 * a], y = d->angles[4 * a + 1], z = d->angles[4 * a + 2];
+            cnt[x]++;
+            if (y != x) cnt[y]++;
+            if (z != x && z != y) cnt[z]++;
+        }
+        std::vector<int> aoff2(na + 1, 0);
+        std::vector<int4> alst2(std::max(1, aoff[na]));
+        for (size_t i = 0; i < na; i++) {
+            aoff2[i + 1] = aoff2[i] + cnt[i];
+            for (int k = 0; k < cnt[i]; k++) alst2[aoff2[i] + k] = alst[aoff[i] + k];
+        }
+        aoff.swap(aoff2);
+        alst.swap(alst2);
+    }
+    int* t_boff = (int*)dalloc(r, (na + 1) * 4, &ok);
+    int* t_blst = (int*)dalloc(r, blst.size() * 4, &ok);
+    int* t_aoff = (int*)dalloc(r, (na + 1) * 4, &ok);
+    int4* t_alst = (int4*)dalloc(r, alst.size() * 16, &ok);
+    float2* lj = (float2*)dalloc(r, (size_t)d->ntypes * d->ntypes * 8, &ok);
+    float4* bp = (float4*)dalloc(r, (size_t)std::max(1, d->nbond_types) * 16, &ok);
+    float2* ap = (float2*)dalloc(r, (size_t)std::max(1, d->nangle_types) * 8, &ok);
+    float* ms = (float*)dalloc(r, kMaxTypes * 4, &ok);
+    D.t_boff = t_boff; D.t_blst = t_blst; D.t_aoff = t_aoff; D.t_alst = t_alst;
+    D.lj = lj; D.bparm = bp; D.aparm = ap; D.mass = ms;
+    if (!ok) {
+        for (void* p : r.allocs) (void)hipFree(p);
+        delete R;
+        return fail(FMD_ERR_NOMEM, "device allocation failed");
+    }
+
+    // host staging of the per-slot state
+    std::vector<float4> hpos(ns), hvel(ns);
+    std::vector<int4> himg(ns);
+    for (size_t i = 0; i < ns; i++) {
+        if (i < na) {
+            hpos[i] = make_float4(d->pos[4 * i], d->pos[4 * i + 1], d->pos[4 * i + 2], d->pos[4 * i + 3]);
+            hvel[i] = d->vel ? make_float4(d->vel[3 * i], d->vel[3 * i + 1], d->vel[3 * i + 2], 0.0f)
+                             : make_float4(0, 0, 0, 0);
+            himg[i] = d->image ? make_int4(d->image[3 * i], d->image[3 * i + 1], d->image[3 * i + 2], (int)i)
+                               : make_int4(0, 0, 0, (int)i);
+        } else {
+            hpos[i] = make_float4(0, 0, 0, type_bits(-1));
+            hvel[i] = make_float4(0, 0, 0, 0);
+            himg[i] = make_int4(0, 0, 0, 0x7fffffff);
+        }
+    }
+    std::vector<int> ident(na);
+    for (size_t i = 0; i < na; i++) ident[i] = (int)i;
+    std::vector<float> mass(kMaxTypes, 1.0f);
+    if (d->masses)
+        for (int t = 0; t < d->ntypes; t++) mass[t] = d->masses[t];
+    for (int t = 0; t < d->ntypes; t++)
+        if (!(mass[t] > 0.0f)) {
+            for (void* p : r.allocs) (void)hipFree(p);
+            delete R;
+            return fail(FMD_ERR_ARG, "masses must be > 0");
+        }
+
+    RepState S0;
+    std::memset(&S0, 0, sizeof(S0));
+    S0.step = 0;
+    S0.stop = 0;
+    S0.ramp_n = d->nsteps > 0 ? d->nsteps : 1;
+    S0.active = 0;
+    S0.rebuild = 1;
+    S0.nrebuilds = 1;
+    S0.L = d->L;
+    S0.invL = 1.0f / d->L;
+    {
+        int n = (int)(d->L / D.rcs);
+        int nx = n < 1 ? 1 : n;
+        S0.nx = nx;
+        S0.inv_cs = 1.0f / (d->L / (float)nx);
+        S0.cbits = cbits_for(nx);
+    }
+    S0.kT = d->T_start;
+    S0.vscale = 1.0f;
+    S0.v_pending = 1.0f;
+    S0.exp_vW = 1.0f;
+    S0.fpos = d->dt;
+    S0.T_start = d->T_start; S0.T_stop = d->T_stop;
+    S0.P_start = d->P_start; S0.P_stop = d->P_stop;
+    S0.M = std::min(std::max(1, d->chain_length), kMaxChain);
+    {  // src/integrate/nose_hoover.cu:338-345, in FP64
+        double Nf = 3.0 * (double)d->natoms, kT = d->T_start;
+        S0.Q1 = Nf * kT * (double)d->Tdamp * (double)d->Tdamp;
+        S0.Qr = kT * (double)d->Tdamp * (double)d->Tdamp;
+        S0.W = (Nf + 3.0) * kT * (double)d->Pdamp * (double)d->Pdamp;
+        S0.V0 = (double)d->L * (double)d->L * (double)d->L;
+    }
+    S0.thermo_freq = d->thermo_freq;
+    S0.thermo_cap = S.thermo_cap;
+
+    hipStream_t s = stream_;
+    int32_t rc = FMD_OK;
+    auto up = [&](void* dst, const void* src, size_t n) {
+        if (!rc && n) rc = hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s), "upload");
+    };
+    auto zero = [&](void* dst, size_t n) {
+        if (!rc && n) rc = hip_check(hipMemsetAsync(dst, 0, n, s), "memset");
+    };
+    up(D.pos, hpos.data(), ns * 16);
+    up(D.pos_ref, hpos.data(), ns * 16);
+    up(D.vel, hvel.data(), ns * 16);
+    up(D.img, himg.data(), ns * 16);
+    up(D.slot_of, ident.data(), na * 4);
+    zero(D.force, ns * 16);
+    zero(D.cell_cnt, (size_t)S.ncell_cap * 4);
+    zero(D.nnbr, ns * 4);
+    zero(D.bhdr, ns * 4);
+    zero(D.wpart, nt * kNumPart * 8);
+    up(t_boff, boff.data(), (na + 1) * 4);
+    up(t_blst, blst.data(), blst.size() * 4);
+    up(t_aoff, aoff.data(), (na + 1) * 4);
+    up(t_alst, alst.data(), alst.size() * 16);
+    up(lj, d->lj, (size_t)d->ntypes * d->ntypes * 8);
+    if (d->nbond_types > 0) up(bp, d->bond_params, (size_t)d->nbond_types * 16);
+    if (d->nangle_types > 0) up(ap, d->angle_params, (size_t)d->nangle_types * 8);
+    up(ms, mass.data(), kMaxTypes * 4);
+    up(r.st, &S0, sizeof(S0));
+    up(d_descs_ + id, &D, sizeof(RepDesc));
+    if (!rc) rc = hip_check(hipStreamSynchronize(s), "sync(create)");
+    if (rc) {
+        for (void* p : r.allocs) (void)hipFree(p);
+        delete R;
+        return rc;
+    }
+    reps_[id] = R;
+    in_use_ += r.bytes;
+    // prime: initial build + forces (src/main.cu:311-342)
+    Batch* b = batch_for({id});
+    if (!b) {
+        destroy(id);
+        return FMD_ERR_HIP;
+    }
+    Launch l = launch_of(*b);
+    launch_rebuild(l, 1, 256);
+    launch_force(l, 0, b->ntl, 256, FMD_TERM_ALL, 0, 1);
+    launch_boundary(l, BND_PRIME);
+    rc = hip_check(hipStreamSynchronize(s), "sync(prime)");
+    if (!rc) {
+        RepState st;
+        rc = read_state(r, st);
+        if (!rc && st.err) {
+            int e = st.err;
+            destroy(id);
+            return fail(e & ERR_NBR_OVERFLOW ? FMD_ERR_NBR_OVERFLOW : (e & ERR_STRETCH ? FMD_ERR_STRETCH : FMD_ERR_NAN),
+                        "initial configuration rejected (error bits " + std::to_string(e) + ")");
+        }
+    }
+    if (rc) {
+        destroy(id);
+        return rc;
+    }
+    *sys = id;
+    return FMD_OK;
+}
+
+int32_t Engine::destroy(int32_t sys) {
+    Replica* r = rep(sys);
+    if (!r) return fail(FMD_ERR_ARG, "no such system");
+    (void)hipStreamSynchronize(stream_);
+    drop_batches_with(sys);
+    for (void* p : r->allocs) (void)hipFree(p);
+    in_use_ -= r->bytes;
+    reps_[sys] = nullptr;
+    delete r;
+    return FMD_OK;
+}
+
+void Engine::drop_batches_with(int id) {
+    for (auto it = batches_.begin(); it != batches_.end();) {
+        if (std::find(it->first.begin(), it->first.end(), id) != it->first.end()) {
+            Batch* b = it->second;
+            if (b->graph) (void)hipGraphExecDestroy(b->graph);
+            (void)hipFree(b->d_tiles);
+            (void)hipFree(b->d_reps);
+            delete b;
+            it = batches_.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+Batch* Engine::batch_for(const std::vector<int>& ids_in) {
+    std::vector<int> ids = ids_in;
+    std::sort(ids.begin(), ids.end());
+    ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+    auto it = batches_.find(ids);
+    if (it != batches_.end()) return it->second;
+    for (int id : ids)
+        if (!rep(id)) {
+            fail(FMD_ERR_ARG, "no such system " + std::to_string(id));
+            return nullptr;
+        }
+    Batch* b = new Batch();
+    b->ids = ids;
+    // order replicas by size class so each class is a contiguous tile range
+    std::vector<int> ord = ids;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int c) { return reps_[a]->cls < reps_[c]->cls; });
+    std::vector<Tile> tiles;
+    for (int cl = 0; cl < kNumClasses; cl++) {
+        b->cls_t0[cl] = (int)tiles.size();
+        for (int id : ord)
+            if (reps_[id]->cls == cl)
+                for (int t = 0; t < reps_[id]->ntiles; t++) tiles.push_back(Tile{id, t});
+        b->cls_nt[cl] = (int)tiles.size() - b->cls_t0[cl];
+    }
+    b->ntl = (int)tiles.size();
+    b->nr = (int)ord.size();
+    bool ok = hipMalloc(&b->d_tiles, std::max<size_t>(1, tiles.size()) * sizeof(Tile)) == hipSuccess &&
+              hipMalloc(&b->d_reps, std::max<size_t>(1, ord.size()) * sizeof(int)) == hipSuccess;
+    if (ok)
+        ok = hipMemcpy(b->d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(b->d_reps, ord.data(), ord.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+        (void)hipFree(b->d_tiles);
+        (void)hipFree(b->d_reps);
+        delete b;
+        fail(FMD_ERR_NOMEM, "batch allocation failed");
+        return nullptr;
+    }
+    batches_[ids] = b;
+    return b;
+}
+
+Launch Engine::launch_of(const Batch& b) const {
+    return Launch{d_descs_, b.d_tiles, b.ntl, b.d_reps, b.nr, stream_};
+}
+
+// One MD step for every replica of the batch.
+void Engine::launch_step_seq(Batch& b) {
+    Launch l = launch_of(b);
+    launch_pre(l);
+    launch_decide(l);
+    launch_rebuild(l, 0, 0);
+    for (int c = 0; c < kNumClasses; c++) launch_build(l, b.cls_t0[c], b.cls_nt[c], tuned_[1][c], 0);
+    for (int c = 0; c < kNumClasses; c++)
+        launch_force(l, b.cls_t0[c], b.cls_nt[c], tuned_[0][c], FMD_TERM_ALL, 1, 0);
+    launch_boundary(l, BND_STEP);
+}
+
+int32_t Engine::begin_run(const std::vector<int>& ids, int64_t nsteps) {
+    Batch* b = batch_for(ids);
+    if (!b) return FMD_ERR_ARG;
+    Launch l = launch_of(*b);
+    launch_control(l, CTL_RUN, nsteps);
+    launch_boundary(l, BND_START);
+    return hip_check(hipGetLastError(), "launch(begin_run)");
+}
+
+int32_t Engine::launch_steps(const std::vector<int>& ids, int64_t k, bool use_graph) {
+    Batch* b = batch_for(ids);
+    if (!b) return FMD_ERR_ARG;
+    if (use_graph) {
+        if (!b->graph || b->tune_version != tune_version_) {
+            if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+            hipGraph_t g;
+            int32_t rc = hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "BeginCapture");
+            if (rc) return rc;
+            for (int s = 0; s < kGraphSteps; s++) launch_step_seq(*b);
+            rc = hip_check(hipStreamEndCapture(stream_, &g), "EndCapture");
+            if (rc) return rc;
+            rc = hip_check(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0), "GraphInstantiate");
+            (void)hipGraphDestroy(g);
+            if (rc) return rc;
+            b->graph_steps = kGraphSteps;
+            b->tune_version = tune_version_;
+        }
+        int64_t full = k / b->graph_steps;
+        for (int64_t g = 0; g < full; g++) {
+            int32_t rc = hip_check(hipGraphLaunch(b->graph, stream_), "GraphLaunch");
+            if (rc) return rc;
+        }
+        k -= full * b->graph_steps;
+    }
+    for (int64_t s = 0; s < k; s++) launch_step_seq(*b);
+    return hip_check(hipGetLastError(), "launch(step)");
+}
+
+int32_t Engine::sync() { return hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+
+int32_t Engine::read_state(Replica& r, RepState& s) {
+    return hip_check(hipMemcpy(&s, r.st, sizeof(RepState), hipMemcpyDeviceToHost), "read state");
+}
+
+static int32_t err_code(int e) {
+    if (e & ERR_NAN) return FMD_ERR_NAN;
+    if (e & ERR_NBR_OVERFLOW) return FMD_ERR_NBR_OVERFLOW;
+    if (e & ERR_STRETCH) return FMD_ERR_STRETCH;
+    if (e & ERR_CELL) return FMD_ERR_CELL;
+    if (e & ERR_BONDED) return FMD_ERR_ARG;
+    return FMD_OK;
+}
+
+int32_t Engine::check_errors(const std::vector<int>& ids, std::vector<int>* failed) {
+    int32_t first = FMD_OK;
+    for (int id : ids) {
+        Replica* r = rep(id);
+        if (!r) continue;
+        RepState s;
+        int32_t rc = read_state(*r, s);
+        if (rc) return rc;
+        r->steps_done = s.step;
+        if (s.err) {
+            int32_t c = err_code(s.err);
+            if (failed) failed->push_back(id);
+            if (!first) {
+                first = c;
+                err_ = "system " + std::to_string(id) + " stopped at step " + std::to_string(s.step) +
+                       " (error bits " + std::to_string(s.err) + ")";
+            }
+        }
+    }
+    return first;
+}
+
+int32_t Engine::replica_error(int32_t sys) {
+    Replica* r = rep(sys);
+    if (!r) return FMD_ERR_ARG;
+    RepState s;
+    if (int32_t rc = read_state(*r, s)) return rc;
+    return err_code(s.err);
+}
+
+void Engine::trim_batches(const std::vector<int>& keep_in) {
+    std::vector<int> keep = keep_in;
+    std::sort(keep.begin(), keep.end());
+    keep.erase(std::unique(keep.begin(), keep.end()), keep.end());
+    for (auto it = batches_.begin(); it != batches_.end();) {
+        if (it->first != keep) {
+            Batch* b = it->second;
+            if (b->graph) (void)hipGraphExecDestroy(b->graph);
+            (void)hipFree(b->d_tiles);
+            (void)hipFree(b->d_reps);
+            delete b;
+            it = batches_.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
+int32_t Engine::step(const int32_t* ids, int32_t n, int64_t nsteps, bool use_graph) {
+    if (n <= 0 || !ids) return fail(FMD_ERR_ARG, "empty system list");
+    if (nsteps < 0) return fail(FMD_ERR_ARG, "negative step count");
+    std::vector<int> v(ids, ids + n);
+    for (int id : v)
+        if (!rep(id)) return fail(FMD_ERR_ARG, "no such system " + std::to_string(id));
+    if (nsteps == 0) return FMD_OK;
+    if (int32_t rc = begin_run(v, nsteps)) return rc;
+    if (int32_t rc = launch_steps(v, nsteps, use_graph)) return rc;
+    if (int32_t rc = sync()) return rc;
+    return check_errors(v, nullptr);
+}
+
+int64_t Engine::step_count(int32_t sys) {
+    Replica* r = rep(sys);
+    if (!r) return -1;
+    RepState s;
+    if (read_state(*r, s)) return -1;
+    return s.step;
+}
+int64_t Engine::rebuilds(int32_t sys) {
+    Replica* r = rep(sys);
+    if (!r) return -1;
+    RepState s;
+    if (read_state(*r, s)) return -1;
+    return s.nrebuilds;
+}
+
+int32_t Engine::set_force_cap(int32_t sys, float cap) {
+    Replica* r = rep(sys);
+    if (!r) return fail(FMD_ERR_ARG, "no such system");
+    if (int32_t rc = sync()) return rc;
+    r->desc.fcap = cap > 0.0f ? cap : 0.0f;
+    return hip_check(hipMemcpy(d_descs_ + sys, &r->desc, sizeof(RepDesc), hipMemcpyHostToDevice), "upload desc");
+}
+
+int32_t Engine::set_langevin(int32_t sys, float T0, float T1, float Tdamp) {
+    Replica* r = rep(sys);
+    if (!r || !(Tdamp > 0.0f)) return fail(FMD_ERR_ARG, "bad arguments");
+    if (r->ensemble != FMD_LANGEVIN) return fail(FMD_ERR_STATE, "system is not Langevin");
+    if (int32_t rc = sync()) return rc;
+    r->desc.c1 = std::exp(-(1.0f / Tdamp) * r->desc.dt);
+    r->desc.c2 = std::sqrt(1.0f - r->desc.c1 * r->desc.c1);
+    RepState s;
+    if (int32_t rc = read_state(*r, s)) return rc;
+    s.T_start = T0;
+    s.T_stop = T1;
+    s.pre_ready = 0;  // recompute the current step's target
+    if (int32_t rc = hip_check(hipMemcpy(r->st, &s, sizeof(RepState), hipMemcpyHostToDevice), "upload state")) return rc;
+    return hip_check(hipMemcpy(d_descs_ + sys, &r->desc, sizeof(RepDesc), hipMemcpyHostToDevice), "upload desc");
+}
+
+// ---------------------------------------------------------------------------
+// parity entry points
+// ---------------------------------------------------------------------------
+int32_t Engine::build_neighbors(int32_t sys) {
+    Replica* r = rep(sys);
+    if (!r) return fail(FMD_ERR_ARG, "no such system");
+    Batch* b = batch_for({sys});
+    if (!b) return FMD_ERR_ARG;
+    Launch l = launch_of(*b);
+    launch_control(l, CTL_FORCE_REBUILD, 0);
+    launch_rebuild(l, 1, 256);
+    // slots were permuted: recompute the forces in the new order
+    launch_force(l, 0, b->ntl, 256, FMD_TERM_ALL, 0, 1);
+    launch_boundary(l, BND_REDUCE);
+    if (int32_t rc = sync()) return rc;
+    RepState s;
+    if (int32_t rc = read_state(*r, s)) return rc;
+    if (s.err) return fail(err_code(s.err), "neighbor build failed (error bits " + std::to_string(s.err) + ")");
+    return FMD_OK;
+}
+
+int32_t Engine::get_pairs(int32_t sys, int64_t* npairs, int32_t* ij, int64_t cap) {
+    Replica* r = rep(sys);
+    if (!r || !npairs) return fail(FMD_ERR_ARG, "bad arguments");
+    const RepDesc& D = r->desc;
+    std::vector<int> nbr((size_t)D.ntiles * D.kcap * 64), nn(D.nslots);
+    std::vector<int4> img(D.nslots);
+    int32_t rc = hip_check(hipMemcpy(nbr.data(), D.nbr, nbr.size() * 4, hipMemcpyDeviceToHost), "copy nbr");
+    if (!rc) rc = hip_check(hipMemcpy(nn.data(), D.nnbr, nn.size() * 4, hipMemcpyDeviceToHost), "copy nnbr");
+    if (!rc) rc = hip_check(hipMemcpy(img.data(), D.img, img.size() * 16, hipMemcpyDeviceToHost), "copy img");
+    if (rc) return rc;
+    std::vector<std::pair<int, int>> pairs;
+    for (int i = 0; i < D.natoms; i++) {
+        int lt = i / 64, lane = i % 64;
+        for (int k = 0; k < nn[i]; k++) {
+            int j = nbr[((size_t)lt * D.kcap + k) * 64 + lane];
+            if (j < 0 || j >= D.natoms) return fail(FMD_ERR_STATE, "corrupt neighbor entry");
+            pairs.emplace_back(img[i].w, img[j].w);
+        }
+    }
+    std::sort(pairs.begin(), pairs.end());
+    *npairs = (int64_t)pairs.size();
+    if (ij)
+        for (int64_t p = 0; p < (int64_t)pairs.size() && p < cap; p++) {
+            ij[2 * p] = pairs[p].first;
+            ij[2 * p + 1] = pairs[p].second;
+        }
+    return FMD_OK;
+}
+
+int32_t Engine::get_cell_ids(int32_t sys, int32_t* out, int32_t* nx) {
+    Replica* r = rep(sys);
+    if (!r || !out) return fail(FMD_ERR_ARG, "bad arguments");
+    int* dout = nullptr;
+    int32_t rc = hip_check(hipMalloc(&dout, (size_t)r->natoms * 4), "hipMalloc");
+    if (rc) return rc;
+    launch_ref_cells(d_descs_ + sys, r->natoms, dout, stream_);
+    rc = sync();
+    if (!rc) rc = hip_check(hipMemcpy(out, dout, (size_t)r->natoms * 4, hipMemcpyDeviceToHost), "copy");
+    (void)hipFree(dout);
+    if (!rc && nx) {
+        RepState s;
+        rc = read_state(*r, s);
+        int n = (int)(s.L / r->desc.rcs);
+        *nx = n < 1 ? 1 : n;
+    }
+    return rc;
+}
+
+int32_t Engine::get_morton(int32_t sys, uint32_t* out) {
+    Replica* r = rep(sys);
+    if (!r || !out) return fail(FMD_ERR_ARG, "bad arguments");
+    uint32_t* dout = nullptr;
+    int32_t rc = hip_check(hipMalloc(&dout, (size_t)r->natoms * 4), "hipMalloc");
+    if (rc) return rc;
+    launch_morton_keys(d_descs_ + sys, r->natoms, dout, stream_);
+    rc = sync();
+    if (!rc) rc = hip_check(hipMemcpy(out, dout, (size_t)r->natoms * 4, hipMemcpyDeviceToHost), "copy");
+    (void)hipFree(dout);
+    return rc;
+}
+
+int32_t Engine::compute_forces(int32_t sys, uint32_t terms, float* f4, double* vir, double* pe) {
+    Replica* r = rep(sys);
+    if (!r) return fail(FMD_ERR_ARG, "no such system");
+    Batch* b = batch_for({sys});
+    if (!b) return FMD_ERR_ARG;
+    Launch l = launch_of(*b);
+    launch_force(l, 0, b->ntl, 256, (int)(terms & 7u), 0, 1);
+    launch_boundary(l, BND_REDUCE);
+    int32_t rc = sync();
+    if (rc) return rc;
+    RepState s;
+    if ((rc = read_state(*r, s))) return rc;
+    if (f4) {
+        const RepDesc& D = r->desc;
+        std::vector<float4> f(D.nslots);
+        std::vector<int4> img(D.nslots);
+        rc = hip_check(hipMemcpy(f.data(), D.force, f.size() * 16, hipMemcpyDeviceToHost), "copy force");
+        if (!rc) rc = hip_check(hipMemcpy(img.data(), D.img, img.size() * 16, hipMemcpyDeviceToHost), "copy img");
+        if (rc) return rc;
+        for (int i = 0; i < D.natoms; i++) {
+            int g = img[i].w;
+            f4[4 * g] = f[i].x; f4[4 * g + 1] = f[i].y; f4[4 * g + 2] = f[i].z; f4[4 * g + 3] = f[i].w;
+        }
+    }
+    if (vir)
+        for (int c = 0; c < 6; c++) vir[c] = s.vir[c];
+    if (pe) *pe = s.pe;
+    if ((terms & 7u) != 7u) {  // restore the full forces the next step's kick uses
+        launch_force(l, 0, b->ntl, 256, FMD_TERM_ALL, 0, 1);
+        launch_boundary(l, BND_REDUCE);
+        rc = sync();
+    }
+    if (!rc && (s.err & ERR_STRETCH)) return fail(FMD_ERR_STRETCH, "bond stretched beyond R0");
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// observation
+// ---------------------------------------------------------------------------
+int32_t Engine::get_state(int32_t sys, float* pos4, float* vel3, int32_t* img3) {
+    Replica* r = rep(sys);
+    if (!r) return fail(FMD_ERR_ARG, "no such system");
+    const RepDesc& D = r->desc;
+    std::vector<float4> p(D.nslots), v(D.nslots);
+    std::vector<int4> img(D.nslots);
+    RepState s;
+    int32_t rc = sync();
+    if (!rc) rc = hip_check(hipMemcpy(p.data(), D.pos, p.size() * 16, hipMemcpyDeviceToHost), "copy pos");
+    if (!rc) rc = hip_check(hipMemcpy(v.data(), D.vel, v.size() * 16, hipMemcpyDeviceToHost), "copy vel");
+    if (!rc) rc = hip_check(hipMemcpy(img.data(), D.img, img.size() * 16, hipMemcpyDeviceToHost), "copy img");
+    if (!rc) rc = read_state(*r, s);
+    if (rc) return rc;
+    const float vs = s.v_pending;  // post-force scale not yet applied to vel[]
+    for (int i = 0; i < D.natoms; i++) {
+        int g = img[i].w;
+        if (pos4) { pos4[4 * g] = p[i].x; pos4[4 * g + 1] = p[i].y; pos4[4 * g + 2] = p[i].z; pos4[4 * g + 3] = p[i].w; }
+        if (vel3) { vel3[3 * g] = v[i].x * vs; vel3[3 * g + 1] = v[i].y * vs; vel3[3 * g + 2] = v[i].z * vs; }
+        if (img3) { img3[3 * g] = img[i].x; img3[3 * g + 1] = img[i].y; img3[3 * g + 2] = img[i].z; }
+    }
+    return FMD_OK;
+}
+
+int32_t Engine::get_box(int32_t sys, double* L) {
+    Replica* r = rep(sys);
+    if (!r || !L) return fail(FMD_ERR_ARG, "bad arguments");
+    RepState s;
+    int32_t rc = read_state(*r, s);
+    if (!rc) *L = s.L;
+    return rc;
+}
+
+static void thermo_of(const RepState& s, int natoms, fmd_thermo* o) {
+    const double N = natoms, L = s.L, V = L * L * L;
+    o->step = s.rec_step;
+    o->ke = 0.5 * (s.kin[0] + s.kin[3] + s.kin[5]) / N;
+    o->pe = s.pe / N;
+    o->temp = 2.0 * o->ke / 3.0;
+    for (int c = 0; c < 6; c++) o->press[c] = (s.kin[c] + s.vir[c]) / V;
+    o->L = L;
+}
+
+int32_t Engine::get_thermo(int32_t sys, fmd_thermo* out) {
+    Replica* r = rep(sys);
+    if (!r || !out) return fail(FMD_ERR_ARG, "bad arguments");
+    RepState s;
+    int32_t rc = read_state(*r, s);
+    if (!rc) thermo_of(s, r->natoms, out);
+    return rc;
+}
+
+int32_t Engine::drain_thermo(int32_t sys, fmd_thermo* out, int32_t cap, int32_t* n, int64_t* lost) {
+    Replica* r = rep(sys);
+    if (!r || !n) return fail(FMD_ERR_ARG, "bad arguments");
+    *n = 0;
+    if (lost) *lost = 0;
+    if (r->thermo_cap == 0) return FMD_OK;
+    RepState s;
+    int32_t rc = read_state(*r, s);
+    if (rc) return rc;
+    int64_t from = r->thermo_read, to = s.thermo_written;
+    if (to - from > r->thermo_cap) {
+        if (lost) *lost = to - from - r->thermo_cap;
+        from = to - r->thermo_cap;
+    }
+    if (to - from > cap) to = from + cap;
+    if (to <= from) return FMD_OK;
+    std::vector<ThermoRec> ring(r->thermo_cap);
+    rc = hip_check(hipMemcpy(ring.data(), r->desc.thermo, ring.size() * sizeof(ThermoRec), hipMemcpyDeviceToHost), "copy thermo");
+    if (rc) return rc;
+    for (int64_t k = from; k < to; k++) {
+        const ThermoRec& t = ring[k % r->thermo_cap];
+        fmd_thermo& o = out[*n];
+        o.step = t.step; o.ke = t.ke; o.pe = t.pe; o.temp = t.temp;
+        for (int c = 0; c < 6; c++) o.press[c] = t.press[c];
+        o.L = t.L;
+        (*n)++;
+    }
+    r->thermo_read = to;
+    return FMD_OK;
+}
+
+int32_t Engine::time_force(const int32_t* ids, int32_t n, uint32_t terms, int reps, double* ms,
+                           double* bytes, int64_t* pairs) {
+    if (!ids || n <= 0 || reps < 1) return fail(FMD_ERR_ARG, "bad arguments");
+    std::vector<int> v(ids, ids + n);
+    Batch* b = batch_for(v);
+    if (!b) return FMD_ERR_ARG;
+    Launch l = launch_of(*b);
+    // algorithmic bytes from the current lists and bonded tables
+    double B = 0.0;
+    int64_t P = 0;
+    for (int id : b->ids) {
+        Replica* r = rep(id);
+        std::vector<int> nn(r->nslots), hdr(r->nslots);
+        int32_t rc = hip_check(hipMemcpy(nn.data(), r->desc.nnbr, nn.size() * 4, hipMemcpyDeviceToHost), "copy");
+        if (!rc) rc = hip_check(hipMemcpy(hdr.data(), r->desc.bhdr, hdr.size() * 4, hipMemcpyDeviceToHost), "copy");
+        if (rc) return rc;
+        int64_t p = 0, nb = 0, na = 0;
+        for (int i = 0; i < r->natoms; i++) { p += nn[i]; nb += hdr[i] & 0xFF; na += (hdr[i] >> 8) & 0xFF; }
+        P += p;
+        if (terms & 1) B += 36.0 * r->natoms + 4.0 * (double)p;
+        if (terms & 6) B += 4.0 * r->natoms + ((terms & 1) ? 0.0 : 32.0 * r->natoms);
+        if (terms & 2) B += 4.0 * (double)nb;
+        if (terms & 4) B += 8.0 * (double)na;
+    }
+    hipEvent_t e0, e1;
+    if (int32_t rc = hip_check(hipEventCreate(&e0), "event")) return rc;
+    if (int32_t rc = hip_check(hipEventCreate(&e1), "event")) return rc;
+    auto run = [&]() {
+        for (int c = 0; c < kNumClasses; c++)
+            launch_force(l, b->cls_t0[c], b->cls_nt[c], tuned_[0][c], (int)(terms & 7u), 0, 1);
+    };
+    run();
+    (void)hipEventRecord(e0, stream_);
+    for (int k = 0; k < reps; k++) run();
+    (void)hipEventRecord(e1, stream_);
+    int32_t rc = hip_check(hipEventSynchronize(e1), "event sync");
+    float t = 0.0f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    // restore the full forces the next step's kick reads
+    launch_force(l, 0, b->ntl, 256, FMD_TERM_ALL, 0, 1);
+    if ((rc = sync())) return rc;
+    if (ms) *ms = (double)t / reps;
+    if (bytes) *bytes = B;
+    if (pairs) *pairs = P;
+    return FMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner: per size class, time the force and list-build kernels over the
+// class's tiles for each candidate block shape and keep the fastest.
+// ---------------------------------------------------------------------------
+int32_t Engine::autotune(const int32_t* ids, int32_t n, int reps) {
+    if (!ids || n <= 0) return fail(FMD_ERR_ARG, "empty system list");
+    std::vector<int> v(ids, ids + n);
+    Batch* b = batch_for(v);
+    if (!b) return FMD_ERR_ARG;
+    Launch l = launch_of(*b);
+    if (reps < 1) reps = 1;
+    const int cand[4] = {64, 128, 256, 512};
+    hipEvent_t e0, e1;
+    if (int32_t rc = hip_check(hipEventCreate(&e0), "event")) return rc;
+    if (int32_t rc = hip_check(hipEventCreate(&e1), "event")) return rc;
+    int32_t rc = FMD_OK;
+    for (int c = 0; c < kNumClasses && !rc; c++) {
+        if (b->cls_nt[c] == 0) continue;
+        for (int kern = 0; kern < 2 && !rc; kern++) {
+            float best = 1e30f;
+            int bestb = 256;
+            for (int bi = 0; bi < 4 && !rc; bi++) {
+                auto run = [&]() {
+                    if (kern == 0) launch_force(l, b->cls_t0[c], b->cls_nt[c], cand[bi], FMD_TERM_ALL, 0, 1);
+                    else launch_build(l, b->cls_t0[c], b->cls_nt[c], cand[bi], 1);
+                };
+                run();  // warm
+                (void)hipEventRecord(e0, stream_);
+                for (int k = 0; k < reps; k++) run();
+                (void)hipEventRecord(e1, stream_);
+                rc = hip_check(hipEventSynchronize(e1), "autotune sync");
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (ms < best) { best = ms; bestb = cand[bi]; }
+            }
+            tuned_[kern][c] = bestb;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    tune_version_++;
+    return rc;
+}
+
+}  // namespace fmd

@@ -1,0 +1,210 @@
+// fmd_device.h -- data layout shared by the host engine and the gfx950 kernels.
+//
+// Every replica (one simulation system) owns its own device buffers.  A launch
+// covers many replicas through two small device tables:
+//   RepDesc[r] : pointers and constants of replica r (written at creation)
+//   Tile[t]    : (replica, tile) for each 64-slot wave tile in the launch
+// A tile never straddles replicas (slot counts are padded to 64), so every
+// per-wave reduction belongs to exactly one replica and results do not depend
+// on which other replicas share the launch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace fmd {
+
+constexpr int kTile = 64;          // slots per tile = one wave64
+constexpr int kMaxTypes = 16;
+constexpr int kMaxChain = 10;
+constexpr int kNumPart = 16;       // FP64 partials per wave: pe, W[6], K[6], pad
+constexpr int kRebuildEvery = 20;  // forced rebuild period (src/main.cu:401)
+constexpr int kSlotBits = 26;      // slot / gid field of packed bonded entries
+constexpr int kSlotMask = (1 << kSlotBits) - 1;
+
+enum ErrBits : int {
+    ERR_NBR_OVERFLOW = 1,
+    ERR_STRETCH = 2,
+    ERR_NAN = 4,
+    ERR_CELL = 8,
+    ERR_BONDED = 16,
+};
+
+struct Tile {
+    int rep;   // replica index into the RepDesc table
+    int lt;    // tile index within the replica
+};
+
+// Thermo record written by the device every thermo_freq steps.
+struct ThermoRec {
+    int64_t step;
+    double ke, pe, temp;
+    double press[6];
+    double L;
+};
+
+// Mutable per-replica scalar state.  Written by per-replica kernels (one lane),
+// read by the per-atom kernels of the next phase.
+struct RepState {
+    // step control
+    int64_t step;          // steps completed
+    int64_t stop;          // replica runs while step < stop
+    int64_t ramp_n;        // denominator of the T/P ramp (run.conf nsteps)
+    int active;            // 1 while stepping
+    int rebuild;           // rebuild decision of the current step
+    int since_rebuild;
+    int maxdisp_bits;      // max |r - r_ref|^2 since the last rebuild (float bits)
+    int err;               // ErrBits
+    int pre_ready;         // start-of-step scalars for step+1 are computed
+    int64_t nrebuilds;
+    // current box and cell grid (float copies for per-atom kernels)
+    float L, invL;
+    int nx;                // cells per axis
+    int cbits;             // Morton bits per axis of the cell key
+    float inv_cs;
+    // per-step factors for the pre-force kernel
+    float kT;              // Langevin target of the current step
+    float vscale;          // velocity scale owed before the kick (NH / NPT)
+    float exp_vW, fpos;    // NPT drift: r = r*exp_vW + v*fpos, pos_ref *= exp_vW
+    float v_pending;       // post-force scale already owed (applied on readback)
+    // FP64 thermostat / barostat state
+    double T_start, T_stop, P_start, P_stop;
+    double xi[kMaxChain], vxi[kMaxChain];
+    int M;
+    double Q1, Qr, W, V0, eps, veps;
+    double ke_carry;
+    // last reductions (after the post-force kick and scaling)
+    double pe, vir[6], kin[6];
+    int64_t rec_step;      // step of the last reduction
+    // thermo ring
+    int thermo_freq;
+    int thermo_cap;
+    int64_t thermo_written;  // total records written
+};
+
+struct RepDesc {
+    int natoms, nslots, ntiles, ntypes;
+    int kcap;              // neighbor rows per tile
+    int bcap, acap;        // bonded rows per tile
+    int ncell_cap;         // capacity of the Morton cell-key arrays
+    int ensemble;
+    float rc2, rcs, rcs2, half_skin2;
+    float dt, half_dt;
+    float c1, c2;          // Langevin: exp(-dt/Tdamp), sqrt(1-c1^2)
+    float fcap;            // LJ pair force cap (0 = off)
+    int clamp_lj;          // force/r clamp to +-1000 (src/force/lj.cu:48)
+    uint32_t seed_lo, seed_hi;
+    // per-slot state (slot = position in the cell-sorted order)
+    float4* pos;
+    float4* vel;
+    float4* force;
+    float4* pos_ref;
+    int4* img;             // image counts x, y, z and the global id in w
+    float4* tmp_pos;
+    float4* tmp_vel;
+    int4* tmp_img;
+    int* cell_of;
+    int* rank;
+    int* order;
+    int* order2;
+    int* slot_of;          // [natoms] global id -> slot
+    int* cell_cnt;         // [ncell_cap]
+    int* cell_start;       // [ncell_cap + 1]
+    int* nnbr;             // [nslots]
+    int* nbr;              // [ntiles][kcap][64]
+    int* bhdr;             // [nslots] nbonds | nangles << 8
+    int* bnd;              // [ntiles][bcap][64] partner slot | type << 26
+    int2* ang;             // [ntiles][acap][64] (slot | role << 26, slot | type << 26)
+    // topology by global id (static)
+    const int* t_boff;     // [natoms + 1]
+    const int* t_blst;     // partner gid | type << 26
+    const int* t_aoff;     // [natoms + 1]
+    const int4* t_alst;    // angle (i, j, k, type) incident on the gid
+    // parameters
+    const float2* lj;      // [ntypes^2] (eps, sig)
+    const float4* bparm;   // per bond type (k, R0, eps, sig)
+    const float2* aparm;   // per angle type (k, theta0)
+    const float* mass;     // per type
+    // reductions and records
+    double* wpart;         // [ntiles][kNumPart]
+    RepState* st;
+    ThermoRec* thermo;     // [thermo_cap]
+};
+
+// ---------------------------------------------------------------------------
+// Geometry.  The pair-selection arithmetic is evaluated without contraction so
+// it is bit-identical to the reference formulas on the CPU.
+// ---------------------------------------------------------------------------
+#pragma clang fp contract(off)
+
+__host__ __device__ inline int type_of(float w) { return __builtin_bit_cast(int, w); }
+__host__ __device__ inline float type_bits(int t) { return __builtin_bit_cast(float, t); }
+
+// min_image of src/core/pbc.cuh:4-6
+__device__ inline float min_image_ref(float dr, float L, float invL) {
+    return dr - L * rintf(dr * invL);
+}
+__device__ inline float dist2_ref(float4 a, float4 b, float L, float invL) {
+    float dx = min_image_ref(a.x - b.x, L, invL);
+    float dy = min_image_ref(a.y - b.y, L, invL);
+    float dz = min_image_ref(a.z - b.z, L, invL);
+    return dx * dx + dy * dy + dz * dz;
+}
+// Force-path displacement: exact difference a-b (TwoSum) folded into the
+// minimum image and rounded once, so the FP32 displacement carries no error
+// from positions near opposite box faces.
+__device__ inline float min_image_exact(float a, float b, float L, float invL) {
+    float s = a - b;
+    float bb = s - a;
+    float err = (a - (s - bb)) + (-b - bb);
+    float m = rintf(s * invL);
+    return (s - L * m) + err;
+}
+
+// Morton interleave of 10 bits (src/core/morton.cu:4-10)
+__host__ __device__ inline uint32_t expand_bits(uint32_t v) {
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__host__ __device__ inline uint32_t cell_key(int cx, int cy, int cz) {
+    return (expand_bits((uint32_t)cz) << 2) | (expand_bits((uint32_t)cy) << 1) |
+           expand_bits((uint32_t)cx);
+}
+// Cell coordinate of one axis (src/neighbor/verlet_list.cu:67-72)
+__device__ inline int cell_coord(float x, float inv_cs, int nx) {
+    int c = (int)floorf(x * inv_cs);
+    return ((c % nx) + nx) % nx;
+}
+
+#pragma clang fp contract(on)
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11) and Box-Muller; bit-identical to the
+// host restatement for the integer part.
+// ---------------------------------------------------------------------------
+__device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        if (r > 0) { k0 += W0; k1 += W1; }
+        uint32_t hi0 = __umulhi(M0, c[0]), lo0 = M0 * c[0];
+        uint32_t hi1 = __umulhi(M1, c[2]), lo1 = M1 * c[2];
+        uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+    }
+}
+__device__ inline float u01(uint32_t x) { return ((float)(x >> 8) + 0.5f) * (1.0f / 16777216.0f); }
+__device__ inline float3 normal3(uint32_t seed_lo, uint32_t seed_hi, uint32_t gid, int64_t step) {
+    uint32_t c[4] = {gid, (uint32_t)step, (uint32_t)((uint64_t)step >> 32), 0x4C414E47u};
+    philox4x32_10(c, seed_lo, seed_hi);
+    const float twopi = 6.28318530717958647692f;
+    float a = sqrtf(-2.0f * logf(u01(c[0])));
+    float b = sqrtf(-2.0f * logf(u01(c[2])));
+    float s1, c1;
+    sincosf(twopi * u01(c[1]), &s1, &c1);
+    return make_float3(a * c1, a * s1, b * cosf(twopi * u01(c[3])));
+}
+
+}  // namespace fmd

@@ -1,0 +1,35 @@
+// kernels.h -- host-side launchers of the gfx950 kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fmd_device.h"
+
+namespace fmd {
+
+// Launch geometry shared by the launchers.
+struct Launch {
+    const RepDesc* descs;  // device table
+    const Tile* tiles;     // device tile list
+    int ntl;               // tiles in the list
+    const int* reps;       // device list of replica indices
+    int nr;                // replicas in the list
+    hipStream_t stream;
+};
+
+enum BoundaryMode { BND_PRIME = 0, BND_STEP = 1, BND_START = 2, BND_REDUCE = 3 };
+enum ControlOp { CTL_RUN = 0, CTL_FORCE_REBUILD = 1 };
+
+void launch_pre(const Launch& l);
+void launch_decide(const Launch& l);
+// The six rebuild phases; `always` ignores the active/rebuild flags.
+void launch_rebuild(const Launch& l, int always, int build_block);
+// Force kernel over tiles [t0, t0+nt) of the list with the given block size.
+void launch_force(const Launch& l, int t0, int nt, int block, int terms, int kick, int always);
+void launch_build(const Launch& l, int t0, int nt, int block, int always);
+void launch_boundary(const Launch& l, int mode);
+void launch_control(const Launch& l, int op, int64_t arg);
+// Reference cell ids / Morton keys by global id for one replica.
+void launch_ref_cells(const RepDesc* d, int natoms, int* out, hipStream_t s);
+void launch_morton_keys(const RepDesc* d, int natoms, uint32_t* out, hipStream_t s);
+
+}  // namespace fmd

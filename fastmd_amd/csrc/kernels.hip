@@ -1,0 +1,855 @@
+// kernels.hip -- gfx950 kernels of the bead-spring MD step.
+//
+// Per-atom kernels run one wave64 per 64-slot tile.  Per-replica kernels run
+// one workgroup (or lane) per replica.  Every kernel reads the replica's
+// RepState to decide whether it has work (active / rebuild flags), so a fixed
+// launch sequence can be captured once into a hipGraph and replayed while
+// replicas rebuild, stop or fail independently.
+//
+// Step sequence (src/main.cu:403-630, one launch each, all replicas at once):
+//   pre       kick + drift (+ Langevin O step, NH/NPT scaling), wrap, skin max
+//   decide    per replica: rebuild if max displacement > (skin/2)^2 or every 20
+//   bin/scan/scatter/rank/gather   Morton-ordered cell sort (rebuilding only)
+//   build     Verlet list + bonded slot tables (rebuilding only)
+//   force     LJ + FENE/WCA + angle, post-force kick, FP64 wave partials
+//   boundary  per replica: FP64 reductions, NH chain / barostat, thermo record
+#include "kernels.h"
+
+namespace fmd {
+
+// ---------------------------------------------------------------------------
+// wave64 helpers
+// ---------------------------------------------------------------------------
+__device__ inline int wave_in_grid() {
+    return __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+}
+__device__ inline int waves_in_grid() { return gridDim.x * (blockDim.x >> 6); }
+__device__ inline int lane_id() { return threadIdx.x & 63; }
+
+__device__ inline float wave_max(float x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o, 64));
+    return x;
+}
+__device__ inline double wave_sum(double x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+__device__ inline int wave_or(int x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o, 64);
+    return x;
+}
+
+#pragma clang fp contract(off)
+__device__ inline float sq3(float x, float y, float z) { return x * x + y * y + z * z; }
+#pragma clang fp contract(fast)
+
+// ---------------------------------------------------------------------------
+// FP64 Nose-Hoover chain (src/integrate/nose_hoover.cu:86-178 / :356-447) and
+// NPT barostat (src/integrate/nose_hoover.cu:185-305), one lane per replica.
+// ---------------------------------------------------------------------------
+__device__ double nh_chain(RepState* st, double KE, double kT, int natoms, double half_dt) {
+    const int M = st->M;
+    double xi[kMaxChain], v[kMaxChain], dd[kMaxChain];
+    for (int k = 0; k < kMaxChain; k++) { xi[k] = st->xi[k]; v[k] = st->vxi[k]; dd[k] = 0.0; }
+    const double Nf = 3.0 * (double)natoms, Q1 = st->Q1, Qr = st->Qr;
+    const double dt2 = half_dt, dt4 = 0.5 * half_dt, dt8 = 0.25 * half_dt;
+    double T = 2.0 * KE / Nf;
+    if (Q1 > 0.0) dd[0] = (Nf * T - Nf * kT) / Q1;
+    for (int ich = M - 1; ich > 0; ich--) {
+        double e = (ich + 1 < M) ? exp(-dt8 * v[ich + 1]) : 1.0;
+        v[ich] = (v[ich] * e + dd[ich] * dt4) * e;
+    }
+    double e0 = (M > 1) ? exp(-dt8 * v[1]) : 1.0;
+    v[0] = (v[0] * e0 + dd[0] * dt4) * e0;
+    double scale = exp(-dt2 * v[0]);
+    T *= scale * scale;
+    if (Q1 > 0.0) dd[0] = (Nf * T - Nf * kT) / Q1;
+    for (int k = 0; k < M; k++) xi[k] += dt2 * v[k];
+    v[0] = (v[0] * e0 + dd[0] * dt4) * e0;
+    for (int ich = 1; ich < M; ich++) {
+        double e = (ich + 1 < M) ? exp(-dt8 * v[ich + 1]) : 1.0;
+        double Qp = (ich == 1) ? Q1 : Qr;
+        dd[ich] = (Qp * v[ich - 1] * v[ich - 1] - kT) / Qr;
+        v[ich] = (v[ich] * e + dd[ich] * dt4) * e;
+    }
+    for (int k = 0; k < kMaxChain; k++) { st->xi[k] = xi[k]; st->vxi[k] = v[k]; }
+    return scale;
+}
+
+struct BaroOut {
+    double L, baro_scale, exp_vW, veps_W_dt, nh_scale, KE_scaled;
+};
+__device__ BaroOut npt_chain_baro(RepState* st, double KE, double vir_trace, bool advance,
+                                  double kT, double Pt, int natoms, double half_dt, double dt) {
+    BaroOut o;
+    const double Nf = 3.0 * (double)natoms;
+    double V = st->V0 * exp(3.0 * st->eps);
+    if (advance) {
+        double P = (2.0 * KE + vir_trace) / (3.0 * V);
+        double cb = cbrt(2.0);
+        double w[3] = {1.0 / (2.0 - cb), -cb / (2.0 - cb), 1.0 / (2.0 - cb)};
+        for (int s = 0; s < 3; s++) {
+            double ws = w[s] * dt;
+            double dv = 3.0 * V * (P - Pt) * ws;
+            double vh = st->veps + 0.5 * dv;
+            st->eps += vh * ws / st->W;
+            dv = 3.0 * st->V0 * exp(3.0 * st->eps) * (P - Pt) * ws;
+            st->veps += dv;
+        }
+        V = st->V0 * exp(3.0 * st->eps);
+    }
+    o.L = cbrt(V);
+    double vW = st->veps / st->W;
+    o.baro_scale = exp(-(1.0 + 3.0 / Nf) * vW * half_dt);
+    o.exp_vW = exp(vW * dt);
+    o.veps_W_dt = vW * dt;
+    o.KE_scaled = KE * o.baro_scale * o.baro_scale;
+    o.nh_scale = nh_chain(st, o.KE_scaled, kT, natoms, half_dt);
+    return o;
+}
+
+__device__ inline double sinch(double x) {
+    if (fabs(x) < 1e-6) return 1.0 + x * x / 6.0;
+    return sinh(x) / x;
+}
+
+// Cell grid of the current box (src/neighbor/verlet_list.cu:8-13), recomputed
+// at every rebuild so the grid follows the box under NPT.
+__device__ void set_grid(const RepDesc& d, RepState* st) {
+    int n = (int)(st->L / d.rcs);
+    int nx = n < 1 ? 1 : n;
+    float cs = st->L / (float)nx;
+    int cb = 0;
+    while ((1 << cb) < nx) cb++;
+    if ((1 << (3 * cb)) > d.ncell_cap) {
+        st->err |= ERR_CELL;
+        st->active = 0;
+        return;
+    }
+    st->nx = nx;
+    st->inv_cs = 1.0f / cs;
+    st->cbits = cb;
+}
+
+// Start-of-step scalars for step s (T/P ramp, NH chain, barostat factors).
+__device__ void start_step(const RepDesc& d, RepState* st, int64_t s) {
+    float frac = (float)(s - 1) / (float)st->ramp_n;  // src/main.cu:407-409
+    float T = (float)st->T_start + ((float)st->T_stop - (float)st->T_start) * frac;
+    float P = (float)st->P_start + ((float)st->P_stop - (float)st->P_start) * frac;
+    const double hdt = 0.5 * (double)d.dt, dt = (double)d.dt;
+    if (d.ensemble == 0) {
+        st->kT = T;
+        st->vscale = 1.0f;
+    } else if (d.ensemble == 1) {
+        double nh = nh_chain(st, st->ke_carry, (double)T, d.natoms, hdt);
+        st->vscale = (float)((double)st->v_pending * nh);
+    } else {
+        BaroOut o = npt_chain_baro(st, st->ke_carry, 0.0, false, (double)T, (double)P, d.natoms,
+                                   hdt, dt);
+        st->vscale = (float)((double)st->v_pending * o.nh_scale * o.baro_scale);
+        st->exp_vW = (float)o.exp_vW;
+        double h = 0.5 * o.veps_W_dt;
+        st->fpos = (float)(dt * exp(h) * sinch(h));
+        st->L = (float)o.L;
+        st->invL = 1.0f / st->L;
+    }
+    st->pre_ready = 1;
+}
+
+// ---------------------------------------------------------------------------
+// pre-force: kick, drift, (O), wrap, image, skin displacement
+// (src/integrate/langevin.cu:34-94, src/integrate/nose_hoover.cu:606-757,
+//  src/neighbor/skin_trigger.cuh:5-16)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_pre(const RepDesc* __restrict__ descs,
+                                             const Tile* __restrict__ tiles, int ntl) {
+    const int w = wave_in_grid();
+    if (w >= ntl) return;
+    const Tile t = tiles[w];
+    const RepDesc& d = descs[t.rep];
+    RepState* st = d.st;
+    if (!st->active) return;
+    const int i = t.lt * kTile + lane_id();
+    float d2 = 0.0f;
+    int bad = 0;
+    if (i < d.natoms) {
+        float4 r = d.pos[i], v = d.vel[i];
+        const float4 f = d.force[i];
+        int4 im = d.img[i];
+        float4 ref = d.pos_ref[i];
+        const float m = d.mass[type_of(r.w)];
+        const float hdtm = d.half_dt / m;
+        const float L = st->L, invL = st->invL;
+        if (d.ensemble == 0) {
+            v.x += hdtm * f.x; v.y += hdtm * f.y; v.z += hdtm * f.z;
+            r.x += d.half_dt * v.x; r.y += d.half_dt * v.y; r.z += d.half_dt * v.z;
+            const float3 z = normal3(d.seed_lo, d.seed_hi, (uint32_t)im.w, st->step + 1);
+            const float ns = d.c2 * sqrtf(st->kT / m);
+            v.x = d.c1 * v.x + ns * z.x; v.y = d.c1 * v.y + ns * z.y; v.z = d.c1 * v.z + ns * z.z;
+            r.x += d.half_dt * v.x; r.y += d.half_dt * v.y; r.z += d.half_dt * v.z;
+        } else {
+            const float sc = st->vscale;
+            v.x *= sc; v.y *= sc; v.z *= sc;
+            v.x += hdtm * f.x; v.y += hdtm * f.y; v.z += hdtm * f.z;
+            if (d.ensemble == 1) {
+                r.x += d.dt * v.x; r.y += d.dt * v.y; r.z += d.dt * v.z;
+            } else {
+                const float ew = st->exp_vW, fp = st->fpos;
+                r.x = r.x * ew + v.x * fp; r.y = r.y * ew + v.y * fp; r.z = r.z * ew + v.z * fp;
+                ref.x *= ew; ref.y *= ew; ref.z *= ew;
+                d.pos_ref[i] = ref;
+            }
+        }
+        const int ix = (int)floorf(r.x * invL), iy = (int)floorf(r.y * invL),
+                  iz = (int)floorf(r.z * invL);
+        r.x -= (float)ix * L; r.y -= (float)iy * L; r.z -= (float)iz * L;
+        im.x += ix; im.y += iy; im.z += iz;
+        float dx = r.x - ref.x, dy = r.y - ref.y, dz = r.z - ref.z;
+        dx -= L * roundf(dx * invL); dy -= L * roundf(dy * invL); dz -= L * roundf(dz * invL);
+        d2 = dx * dx + dy * dy + dz * dz;
+        bad = !(isfinite(r.x) && isfinite(r.y) && isfinite(r.z) && isfinite(v.x) &&
+                isfinite(v.y) && isfinite(v.z));
+        d.pos[i] = r;
+        d.vel[i] = v;
+        d.img[i] = im;
+    }
+    const float m = wave_max(bad ? 0.0f : d2);
+    const int b = wave_or(bad);
+    if (lane_id() == 0) {
+        if (m > 0.0f) atomicMax(&st->maxdisp_bits, __float_as_int(m));
+        if (b) atomicOr(&st->err, ERR_NAN);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// per-replica rebuild decision (src/main.cu:486-496, skin_trigger.cuh:18-33)
+// ---------------------------------------------------------------------------
+__global__ void k_decide(const RepDesc* __restrict__ descs, const int* __restrict__ reps, int nr) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nr) return;
+    const RepDesc& d = descs[reps[k]];
+    RepState* st = d.st;
+    if (!st->active) return;
+    if (st->err) {
+        st->active = 0;
+        st->rebuild = 0;
+        return;
+    }
+    const float md = __int_as_float(st->maxdisp_bits);
+    const int since = st->since_rebuild + 1;
+    const int rb = (md > d.half_skin2) || (since >= kRebuildEvery);
+    st->rebuild = rb;
+    if (rb) {
+        st->since_rebuild = 0;
+        st->maxdisp_bits = 0;
+        st->nrebuilds++;
+        set_grid(d, st);
+    } else {
+        st->since_rebuild = since;
+    }
+}
+
+// Host-requested state changes.
+__global__ void k_control(const RepDesc* __restrict__ descs, const int* __restrict__ reps, int nr,
+                          int op, int64_t arg) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nr) return;
+    const RepDesc& d = descs[reps[k]];
+    RepState* st = d.st;
+    if (op == CTL_RUN) {
+        st->stop = st->step + arg;
+        st->active = (st->err == 0 && arg > 0) ? 1 : 0;
+    } else if (op == CTL_FORCE_REBUILD) {
+        st->rebuild = 1;
+        st->since_rebuild = 0;
+        st->maxdisp_bits = 0;
+        st->nrebuilds++;
+        set_grid(d, st);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Morton-ordered cell sort.  Cells have side >= rc+skin on the reference grid;
+// the cell key is the 3-D Morton interleave of the cell coordinates, so the
+// slot order is a Morton curve over cells, and within a cell atoms are ordered
+// by global id (deterministic).
+// ---------------------------------------------------------------------------
+#define REBUILD_PROLOGUE                                      \
+    const Tile t = tiles[w];                                  \
+    const RepDesc& d = descs[t.rep];                          \
+    RepState* st = d.st;                                      \
+    if (!always && !(st->active && st->rebuild)) continue;    \
+    const int i = t.lt * kTile + lane_id();
+
+__global__ void __launch_bounds__(256) k_bin(const RepDesc* __restrict__ descs,
+                                             const Tile* __restrict__ tiles, int ntl, int always) {
+    for (int w = wave_in_grid(); w < ntl; w += waves_in_grid()) {
+        REBUILD_PROLOGUE
+        if (i < d.natoms) {
+            const float4 r = d.pos[i];
+            const int nx = st->nx;
+            const float ics = st->inv_cs;
+            const uint32_t key = cell_key(cell_coord(r.x, ics, nx), cell_coord(r.y, ics, nx),
+                                          cell_coord(r.z, ics, nx));
+            d.cell_of[i] = (int)key;
+            d.rank[i] = atomicAdd(&d.cell_cnt[key], 1);
+        }
+    }
+}
+
+// Exclusive scan of the cell counts, one 1024-thread workgroup per replica;
+// the counts are cleared for the next rebuild.
+__global__ void __launch_bounds__(1024) k_scan(const RepDesc* __restrict__ descs,
+                                               const int* __restrict__ reps, int nr, int always) {
+    __shared__ int wsum[16];
+    const RepDesc& d = descs[reps[blockIdx.x]];
+    RepState* st = d.st;
+    if (!always && !(st->active && st->rebuild)) return;
+    const int nkeys = 1 << (3 * st->cbits);
+    const int per = (nkeys + 1023) / 1024;
+    const int b0 = threadIdx.x * per;
+    int local = 0;
+    for (int k = 0; k < per; k++) {
+        const int c = b0 + k;
+        if (c < nkeys) local += d.cell_cnt[c];
+    }
+    // inclusive scan of `local` across the workgroup
+    int x = local;
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < 16; k++) { int s = wsum[k]; wsum[k] = acc; acc += s; }
+    }
+    __syncthreads();
+    int run = wsum[wv] + x - local;  // exclusive prefix of this thread's chunk
+    for (int k = 0; k < per; k++) {
+        const int c = b0 + k;
+        if (c < nkeys) {
+            const int n = d.cell_cnt[c];
+            d.cell_start[c] = run;
+            d.cell_cnt[c] = 0;
+            run += n;
+        }
+    }
+    if (threadIdx.x == 1023) d.cell_start[nkeys] = run;
+}
+
+__global__ void __launch_bounds__(256) k_scatter(const RepDesc* __restrict__ descs,
+                                                 const Tile* __restrict__ tiles, int ntl,
+                                                 int always) {
+    for (int w = wave_in_grid(); w < ntl; w += waves_in_grid()) {
+        REBUILD_PROLOGUE
+        if (i < d.natoms) {
+            const int dst = d.cell_start[d.cell_of[i]] + d.rank[i];
+            d.order[dst] = i;
+            d.tmp_pos[i] = d.pos[i];
+            d.tmp_vel[i] = d.vel[i];
+            d.tmp_img[i] = d.img[i];
+        }
+    }
+}
+
+// Rank inside the cell by global id: deterministic order independent of the
+// atomic arrival order of k_bin.
+__global__ void __launch_bounds__(256) k_rank(const RepDesc* __restrict__ descs,
+                                              const Tile* __restrict__ tiles, int ntl,
+                                              int always) {
+    for (int w = wave_in_grid(); w < ntl; w += waves_in_grid()) {
+        REBUILD_PROLOGUE
+        if (i < d.natoms) {
+            const int src = d.order[i];
+            const int key = d.cell_of[src];
+            const int s = d.cell_start[key], e = d.cell_start[key + 1];
+            const int g = d.tmp_img[src].w;
+            int r = 0;
+            for (int q = s; q < e; q++) r += (d.tmp_img[d.order[q]].w < g);
+            d.order2[s + r] = src;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_gather(const RepDesc* __restrict__ descs,
+                                                const Tile* __restrict__ tiles, int ntl,
+                                                int always) {
+    for (int w = wave_in_grid(); w < ntl; w += waves_in_grid()) {
+        REBUILD_PROLOGUE
+        if (i < d.natoms) {
+            const int src = d.order2[i];
+            const float4 p = d.tmp_pos[src];
+            const int4 im = d.tmp_img[src];
+            d.pos[i] = p;
+            d.pos_ref[i] = p;
+            d.vel[i] = d.tmp_vel[src];
+            d.img[i] = im;
+            d.slot_of[im.w] = i;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Verlet list + bonded slot tables (src/neighbor/verlet_list.cu:108-335).
+// Full list, r^2 < (rc+skin)^2 on the reference FP32 distance, i != j, 1-2
+// bonded partners excluded; stored tile-major nbr[tile][k][lane] so the force
+// kernel streams one contiguous 256-byte row per neighbor index.  A count above
+// kcap is reported as ERR_NBR_OVERFLOW, never silently truncated.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_build(const RepDesc* __restrict__ descs,
+                                               const Tile* __restrict__ tiles, int t0, int ntl,
+                                               int always) {
+    for (int w = t0 + wave_in_grid(); w < t0 + ntl; w += waves_in_grid()) {
+        REBUILD_PROLOGUE
+        int err = 0;
+        if (i < d.natoms) {
+            const float4 pi = d.pos[i];
+            const int g = d.img[i].w;
+            const int lane = lane_id();
+            // bonded partners (slots) and the exclusion set
+            int ex[4] = {-1, -1, -1, -1};
+            int nb = 0;
+            const int b0 = d.t_boff[g], b1 = d.t_boff[g + 1];
+            for (int e = b0; e < b1; e++) {
+                const int ent = d.t_blst[e];
+                const int ps = d.slot_of[ent & kSlotMask];
+                if (nb < d.bcap) d.bnd[((size_t)t.lt * d.bcap + nb) * kTile + lane] = ps | (ent & ~kSlotMask);
+                if (nb < 4) ex[nb] = ps;
+                nb++;
+            }
+            if (nb > d.bcap) err |= ERR_BONDED;
+            int na = 0;
+            const int a0 = d.t_aoff[g], a1 = d.t_aoff[g + 1];
+            for (int e = a0; e < a1; e++) {
+                const int4 A = d.t_alst[e];
+                const int role = (A.x == g) ? 0 : ((A.y == g) ? 1 : 2);
+                const int o1 = (role == 0) ? A.y : A.x;
+                const int o2 = (role == 2) ? A.y : A.z;
+                const int s1 = d.slot_of[o1], s2 = d.slot_of[o2];
+                if (na < d.acap)
+                    d.ang[((size_t)t.lt * d.acap + na) * kTile + lane] =
+                        make_int2(s1 | (role << kSlotBits), s2 | (A.w << kSlotBits));
+                na++;
+            }
+            if (na > d.acap) err |= ERR_BONDED;
+            d.bhdr[i] = min(nb, d.bcap) | (min(na, d.acap) << 8);
+            const int* bt = d.bnd + (size_t)t.lt * d.bcap * kTile + lane;
+            // neighbor cells
+            const int nx = st->nx;
+            const float ics = st->inv_cs;
+            const float L = st->L, invL = st->invL;
+            const int cx = cell_coord(pi.x, ics, nx), cy = cell_coord(pi.y, ics, nx),
+                      cz = cell_coord(pi.z, ics, nx);
+            int* out = d.nbr + (size_t)t.lt * d.kcap * kTile + lane;
+            const int kcap = d.kcap;
+            const float rcs2 = d.rcs2;
+            int cnt = 0;
+            int visited[27];
+            int nvis = 0;
+            for (int dz = -1; dz <= 1; dz++) {
+                int z = cz + dz; z += (z < 0) ? nx : 0; z -= (z >= nx) ? nx : 0;
+                for (int dy = -1; dy <= 1; dy++) {
+                    int y = cy + dy; y += (y < 0) ? nx : 0; y -= (y >= nx) ? nx : 0;
+                    for (int dx = -1; dx <= 1; dx++) {
+                        int x = cx + dx; x += (x < 0) ? nx : 0; x -= (x >= nx) ? nx : 0;
+                        const int key = (int)cell_key(x, y, z);
+                        if (nx < 3) {  // small boxes visit a cell once (verlet_list.cu:154-187)
+                            bool seen = false;
+                            for (int v = 0; v < nvis; v++) seen |= (visited[v] == key);
+                            if (seen) continue;
+                            visited[nvis++] = key;
+                        }
+                        const int s = d.cell_start[key], e = d.cell_start[key + 1];
+                        for (int q = s; q < e; q++) {
+                            if (q == i) continue;
+                            const float4 pj = d.pos[q];
+                            if (!(dist2_ref(pi, pj, L, invL) < rcs2)) continue;
+                            bool exc = (q == ex[0]) | (q == ex[1]) | (q == ex[2]) | (q == ex[3]);
+                            for (int b = 4; b < nb && b < d.bcap; b++)
+                                exc |= ((bt[b * kTile] & kSlotMask) == q);
+                            if (exc) continue;
+                            if (cnt < kcap) out[cnt * kTile] = q;
+                            cnt++;
+                        }
+                    }
+                }
+            }
+            if (cnt > kcap) err |= ERR_NBR_OVERFLOW;
+            d.nnbr[i] = min(cnt, kcap);
+        }
+        err = wave_or(err);
+        if (err && lane_id() == 0) atomicOr(&st->err, err);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Force kernel: LJ over the Verlet list (src/force/lj.cu:27-63), FENE+WCA
+// (src/force/fene.cu:21-65) and harmonic angle (src/force/angle.cu:22-79) as
+// per-atom gathers (no float atomics), then the post-force half kick
+// (src/integrate/langevin.cu:109-130).  Writes force[i] = (F, pe_i) and 13
+// FP64 partials per wave: pe, virial[6], kinetic tensor[6] (after the kick).
+// ---------------------------------------------------------------------------
+template <int TERMS>
+__global__ void __launch_bounds__(256) k_force(const RepDesc* __restrict__ descs,
+                                               const Tile* __restrict__ tiles, int t0, int ntl,
+                                               int kick, int always) {
+    constexpr int terms = TERMS;
+    const int w = t0 + wave_in_grid();
+    if (w >= t0 + ntl) return;
+    const Tile t = tiles[w];
+    const RepDesc& d = descs[t.rep];
+    RepState* st = d.st;
+    if (!always && !st->active) return;
+    const int lane = lane_id();
+    const int i = t.lt * kTile + lane;
+    float fx = 0.f, fy = 0.f, fz = 0.f, pe = 0.f;
+    float w0 = 0.f, w1 = 0.f, w2 = 0.f, w3 = 0.f, w4 = 0.f, w5 = 0.f;
+    double k0 = 0, k1 = 0, k2 = 0, k3 = 0, k4 = 0, k5 = 0;
+    int err = 0;
+    if (i < d.natoms) {
+        const float4 pi = d.pos[i];
+        const int ti = type_of(pi.w);
+        const float L = st->L, invL = st->invL;
+        if (terms & 1) {
+            const int nn = d.nnbr[i];
+            const int* nl = d.nbr + (size_t)t.lt * d.kcap * kTile + lane;
+            const float2* ljrow = d.lj + ti * d.ntypes;
+            const float rc2 = d.rc2;
+            const bool clamp = d.clamp_lj != 0;
+            const float fcap = d.fcap;
+            for (int k = 0; k < nn; k++) {
+                const int j = nl[k * kTile];
+                const float4 pj = d.pos[j];
+                const float dx = min_image_exact(pi.x, pj.x, L, invL);
+                const float dy = min_image_exact(pi.y, pj.y, L, invL);
+                const float dz = min_image_exact(pi.z, pj.z, L, invL);
+                const float r2 = sq3(dx, dy, dz);
+                if (r2 < rc2 && r2 > 1e-10f) {
+                    const float2 p = ljrow[type_of(pj.w)];
+                    const float r2inv = 1.0f / r2;
+                    const float sr2 = p.y * p.y * r2inv;
+                    const float sr6 = sr2 * sr2 * sr2;
+                    const float sr12 = sr6 * sr6;
+                    float fr = 24.0f * p.x * r2inv * (2.0f * sr12 - sr6);
+                    if (clamp) fr = fminf(fmaxf(fr, -1000.0f), 1000.0f);
+                    if (fcap > 0.0f) {
+                        const float cap = fcap * rsqrtf(r2);
+                        fr = fminf(fmaxf(fr, -cap), cap);
+                    }
+                    const float hx = 0.5f * fr * dx, hy = 0.5f * fr * dy, hz = 0.5f * fr * dz;
+                    fx += fr * dx; fy += fr * dy; fz += fr * dz;
+                    pe += 2.0f * p.x * (sr12 - sr6);
+                    w0 += hx * dx; w1 += hx * dy; w2 += hx * dz;
+                    w3 += hy * dy; w4 += hy * dz; w5 += hz * dz;
+                }
+            }
+        }
+        const int hdr = (terms & 6) ? d.bhdr[i] : 0;
+        if (terms & 2) {
+            const int nb = hdr & 0xFF;
+            const int* bt = d.bnd + (size_t)t.lt * d.bcap * kTile + lane;
+            for (int b = 0; b < nb; b++) {
+                const int e = bt[b * kTile];
+                const float4 pj = d.pos[e & kSlotMask];
+                const float4 bp = d.bparm[(unsigned)e >> kSlotBits];
+                const float dx = min_image_exact(pi.x, pj.x, L, invL);
+                const float dy = min_image_exact(pi.y, pj.y, L, invL);
+                const float dz = min_image_exact(pi.z, pj.z, L, invL);
+                float r2 = sq3(dx, dy, dz);
+                const float R02 = bp.y * bp.y;
+                // a bond at or beyond R0 is a failure of the run, except during the
+                // capped soft push-off, where the clamp below keeps it finite
+                if (!(r2 < R02) && !(d.fcap > 0.0f)) err |= ERR_STRETCH;
+                r2 = fminf(fmaxf(r2, 1e-6f), R02 * 0.9999f);  // src/force/fene.cu:33
+                float ff = -bp.x * R02 / (R02 - r2);
+                float bpe = -0.5f * bp.x * R02 * logf(1.0f - r2 / R02);
+                const float sig2 = bp.w * bp.w;
+                if (r2 < sig2 * 1.2599210498948732f) {
+                    const float r2inv = 1.0f / r2;
+                    const float sr2 = sig2 * r2inv;
+                    const float sr6 = sr2 * sr2 * sr2;
+                    const float sr12 = sr6 * sr6;
+                    ff += 24.0f * bp.z * r2inv * (2.0f * sr12 - sr6);
+                    bpe += 4.0f * bp.z * (sr12 - sr6) + bp.z;
+                }
+                const float hx = 0.5f * ff * dx, hy = 0.5f * ff * dy, hz = 0.5f * ff * dz;
+                fx += ff * dx; fy += ff * dy; fz += ff * dz;
+                pe += 0.5f * bpe;
+                w0 += hx * dx; w1 += hx * dy; w2 += hx * dz;
+                w3 += hy * dy; w4 += hy * dz; w5 += hz * dz;
+            }
+        }
+        if (terms & 4) {
+            const int na = (hdr >> 8) & 0xFF;
+            const int2* at = d.ang + (size_t)t.lt * d.acap * kTile + lane;
+            for (int a = 0; a < na; a++) {
+                const int2 e = at[a * kTile];
+                const int role = (unsigned)e.x >> kSlotBits;
+                const float2 ap = d.aparm[(unsigned)e.y >> kSlotBits];
+                const float4 q1 = d.pos[e.x & kSlotMask], q2 = d.pos[e.y & kSlotMask];
+                // angle atoms (A, B=center, C) from this atom's role
+                const float4 pa = (role == 0) ? pi : q1;
+                const float4 pb = (role == 1) ? pi : ((role == 0) ? q1 : q2);
+                const float4 pc = (role == 2) ? pi : q2;
+                const float ax = min_image_exact(pa.x, pb.x, L, invL);
+                const float ay = min_image_exact(pa.y, pb.y, L, invL);
+                const float az = min_image_exact(pa.z, pb.z, L, invL);
+                const float bx = min_image_exact(pc.x, pb.x, L, invL);
+                const float by = min_image_exact(pc.y, pb.y, L, invL);
+                const float bz = min_image_exact(pc.z, pb.z, L, invL);
+                const float ra2 = ax * ax + ay * ay + az * az;
+                const float rb2 = bx * bx + by * by + bz * bz;
+                const float cxp = ay * bz - az * by, cyp = az * bx - ax * bz, czp = ax * by - ay * bx;
+                const float cr = sqrtf(cxp * cxp + cyp * cyp + czp * czp);
+                const float dot = ax * bx + ay * by + az * bz;
+                const float theta = atan2f(cr, dot);
+                const float inv_ab = rsqrtf(ra2) * rsqrtf(rb2);
+                const float c = dot * inv_ab;
+                const float s = fmaxf(cr * inv_ab, 1e-6f);
+                const float dth = theta - ap.y;
+                const float pref = 2.0f * ap.x * dth / s;
+                const float ia2 = 1.0f / ra2, ib2 = 1.0f / rb2;
+                const float fix = pref * (bx * inv_ab - c * ax * ia2);
+                const float fiy = pref * (by * inv_ab - c * ay * ia2);
+                const float fiz = pref * (bz * inv_ab - c * az * ia2);
+                const float fkx = pref * (ax * inv_ab - c * bx * ib2);
+                const float fky = pref * (ay * inv_ab - c * by * ib2);
+                const float fkz = pref * (az * inv_ab - c * bz * ib2);
+                if (role == 0) { fx += fix; fy += fiy; fz += fiz; }
+                else if (role == 2) { fx += fkx; fy += fky; fz += fkz; }
+                else { fx -= fix + fkx; fy -= fiy + fky; fz -= fiz + fkz; }
+                const float third = 1.0f / 3.0f;
+                pe += ap.x * dth * dth * third;
+                w0 += (fix * ax + fkx * bx) * third;
+                w1 += (fix * ay + fkx * by) * third;
+                w2 += (fix * az + fkx * bz) * third;
+                w3 += (fiy * ay + fky * by) * third;
+                w4 += (fiy * az + fky * bz) * third;
+                w5 += (fiz * az + fkz * bz) * third;
+            }
+        }
+        d.force[i] = make_float4(fx, fy, fz, pe);
+        float4 v = d.vel[i];
+        const float m = d.mass[ti];
+        if (kick) {
+            const float hdtm = d.half_dt / m;
+            v.x += hdtm * fx; v.y += hdtm * fy; v.z += hdtm * fz;
+            d.vel[i] = v;
+        }
+        const double md = m, vx = v.x, vy = v.y, vz = v.z;
+        k0 = md * vx * vx; k1 = md * vx * vy; k2 = md * vx * vz;
+        k3 = md * vy * vy; k4 = md * vy * vz; k5 = md * vz * vz;
+    }
+    double part[13] = {(double)pe, (double)w0, (double)w1, (double)w2, (double)w3, (double)w4,
+                       (double)w5, k0, k1, k2, k3, k4, k5};
+#pragma unroll
+    for (int c = 0; c < 13; c++) part[c] = wave_sum(part[c]);
+    err = wave_or(err);
+    if (lane < 13) {
+        double v = part[0];
+#pragma unroll
+        for (int c = 1; c < 13; c++) v = (lane == c) ? part[c] : v;
+        d.wpart[(size_t)t.lt * kNumPart + lane] = v;
+    }
+    if (err && lane == 0) atomicOr(&st->err, err);
+}
+
+// ---------------------------------------------------------------------------
+// Per-replica boundary: fixed-order FP64 reduction of the wave partials, then
+// the scalar integrator work (post-force chain / barostat of this step,
+// start-of-step scalars of the next), thermo record, step counter.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_boundary(const RepDesc* __restrict__ descs,
+                                                  const int* __restrict__ reps, int nr, int mode) {
+    __shared__ double red[13][256];
+    const RepDesc& d = descs[reps[blockIdx.x]];
+    RepState* st = d.st;
+    if (mode == BND_START) {
+        if (threadIdx.x == 0 && st->active && !st->pre_ready) start_step(d, st, st->step + 1);
+        return;
+    }
+    if (mode == BND_STEP && !st->active) return;
+    double acc[13];
+#pragma unroll
+    for (int c = 0; c < 13; c++) acc[c] = 0.0;
+    for (int tl = threadIdx.x; tl < d.ntiles; tl += 256)
+#pragma unroll
+        for (int c = 0; c < 13; c++) acc[c] += d.wpart[(size_t)tl * kNumPart + c];
+#pragma unroll
+    for (int c = 0; c < 13; c++) red[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+#pragma unroll
+            for (int c = 0; c < 13; c++) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double pe = red[0][0];
+    double W[6], K[6];
+    for (int c = 0; c < 6; c++) { W[c] = red[1 + c][0]; K[c] = red[7 + c][0]; }
+    const double KE = 0.5 * (K[0] + K[3] + K[5]);
+    if (mode == BND_REDUCE) {
+        st->pe = pe;
+        for (int c = 0; c < 6; c++) { st->vir[c] = W[c]; st->kin[c] = K[c] * (double)st->v_pending * (double)st->v_pending; }
+        return;
+    }
+    if (mode == BND_PRIME) {
+        st->pe = pe;
+        for (int c = 0; c < 6; c++) { st->vir[c] = W[c]; st->kin[c] = K[c]; }
+        st->ke_carry = KE;  // src/main.cu:373-386
+        st->v_pending = 1.0f;
+        st->rec_step = st->step;
+        st->pre_ready = 0;
+        return;
+    }
+    // BND_STEP: end of step cur
+    const int64_t cur = st->step + 1;
+    const double hdt = 0.5 * (double)d.dt, dt = (double)d.dt;
+    double ks = 1.0;
+    if (d.ensemble == 1) {
+        const float frac = (float)(cur - 1) / (float)st->ramp_n;
+        const float T = (float)st->T_start + ((float)st->T_stop - (float)st->T_start) * frac;
+        const double s = nh_chain(st, KE, (double)T, d.natoms, hdt);
+        st->ke_carry = KE * s * s;
+        ks = s;
+    } else if (d.ensemble == 2) {
+        const float frac = (float)(cur - 1) / (float)st->ramp_n;
+        const float T = (float)st->T_start + ((float)st->T_stop - (float)st->T_start) * frac;
+        const float P = (float)st->P_start + ((float)st->P_stop - (float)st->P_start) * frac;
+        const BaroOut o = npt_chain_baro(st, KE, W[0] + W[3] + W[5], true, (double)T, (double)P,
+                                         d.natoms, hdt, dt);
+        st->ke_carry = o.KE_scaled * o.nh_scale * o.nh_scale;
+        ks = o.baro_scale * o.nh_scale;
+        st->L = (float)o.L;
+        st->invL = 1.0f / st->L;
+    }
+    st->v_pending = (float)ks;
+    const double ks2 = (double)st->v_pending * (double)st->v_pending;
+    st->pe = pe;
+    for (int c = 0; c < 6; c++) { st->vir[c] = W[c]; st->kin[c] = K[c] * ks2; }
+    st->step = cur;
+    st->rec_step = cur;
+    if (st->thermo_freq > 0 && (cur % st->thermo_freq) == 0 && st->thermo_cap > 0) {
+        ThermoRec r;
+        const double N = (double)d.natoms, Ld = (double)st->L, V = Ld * Ld * Ld;
+        r.step = cur;
+        r.ke = 0.5 * (st->kin[0] + st->kin[3] + st->kin[5]) / N;
+        r.pe = pe / N;
+        r.temp = 2.0 * r.ke / 3.0;  // src/analysis/thermo.cu:178
+        for (int c = 0; c < 6; c++) r.press[c] = (st->kin[c] + W[c]) / V;
+        r.L = Ld;
+        d.thermo[st->thermo_written % st->thermo_cap] = r;
+        st->thermo_written++;
+    }
+    if (cur >= st->stop) {
+        st->active = 0;
+        st->pre_ready = 0;
+    } else {
+        start_step(d, st, cur + 1);
+    }
+}
+
+// Reference cell id and Morton key by global id (parity entry points).
+__global__ void k_ref_cells(const RepDesc* __restrict__ dp, int natoms, int* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natoms) return;
+    const RepDesc& d = *dp;
+    const RepState* st = d.st;
+    int n = (int)(st->L / d.rcs);
+    const int nx = n < 1 ? 1 : n;
+    const float ics = 1.0f / (st->L / (float)nx);
+    const float4 r = d.pos[i];
+    out[d.img[i].w] = (cell_coord(r.z, ics, nx) * nx + cell_coord(r.y, ics, nx)) * nx +
+                      cell_coord(r.x, ics, nx);
+}
+
+#pragma clang fp contract(off)
+__device__ inline uint32_t quant1024(float x, float invL) {
+    const float q = x * invL * 1024.0f;
+    const uint32_t u = (q > 0.0f) ? (uint32_t)q : 0u;  // truncation, saturating at 0
+    return u > 1023u ? 1023u : u;
+}
+#pragma clang fp contract(fast)
+
+__global__ void k_morton_keys(const RepDesc* __restrict__ dp, int natoms, uint32_t* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natoms) return;
+    const RepDesc& d = *dp;
+    const float invL = d.st->invL;
+    const float4 r = d.pos[i];
+    out[d.img[i].w] = (expand_bits(quant1024(r.z, invL)) << 2) |
+                      (expand_bits(quant1024(r.y, invL)) << 1) | expand_bits(quant1024(r.x, invL));
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+static constexpr int kStrideBlocks = 4096;  // grid cap of the grid-stride rebuild kernels
+
+void launch_pre(const Launch& l) {
+    if (l.ntl <= 0) return;
+    k_pre<<<cdiv(l.ntl, 4), 256, 0, l.stream>>>(l.descs, l.tiles, l.ntl);
+}
+void launch_decide(const Launch& l) {
+    if (l.nr <= 0) return;
+    k_decide<<<cdiv(l.nr, 64), 64, 0, l.stream>>>(l.descs, l.reps, l.nr);
+}
+void launch_control(const Launch& l, int op, int64_t arg) {
+    if (l.nr <= 0) return;
+    k_control<<<cdiv(l.nr, 64), 64, 0, l.stream>>>(l.descs, l.reps, l.nr, op, arg);
+}
+void launch_build(const Launch& l, int t0, int nt, int block, int always) {
+    if (nt <= 0) return;
+    const int wpb = block / 64;
+    int g = cdiv(nt, wpb);
+    if (g > kStrideBlocks) g = kStrideBlocks;
+    k_build<<<g, block, 0, l.stream>>>(l.descs, l.tiles, t0, nt, always);
+}
+void launch_rebuild(const Launch& l, int always, int build_block) {
+    if (l.ntl <= 0) return;
+    int g = cdiv(l.ntl, 4);
+    if (g > kStrideBlocks) g = kStrideBlocks;
+    k_bin<<<g, 256, 0, l.stream>>>(l.descs, l.tiles, l.ntl, always);
+    k_scan<<<l.nr, 1024, 0, l.stream>>>(l.descs, l.reps, l.nr, always);
+    k_scatter<<<g, 256, 0, l.stream>>>(l.descs, l.tiles, l.ntl, always);
+    k_rank<<<g, 256, 0, l.stream>>>(l.descs, l.tiles, l.ntl, always);
+    k_gather<<<g, 256, 0, l.stream>>>(l.descs, l.tiles, l.ntl, always);
+    if (build_block > 0) launch_build(l, 0, l.ntl, build_block, always);
+}
+void launch_force(const Launch& l, int t0, int nt, int block, int terms, int kick, int always) {
+    if (nt <= 0) return;
+    const int g = cdiv(nt, block / 64);
+    switch (terms & 7) {
+        case 1: k_force<1><<<g, block, 0, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
+        case 2: k_force<2><<<g, block, 0, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
+        case 3: k_force<3><<<g, block, 0, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
+        case 4: k_force<4><<<g, block, 0, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
+        case 5: k_force<5><<<g, block, 0, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
+        case 6: k_force<6><<<g, block, 0, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
+        case 7: k_force<7><<<g, block, 0, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
+        default: k_force<0><<<g, block, 0, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
+    }
+}
+void launch_boundary(const Launch& l, int mode) {
+    if (l.nr <= 0) return;
+    k_boundary<<<l.nr, 256, 0, l.stream>>>(l.descs, l.reps, l.nr, mode);
+}
+void launch_ref_cells(const RepDesc* d, int natoms, int* out, hipStream_t s) {
+    if (natoms <= 0) return;
+    k_ref_cells<<<cdiv(natoms, 256), 256, 0, s>>>(d, natoms, out);
+}
+void launch_morton_keys(const RepDesc* d, int natoms, uint32_t* out, hipStream_t s) {
+    if (natoms <= 0) return;
+    k_morton_keys<<<cdiv(natoms, 256), 256, 0, s>>>(d, natoms, out);
+}
+
+}  // namespace fmd

@@ -1,0 +1,303 @@
+// oracle_capi.cpp -- TEST INFRASTRUCTURE ONLY: extern "C" entry points of the
+// CPU oracle for tests/ (ctypes), __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg.  Never linked into the product library.
+#include <cstdio>
+#include "oracle.hpp"
+
+using namespace orc;
+
+extern "C" {
+
+float orc_min_image(float dr, float L, float invL) { return min_image_ref(dr, L, invL); }
+float orc_min_image_exact(float a, float b, float L, float invL) {
+    return min_image_exact_f(a, b, L, invL);
+}
+float orc_wrap(float x, float L, float invL) { return wrap_ref(x, L, invL); }
+uint32_t orc_morton3d(float x, float y, float z, float invL) { return morton3d(x, y, z, invL); }
+
+void orc_morton_perm(const float* pos4, int n, float invL, int* perm) {
+    std::vector<int> p = morton_perm(pos4, n, invL);
+    for (int i = 0; i < n; i++) perm[i] = p[i];
+}
+
+// Reference cell ids (src/neighbor/verlet_list.cu:58-75); returns nx.
+int orc_cell_ids(const float* pos4, int n, float L, float rcs, int* cell) {
+    CellGrid g = cell_grid(L, rcs);
+    for (int i = 0; i < n; i++) cell[i] = cell_id(pos4 + 4 * i, g);
+    return g.nx;
+}
+
+int orc_build_exclusions(int n, const int* bonds, int nbonds, int* off, int* lst, int cap) {
+    std::vector<int> o, l;
+    build_exclusions(n, bonds, nbonds, o, l);
+    for (int i = 0; i <= n; i++) off[i] = o[i];
+    int m = (int)l.size();
+    for (int e = 0; e < m && e < cap; e++) lst[e] = l[e];
+    return m;
+}
+
+// Ordered pair set; method 0 = brute force, 1 = cell grid.  Writes up to cap
+// pairs (2 ints each), returns the total number of pairs.
+long orc_pair_set(const float* pos4, int n, float L, float rcs, const int* eoff, const int* elst,
+                  int method, int* ij, long cap) {
+    std::vector<int> out;
+    float invL = 1.0f / L;
+    if (method == 0)
+        pair_set_brute(pos4, n, L, invL, rcs, eoff, elst, out);
+    else
+        pair_set_cells(pos4, n, L, invL, rcs, eoff, elst, out);
+    long np = (long)out.size() / 2;
+    for (long p = 0; p < np && p < cap; p++) { ij[2 * p] = out[2 * p]; ij[2 * p + 1] = out[2 * p + 1]; }
+    return np;
+}
+
+void orc_lj_literal(const float* pos4, int n, const float* lj2, int ntypes, float rc, float L,
+                    const int* eoff, const int* elst, float* f4, float* vir) {
+    lj_literal_f32(pos4, n, lj2, ntypes, rc, L, 1.0f / L, eoff, elst, f4, vir);
+}
+void orc_fene_literal(const float* pos4, int n, const int* bonds, const int* btype, int nb,
+                      const float* bp, float L, float* f4, float* vir) {
+    fene_literal_f32(pos4, n, bonds, btype, nb, bp, L, 1.0f / L, f4, vir);
+}
+void orc_angle_literal(const float* pos4, int n, const int* ang4, int na, const float* ap,
+                       float L, float* f4, float* vir) {
+    angle_literal_f32(pos4, n, ang4, na, ap, L, 1.0f / L, f4, vir);
+}
+
+void orc_forces_f64(const float* pos4, int n, float L, int ntypes, const float* lj2, float rc,
+                    const float* bp, const float* ap, int clamp_lj, int clamp_fene, float fcap,
+                    int terms, const int* eoff, const int* elst, const int* bonds,
+                    const int* btype, int nbonds, const int* ang4, int nang, double* f8,
+                    double* vir) {
+    ForceParams P{ntypes, lj2, rc, bp, ap, clamp_lj, clamp_fene, fcap};
+    forces_f64(pos4, n, L, P, terms, eoff, elst, bonds, btype, nbonds, ang4, nang, f8, vir);
+}
+
+void orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+    philox4x32_10(ctr, key, out);
+}
+void orc_normal3(uint64_t seed, uint32_t gid, int64_t step, float* z) { normal3(seed, gid, step, z); }
+
+// One NH chain application.  prec 0 = FP32 literal (nh_expf), 1 = FP64.
+// state: xi[10], vxi[10] in/out (double storage either way).  Returns scale.
+double orc_nh_chain(int prec, int M, double Q1, double Qr, double* xi, double* vxi, double KE,
+                    double kT, int natoms, double half_dt) {
+    if (prec == 0) {
+        NHChain<float> c;
+        c.M = M; c.Q1 = (float)Q1; c.Qr = (float)Qr;
+        for (int k = 0; k < 10; k++) { c.xi[k] = (float)xi[k]; c.vxi[k] = (float)vxi[k]; }
+        float s = nh_chain_step<float>(c, (float)KE, (float)kT, natoms, (float)half_dt);
+        for (int k = 0; k < 10; k++) { xi[k] = c.xi[k]; vxi[k] = c.vxi[k]; }
+        return s;
+    }
+    NHChain<double> c;
+    c.M = M; c.Q1 = Q1; c.Qr = Qr;
+    for (int k = 0; k < 10; k++) { c.xi[k] = xi[k]; c.vxi[k] = vxi[k]; }
+    double s = nh_chain_step<double>(c, KE, kT, natoms, half_dt);
+    for (int k = 0; k < 10; k++) { xi[k] = c.xi[k]; vxi[k] = c.vxi[k]; }
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// MD driver in the reference's operator order (src/main.cu:403-630).
+// Positions/velocities in FP32 (as on the device), forces from forces_f64
+// rounded to FP32, KE/virial reductions and NH/barostat scalars in FP64.
+// Langevin noise from normal3(seed, gid, step).  Thermo rows (every step):
+// step, KE/N, PE/N, T, Pxx, Pxy, Pxz, Pyy, Pyz, Pzz, L  (11 doubles).
+// ---------------------------------------------------------------------------
+struct OrcSys {
+    int n, ntypes;
+    float* pos4;   // in/out
+    float* vel3;   // in/out
+    int* img3;     // in/out
+    const float* masses;
+    const float* lj2;
+    int nbonds;
+    const int* bonds;
+    const int* btype;
+    const float* bp;
+    int nang;
+    const int* ang4;
+    const float* ap;
+    float L, rc, skin, dt;
+    int ensemble;  // 0 Langevin, 1 NVT-NH, 2 NPT-NH
+    float T_start, T_stop, Tdamp, P_start, P_stop, Pdamp;
+    int chain;
+    uint64_t seed;
+    long nsteps_total;
+    int clamp_lj, clamp_fene;
+    float fcap;
+};
+
+static void eval_forces(const OrcSys& s, float L, const std::vector<int>& eoff,
+                        const std::vector<int>& elst, std::vector<float>& f4, double* vir,
+                        double* pe) {
+    ForceParams P{s.ntypes, s.lj2, s.rc, s.bp, s.ap, s.clamp_lj, s.clamp_fene, s.fcap};
+    std::vector<double> f8(4 * (size_t)s.n);
+    forces_f64(s.pos4, s.n, L, P, 7, eoff.data(), elst.data(), s.bonds, s.btype, s.nbonds,
+               s.ang4, s.nang, f8.data(), vir);
+    *pe = 0.0;
+    for (int i = 0; i < s.n; i++) {
+        for (int d = 0; d < 4; d++) f4[4 * i + d] = (float)f8[4 * i + d];
+        *pe += f8[4 * i + 3];
+    }
+}
+
+static void kin_tensor(const OrcSys& s, double* k6) {
+    for (int c = 0; c < 6; c++) k6[c] = 0.0;
+    for (int i = 0; i < s.n; i++) {
+        double m = s.masses[type_of(s.pos4[4 * i + 3])];
+        double vx = s.vel3[3 * i], vy = s.vel3[3 * i + 1], vz = s.vel3[3 * i + 2];
+        k6[0] += m * vx * vx; k6[1] += m * vx * vy; k6[2] += m * vx * vz;
+        k6[3] += m * vy * vy; k6[4] += m * vy * vz; k6[5] += m * vz * vz;
+    }
+}
+
+static void wrap_atom(float* r, int* img, float L, float invL) {
+    for (int d = 0; d < 3; d++) {
+        int ix = (int)std::floor(r[d] * invL);
+        r[d] -= (float)ix * L;
+        img[d] += ix;
+    }
+}
+
+int orc_md_run(OrcSys* sp, long nsteps, double* thermo) {
+    OrcSys& s = *sp;
+    const int n = s.n;
+    std::vector<int> eoff, elst;
+    build_exclusions(n, s.bonds, s.nbonds, eoff, elst);
+    std::vector<float> f4(4 * (size_t)n);
+    double vir[6], pe;
+    float L = s.L;
+    eval_forces(s, L, eoff, elst, f4, vir, &pe);
+    const float half_dt = 0.5f * s.dt;
+    const double hdt_d = 0.5 * (double)s.dt, dt_d = (double)s.dt;
+    // Langevin constants (src/integrate/langevin.cu:15-21)
+    float c1 = std::exp(-(1.0f / s.Tdamp) * s.dt);
+    float c2 = std::sqrt(1.0f - c1 * c1);
+    // NH init (src/integrate/nose_hoover.cu:309-346), FP64
+    NHChain<double> ch;
+    ch.M = s.chain > 10 ? 10 : s.chain;
+    double Nf = 3.0 * n, kT0 = s.T_start;
+    ch.Q1 = Nf * kT0 * (double)s.Tdamp * s.Tdamp;
+    ch.Qr = kT0 * (double)s.Tdamp * s.Tdamp;
+    for (int k = 0; k < 10; k++) ch.xi[k] = ch.vxi[k] = 0.0;
+    Baro<double> baro{0.0, 0.0, (Nf + 3.0) * kT0 * (double)s.Pdamp * s.Pdamp,
+                      (double)s.L * s.L * s.L, (double)s.P_start};
+    double k6[6];
+    kin_tensor(s, k6);
+    double carry = 0.5 * (k6[0] + k6[3] + k6[5]);
+    for (long step = 1; step <= nsteps; step++) {
+        float frac = (float)(step - 1) / (float)s.nsteps_total;
+        float kT = s.T_start + (s.T_stop - s.T_start) * frac;
+        float invL = 1.0f / L;
+        if (s.ensemble == 0) {
+            for (int i = 0; i < n; i++) {
+                float* r = s.pos4 + 4 * i;
+                float* v = s.vel3 + 3 * i;
+                float m = s.masses[type_of(r[3])];
+                float hdtm = half_dt / m;
+                for (int d = 0; d < 3; d++) v[d] += hdtm * f4[4 * i + d];
+                for (int d = 0; d < 3; d++) r[d] += half_dt * v[d];
+                float z[3];
+                normal3(s.seed, (uint32_t)i, step, z);
+                float ns = c2 * std::sqrt(kT / m);
+                for (int d = 0; d < 3; d++) v[d] = c1 * v[d] + ns * z[d];
+                for (int d = 0; d < 3; d++) r[d] += half_dt * v[d];
+                wrap_atom(r, s.img3 + 3 * i, L, invL);
+            }
+        } else {
+            float Pt = s.P_start + (s.P_stop - s.P_start) * frac;
+            baro.Ptarget = Pt;
+            double nh, bs = 1.0, exp_vW = 1.0, fpos = s.dt;
+            if (s.ensemble == 1) {
+                nh = nh_chain_step<double>(ch, carry, (double)kT, n, hdt_d);
+            } else {
+                BaroOut<double> o = npt_chain_baro<double>(ch, baro, carry, 0.0, false, (double)kT,
+                                                           n, hdt_d, dt_d);
+                nh = o.nh_scale;
+                bs = o.baro_scale;
+                exp_vW = o.exp_vW;
+                double h = 0.5 * o.veps_W_dt;
+                fpos = dt_d * std::exp(h) * sinch(h);
+                L = (float)o.L;
+                invL = 1.0f / L;
+            }
+            float sc = (float)(nh * bs);
+            float ew = (float)exp_vW, fp = (float)fpos;
+            for (int i = 0; i < n; i++) {
+                float* r = s.pos4 + 4 * i;
+                float* v = s.vel3 + 3 * i;
+                float m = s.masses[type_of(r[3])];
+                float hdtm = half_dt / m;
+                for (int d = 0; d < 3; d++) v[d] *= sc;
+                for (int d = 0; d < 3; d++) v[d] += hdtm * f4[4 * i + d];
+                if (s.ensemble == 1)
+                    for (int d = 0; d < 3; d++) r[d] += s.dt * v[d];
+                else
+                    for (int d = 0; d < 3; d++) r[d] = r[d] * ew + v[d] * fp;
+                wrap_atom(r, s.img3 + 3 * i, L, invL);
+            }
+        }
+        eval_forces(s, L, eoff, elst, f4, vir, &pe);
+        for (int i = 0; i < n; i++) {
+            float m = s.masses[type_of(s.pos4[4 * i + 3])];
+            float hdtm = half_dt / m;
+            for (int d = 0; d < 3; d++) s.vel3[3 * i + d] += hdtm * f4[4 * i + d];
+        }
+        kin_tensor(s, k6);
+        double ke = 0.5 * (k6[0] + k6[3] + k6[5]);
+        double vscale = 1.0;
+        if (s.ensemble == 1) {
+            double sc = nh_chain_step<double>(ch, ke, (double)kT, n, hdt_d);
+            carry = ke * sc * sc;
+            vscale = sc;
+        } else if (s.ensemble == 2) {
+            BaroOut<double> o = npt_chain_baro<double>(ch, baro, ke, vir[0] + vir[3] + vir[5], true,
+                                                       (double)kT, n, hdt_d, dt_d);
+            carry = o.KE_scaled * o.nh_scale * o.nh_scale;
+            vscale = o.baro_scale * o.nh_scale;
+            L = (float)o.L;
+        }
+        if (vscale != 1.0) {
+            float b = (float)(vscale);
+            for (int i = 0; i < 3 * n; i++) s.vel3[i] *= b;
+            kin_tensor(s, k6);
+        }
+        if (thermo) {
+            double* t = thermo + 11 * (step - 1);
+            double V = (double)L * L * L;
+            double kin = 0.5 * (k6[0] + k6[3] + k6[5]);
+            t[0] = (double)step;
+            t[1] = kin / n;
+            t[2] = pe / n;
+            t[3] = 2.0 * (kin / n) / 3.0;
+            for (int c = 0; c < 6; c++) t[4 + c] = (k6[c] + vir[c]) / V;
+            t[10] = L;
+        }
+    }
+    s.L = L;
+    return 0;
+}
+
+// Thermo from state (src/analysis/thermo.cu:140-201): KE/N, PE/N, T=2KE/(3N),
+// P_ab = (sum m v_a v_b + W_ab)/V, virial order xx,xy,xz,yy,yz,zz.
+void orc_thermo(int n, const float* pos4, const float* vel3, const float* masses, const double* pe_atom,
+                const double* vir, float L, double* out) {
+    double k6[6] = {0, 0, 0, 0, 0, 0}, pe = 0;
+    for (int i = 0; i < n; i++) {
+        double m = masses[type_of(pos4[4 * i + 3])];
+        double vx = vel3[3 * i], vy = vel3[3 * i + 1], vz = vel3[3 * i + 2];
+        k6[0] += m * vx * vx; k6[1] += m * vx * vy; k6[2] += m * vx * vz;
+        k6[3] += m * vy * vy; k6[4] += m * vy * vz; k6[5] += m * vz * vz;
+        pe += pe_atom[i];
+    }
+    double ke = 0.5 * (k6[0] + k6[3] + k6[5]);
+    double V = (double)L * L * L;
+    out[0] = ke / n;
+    out[1] = pe / n;
+    out[2] = 2.0 * (ke / n) / 3.0;
+    for (int c = 0; c < 6; c++) out[3 + c] = (k6[c] + vir[c]) / V;
+}
+
+}  // extern "C"

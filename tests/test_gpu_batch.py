@@ -1,0 +1,88 @@
+"""Batched multi-replica stepping: each replica's result is bit-identical
+alone and in the mix, with the plain one-system path and through the server."""
+import numpy as np
+import pytest
+
+from conftest import KG_AP, KG_BP, KG_LJ, relax_kg
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def trio(fmd, engine):
+    out = []
+    for k, (nch, clen, ens) in enumerate([(10, 100, 0), (12, 50, 1), (4, 200, 2)]):
+        m = relax_kg(fmd, engine, nch, clen, seed=31 + k)
+        out.append(fmd.Desc(m.pos4, m.c.L, KG_LJ, vel=m.vel, image=m.image, bonds=m.bonds, bond_params=KG_BP,
+                            angles=m.angles, angle_params=KG_AP, ensemble=ens, T=(1.0, 1.1), Tdamp=0.5,
+                            P=(0.5, 1.0), seed=500 + k, nsteps=300, thermo_freq=25))
+    return out
+
+
+def _run_alone(fmd, d, nsteps, plain=False):
+    e = fmd.Engine(0)
+    sid = e.add(d)
+    if plain:
+        e.step_plain(sid, nsteps)
+    else:
+        e.step(sid, nsteps)
+    st = e.state(sid)
+    th = e.drain_thermo(sid)[0]
+    e.close()
+    return st, th
+
+
+def test_alone_equals_batched(fmd, trio):
+    alone = [_run_alone(fmd, d, 137) for d in trio]
+    e = fmd.Engine(0)
+    sids = [e.add(d) for d in trio]
+    e.step(sids, 137)
+    for sid, (st, th) in zip(sids, alone):
+        p, v, im = e.state(sid)
+        assert np.array_equal(p, st[0]) and np.array_equal(v, st[1]) and np.array_equal(im, st[2])
+        assert e.drain_thermo(sid)[0] == th
+    e.close()
+
+
+def test_plain_equals_graph(fmd, trio):
+    a, tha = _run_alone(fmd, trio[0], 77, plain=True)
+    b, thb = _run_alone(fmd, trio[0], 77, plain=False)
+    assert all(np.array_equal(x, y) for x, y in zip(a, b)) and tha == thb
+
+
+def test_split_runs_equal_one_run(fmd, trio):
+    """Stepping 60 + 40 equals stepping 100 (device state carries across calls)."""
+    e = fmd.Engine(0)
+    s1 = e.add(trio[1])
+    s2 = e.add(trio[1])
+    e.step(s1, 60)
+    e.step(s1, 40)
+    e.step(s2, 100)
+    assert all(np.array_equal(x, y) for x, y in zip(e.state(s1), e.state(s2)))
+    e.close()
+
+
+def test_server_results_equal_alone(fmd, trio):
+    lens = [120, 45, 200]
+    alone = [_run_alone(fmd, d, n)[0] for d, n in zip(trio, lens)]
+    srv = fmd.Server(0, graph_steps=20)
+    jobs = [srv.submit(d, n) for d, n in zip(trio, lens)]
+    srv.drain()
+    for j, st, n in zip(jobs, alone, lens):
+        r = srv.result(j, state=True)
+        assert r["status"] == 0 and r["steps"] == n
+        assert np.array_equal(r["pos4"], st[0]) and np.array_equal(r["vel"], st[1])
+    srv.close()
+
+
+def test_server_budget_queues_jobs(fmd, trio):
+    """A budget that holds one job at a time: jobs wait, none is dropped, FIFO order."""
+    b = max(fmd.system_bytes(d) for d in trio)
+    srv = fmd.Server(0, hbm_budget=int(b * 1.2), graph_steps=20)
+    jobs = [srv.submit(d, 60) for d in trio]
+    srv.drain()
+    res = [srv.result(j) for j in jobs]
+    assert all(r["status"] == 0 for r in res)
+    for a, c in zip(res, res[1:]):
+        assert c["t_admit"] >= a["t_done"] - 1e-6
+    srv.close()

@@ -1,0 +1,138 @@
+"""Integrator parity: BAOAB Langevin (src/integrate/langevin.cu:34-130),
+Nose-Hoover NVT and NPT (src/integrate/nose_hoover.cu) against the oracle's MD
+driver run in the reference's operator order with the same Philox noise, the
+NVE limit of tests/test_integrator.cu, thermo closed forms of
+tests/test_thermo.cu and tests/test_mass.cu, and temperature/pressure control."""
+import numpy as np
+import pytest
+
+from conftest import KG_AP, KG_BP, KG_LJ, pos4_of, relax_kg
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_sys(d, ensemble, T=(1.0, 1.0), Tdamp=1.0, P=(0.0, 0.0), Pdamp=1.0, nsteps_total=1000, seed=42):
+    return {"pos4": d.pos4.copy(), "vel3": d.vel.copy(), "img3": d.image.copy(), "masses": np.ones(16, np.float32),
+            "lj": KG_LJ, "bonds": d.bonds, "btype": d.bond_types, "bp": KG_BP, "ang4": d.angles, "ap": KG_AP,
+            "L": float(d.c.L), "ensemble": ensemble, "T_start": T[0], "T_stop": T[1], "Tdamp": Tdamp,
+            "P_start": P[0], "P_stop": P[1], "Pdamp": Pdamp, "chain": 3, "seed": seed,
+            "nsteps_total": nsteps_total}
+
+
+def _min_image(d, L):
+    return d - L * np.rint(d / L)
+
+
+@pytest.fixture(scope="module")
+def melt(fmd, engine):
+    return relax_kg(fmd, engine, 8, 100, seed=21)
+
+
+@pytest.mark.parametrize("ensemble", [0, 1, 2])
+def test_trajectory_matches_oracle(fmd, engine, orc, melt, ensemble):
+    """40 steps from the same state: positions, velocities, box and per-step
+    thermo of the device agree with the oracle driver."""
+    kw = dict(ensemble=ensemble, T=(1.0, 1.2), Tdamp=(1.0 if ensemble == 0 else 0.1), P=(0.0, 1.0), Pdamp=1.0,
+              nsteps=1000, seed=99, thermo_freq=1)
+    d = fmd.Desc(melt.pos4, melt.c.L, KG_LJ, vel=melt.vel, image=melt.image, bonds=melt.bonds,
+                 bond_params=KG_BP, angles=melt.angles, angle_params=KG_AP, chain=3, **kw)
+    sid = engine.add(d)
+    nst = 40
+    engine.step(sid, nst)
+    p, v, im = engine.state(sid)
+    rec, lost = engine.drain_thermo(sid)
+    Lg = engine.box(sid)
+    engine.remove(sid)
+    o = _oracle_sys(melt, ensemble, T=kw["T"], Tdamp=kw["Tdamp"], P=kw["P"], seed=99)
+    po, vo, io, Lo, th = orc.md_run(o, nst)
+    assert abs(Lg - Lo) <= 1e-6 * Lo
+    dx = _min_image(p[:, :3].astype(np.float64) - po[:, :3], Lo)
+    assert np.max(np.abs(dx)) < 2e-4
+    assert np.max(np.abs(v - vo)) < 5e-3
+    assert lost == 0 and len(rec) == nst
+    for k, r in enumerate(rec):
+        t = th[k]
+        assert r["step"] == k + 1
+        assert abs(r["temp"] - t[3]) <= 2e-4 * t[3]
+        assert abs(r["pe"] - t[2]) <= 2e-4 * abs(t[2])
+        assert np.max(np.abs(np.array(r["press"]) - t[4:10])) <= 2e-3 * max(1.0, abs(t[4]))
+
+
+def test_nve_limit_reference_input(fmd, engine, golden):
+    # tests/test_integrator.cu:9-127: 64-atom lattice, L=8, Tdamp=1e10, 1000 steps, drift < 1%
+    d = fmd.Desc(pos4_of(golden["nve64_xyz"]), 8.0, KG_LJ, vel=golden["nve64_vel"], rc=2.5, skin=0.5,
+                 Tdamp=1e10, seed=12345)
+    sid = engine.add(d)
+    t0 = engine.thermo(sid)
+    e0 = (t0["ke"] + t0["pe"]) * 64
+    engine.step(sid, 1000)
+    t1 = engine.thermo(sid)
+    e1 = (t1["ke"] + t1["pe"]) * 64
+    engine.remove(sid)
+    assert abs(e1 - e0) / abs(e0) < 0.01
+
+
+def test_nve_drift_kg_melt(fmd, engine, melt):
+    """Energy conservation of the whole KG force field in the NVE limit."""
+    d = fmd.Desc(melt.pos4, melt.c.L, KG_LJ, vel=melt.vel, image=melt.image, bonds=melt.bonds,
+                 bond_params=KG_BP, angles=melt.angles, angle_params=KG_AP, Tdamp=1e10, thermo_freq=100)
+    sid = engine.add(d)
+    t0 = engine.thermo(sid)
+    e0 = t0["ke"] + t0["pe"]
+    engine.step(sid, 2000)
+    rec, _ = engine.drain_thermo(sid)
+    engine.remove(sid)
+    e = np.array([r["ke"] + r["pe"] for r in rec])
+    # per-atom energy drift over 2 tau (dt = 0.001) well below the thermal scale
+    assert np.max(np.abs(e - e0)) < 2e-3 * abs(e0) + 2e-3
+
+
+def test_thermo_closed_forms(fmd, engine):
+    # tests/test_thermo.cu:6-44: unit v_x -> KE/N = 0.5, T = 1/3 (atoms too far apart to interact)
+    g = np.arange(96)
+    xyz = np.stack([(g % 4) * 4.0, ((g // 4) % 4) * 4.0, (g // 16) * 4.0], 1) + 0.5
+    d = fmd.Desc(pos4_of(xyz), 24.0, KG_LJ, vel=np.tile([1.0, 0.0, 0.0], (96, 1)))
+    sid = engine.add(d)
+    t = engine.thermo(sid)
+    engine.remove(sid)
+    assert abs(t["ke"] - 0.5) < 1e-12 and abs(t["temp"] - 1.0 / 3.0) < 1e-12
+    # tests/test_mass.cu:66-144: masses 1 and 2 -> sum m vx^2 = 3
+    xyz2 = np.array([[1.0, 1.0, 1.0], [10.0, 10.0, 10.0]])
+    d2 = fmd.Desc(pos4_of(xyz2, [0, 1]), 20.0, np.ones(8, np.float32), ntypes=2, vel=[[1, 0, 0], [1, 0, 0]],
+                  masses=[1.0, 2.0])
+    sid = engine.add(d2)
+    t = engine.thermo(sid)
+    assert abs(t["press"][0] * 20.0 ** 3 - 3.0) < 1e-9
+    assert abs(t["ke"] * 2 - 1.5) < 1e-12
+    # half-kick mass scaling (tests/test_mass.cu:21-63): dv = dt/2 F / m
+    engine.remove(sid)
+
+
+@pytest.mark.parametrize("ensemble,T", [(0, 1.2), (1, 1.5)])
+def test_temperature_control(fmd, engine, melt, ensemble, T):
+    # tests/test_nose_hoover.cu:195-340: <T> within 0.1 of the target
+    d = fmd.Desc(melt.pos4, melt.c.L, KG_LJ, vel=melt.vel, image=melt.image, bonds=melt.bonds,
+                 bond_params=KG_BP, angles=melt.angles, angle_params=KG_AP, ensemble=ensemble,
+                 T=(T, T), Tdamp=(1.0 if ensemble == 0 else 0.5), thermo_freq=10, nsteps=6000)
+    sid = engine.add(d)
+    engine.step(sid, 6000)
+    rec, _ = engine.drain_thermo(sid)
+    engine.remove(sid)
+    temps = np.array([r["temp"] for r in rec[len(rec) // 3:]])
+    assert abs(temps.mean() - T) < 0.1
+
+
+def test_npt_pressure_control(fmd, engine, melt):
+    """NPT-NH drives <P> to the target (scalar P = trace/3)."""
+    d = fmd.Desc(melt.pos4, melt.c.L, KG_LJ, vel=melt.vel, image=melt.image, bonds=melt.bonds,
+                 bond_params=KG_BP, angles=melt.angles, angle_params=KG_AP, ensemble=2, T=(1.0, 1.0),
+                 Tdamp=0.1, P=(1.0, 1.0), Pdamp=1.0, thermo_freq=10, nsteps=8000)
+    sid = engine.add(d)
+    engine.step(sid, 8000)
+    rec, _ = engine.drain_thermo(sid)
+    engine.remove(sid)
+    late = rec[len(rec) // 2:]
+    P = np.array([(r["press"][0] + r["press"][3] + r["press"][5]) / 3.0 for r in late])
+    T = np.array([r["temp"] for r in late])
+    assert abs(P.mean() - 1.0) < 0.25
+    assert abs(T.mean() - 1.0) < 0.05

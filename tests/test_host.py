@@ -1,0 +1,203 @@
+"""Host logic on the CPU: run.conf and LAMMPS data parsing (src/io/config.cpp,
+src/io/lammps_data.cpp), exclusions, the restart writer, the KG generator, the
+admission policy, and that the C-ABI library exports every declared symbol."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+
+def test_library_exports_every_declared_symbol(fmd):
+    hdr = open(os.path.join(ROOT, "include", "fmd", "fmd.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    names = set(re.findall(r"\b(fmd_[a-z_0-9]+)\s*\(", hdr))
+    assert len(names) > 40
+    lib = fmd.lib()
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    bound = {n for n, _, _ in fmd.api.SIGNATURES}
+    assert names == bound, (names - bound, bound - names)
+
+
+def test_mini_data_reference_fixture(fmd):
+    # tests/test_lammps_data.cu:5-37 on tests/fixtures/mini.data
+    t = fmd.Topo.from_data(os.path.join(GOLDEN, "mini.data"))
+    a = t.arrays()
+    assert t.natoms == 4
+    assert a["pos4"][0, 0] == 1.0 and a["pos4"][3, 2] == 6.0
+    assert a["pos4"][0, 3].view(np.int32) == 0
+    assert list(a["mol"]) == [1, 1, 1, 1]
+    assert a["image"].shape == (4, 3) and not a["image"].any()
+    assert a["bonds"].tolist() == [[0, 1], [1, 2]] and a["bond_types"].tolist() == [0, 0]
+    assert a["angles"].tolist() == [[0, 1, 2, 0]]
+    assert t.c.L == 10.0 and a["masses"].tolist() == [1.0]
+
+
+def _write(tmp_path, name, text):
+    p = tmp_path / name
+    p.write_text(text)
+    return str(p)
+
+
+def test_run_conf_keys_and_defaults(fmd, tmp_path):
+    data = open(os.path.join(GOLDEN, "mini.data")).read()
+    dp = _write(tmp_path, "m.data", data)
+    conf = f"""# comment
+natoms 4
+ntypes 1
+rc 2.5
+skin 0.3
+dt 0.001
+nsteps 100
+unknown_key 7
+thermo 1 10 thermo.dat
+restart 50 restart.data
+nvt_langevin 1.0 1.2 1.0
+lj 0 0 1.0 1.0
+bond_type 0 30.0 1.5 1.0 1.0
+angle_type 0 5.0 180.0
+lammps_data_file {dp}
+forces_dump_file f.txt
+"""
+    t = fmd.Topo.from_conf(_write(tmp_path, "run.conf", conf))
+    d = t.c
+    assert d.natoms == 4 and d.ntypes == 1 and abs(d.rc - 2.5) < 1e-7 and d.nsteps == 100
+    assert d.ensemble == fmd.LANGEVIN and d.seed == 42 and abs(d.T_stop - 1.2) < 1e-7
+    # theta0 converted to radians in FP32 (src/io/config.cpp:176)
+    assert t.arrays()["angle_params"][0, 1] == np.float32(180.0) * np.float32(np.pi) / np.float32(180.0)
+    e = t.extra()
+    assert e.thermo_on == 1 and e.thermo_freq == 10 and e.thermo_file == b"thermo.dat"
+    assert e.restart_freq == 50 and e.forces_dump_file == b"f.txt"
+    # nh defaults: chain 3; explicit seed for Langevin
+    t2 = fmd.Topo.from_conf(_write(tmp_path, "r2.conf", conf.replace("nvt_langevin 1.0 1.2 1.0",
+                                                                     "nvt_nh 1.0 1.0 0.1")))
+    assert t2.c.ensemble == fmd.NVT_NH and t2.c.chain_length == 3
+    t3 = fmd.Topo.from_conf(_write(tmp_path, "r3.conf", conf.replace("nvt_langevin 1.0 1.2 1.0",
+                                                                     "nvt_langevin 1.0 1.0 1.0 777")))
+    assert t3.c.seed == 777
+    t4 = fmd.Topo.from_conf(_write(tmp_path, "r4.conf", conf.replace("nvt_langevin 1.0 1.2 1.0",
+                                                                     "npt_nh 1 1 0.1 0 1 1 5")))
+    assert t4.c.ensemble == fmd.NPT_NH and t4.c.chain_length == 5 and abs(t4.c.P_stop - 1.0) < 1e-7
+
+
+@pytest.mark.parametrize("line,code", [
+    ("temperature 1.0", "PARSE"), ("gamma 1.0", "PARSE"), ("seed 3", "PARSE"),
+    ("nvt_langevin 1.0 1.0 0.0", "PARSE"), ("npt_nh 1 1 0.1 0 1 0.0", "PARSE"),
+    ("nvt_nh 1 1 0.1 0", "PARSE"), ("table 0 0 t.txt KEY", "PARSE"), ("natoms 5", "PARSE"),
+])
+def test_run_conf_errors_are_returned(fmd, tmp_path, line, code):
+    dp = _write(tmp_path, "m.data", open(os.path.join(GOLDEN, "mini.data")).read())
+    base = f"ntypes 1\nrc 2.5\nskin 0.3\ndt 0.001\nnsteps 1\nnvt_langevin 1 1 1\nlj 0 0 1 1\n" \
+           f"bond_type 0 30 1.5 1 1\nangle_type 0 5 180\nlammps_data_file {dp}\n"
+    with pytest.raises(fmd.FmdError) as ei:
+        fmd.Topo.from_conf(_write(tmp_path, "bad.conf", base + line + "\n"))
+    assert ei.value.code == getattr(fmd.api, "ERR_" + code)
+
+
+def test_missing_and_malformed_files(fmd, tmp_path):
+    with pytest.raises(fmd.FmdError) as ei:
+        fmd.Topo.from_conf(str(tmp_path / "nope.conf"))
+    assert ei.value.code == fmd.api.ERR_IO
+    bad = "x\n\n2 atoms\n0 10 xlo xhi\n0 10 ylo yhi\n0 12 zlo zhi\n\nAtoms\n\n1 1 1 1 1\n"
+    with pytest.raises(fmd.FmdError) as ei:
+        fmd.Topo.from_data(_write(tmp_path, "nc.data", bad))
+    assert ei.value.code == fmd.ERR_PARSE  # non-cubic
+    bad2 = "x\n\n0 10 xlo xhi\n0 10 ylo yhi\n0 10 zlo zhi\n\nAtoms # atomic\n\n1 1 abc 1 1\n"
+    with pytest.raises(fmd.FmdError):
+        fmd.Topo.from_data(_write(tmp_path, "bad.data", bad2))
+
+
+def test_atom_styles(fmd, tmp_path):
+    hdr = "t\n\n2 atoms\n\n0 5 xlo xhi\n0 5 ylo yhi\n0 5 zlo zhi\n\n"
+    full = hdr + "Atoms # full\n\n1 7 1 0.5 1.0 2.0 3.0 1 0 -1\n2 7 1 -0.5 2.0 2.0 3.0\n"
+    a = fmd.Topo.from_data(_write(tmp_path, "f.data", full)).arrays()
+    assert a["pos4"][0, :3].tolist() == [1.0, 2.0, 3.0] and a["image"][0].tolist() == [1, 0, -1]
+    assert a["mol"].tolist() == [7, 7]
+    atomic = hdr + "Atoms\n\n1 1 1.0 2.0 3.0\n2 1 4.0 2.0 3.0\n\nVelocities\n\n2 0.5 0 0\n"
+    a = fmd.Topo.from_data(_write(tmp_path, "a.data", atomic)).arrays()
+    assert a["pos4"][1, 0] == 4.0 and a["vel"][1].tolist() == [0.5, 0.0, 0.0]
+
+
+def test_exclusions_match_reference_rule(fmd):
+    t = fmd.Topo.kg_melt(3, 10, 0.85, 1)
+    off, lst = t.exclusions()
+    b = t.arrays()["bonds"]
+    n = t.natoms
+    ref = [sorted(set([j for i, j in b if i == a] + [i for i, j in b if j == a])) for a in range(n)]
+    assert [list(lst[off[a]:off[a + 1]]) for a in range(n)] == ref
+
+
+def test_restart_writer_round_trip(fmd, tmp_path):
+    t = fmd.Topo.kg_melt(4, 12, 0.85, 3)
+    a = t.arrays()
+    p = str(tmp_path / "r.data")
+    t.write_data(p, step=123)
+    txt = open(p).read()
+    assert txt.startswith("# Restart from step 123\n") and f"{t.c.L:.6f} zlo zhi" in txt
+    r = fmd.Topo.from_data(p).arrays()
+    assert np.allclose(r["pos4"][:, :3], a["pos4"][:, :3], atol=1e-6)
+    assert np.array_equal(r["bonds"], a["bonds"]) and np.array_equal(r["angles"], a["angles"])
+    assert np.array_equal(r["image"], a["image"]) and np.array_equal(r["mol"], a["mol"])
+
+
+def test_kg_generator(fmd):
+    t = fmd.Topo.kg_melt(20, 50, 0.85, 9)
+    a = t.arrays()
+    n = t.natoms
+    L = t.c.L
+    assert n == 1000 and abs(n / L ** 3 - 0.85) < 1e-5
+    x = a["pos4"][:, :3]
+    assert np.all(x >= 0) and np.all(x < L)
+    un = x.astype(np.float64) + a["image"] * L
+    bl = np.linalg.norm(un[a["bonds"][:, 0]] - un[a["bonds"][:, 1]], axis=1)
+    assert np.allclose(bl, 0.97, atol=1e-4)
+    assert len(a["bonds"]) == 20 * 49 and len(a["angles"]) == 20 * 48
+    assert a["bond_params"].tolist() == [[30.0, 1.5, 1.0, 1.0]]
+    t2 = fmd.Topo.kg_melt(20, 50, 0.85, 9)
+    assert np.array_equal(t2.arrays()["pos4"], a["pos4"])  # deterministic in the seed
+
+
+def test_uneven_workload_specs(fmd):
+    from fastmd_amd import workloads
+    s = workloads.uneven_replica_specs(256)
+    beads = np.array([x["nchains"] * x["chain_len"] for x in s])
+    clen = np.array([x["chain_len"] for x in s])
+    assert len(s) == 256 and clen.min() >= 20 and clen.max() <= 200
+    assert beads.min() >= 1900 and beads.max() <= 64200
+    assert 3.5e6 < beads.sum() < 5.8e6
+    assert len({x["seed"] for x in s}) == 256
+    assert all(0.8 <= x["T"] <= 1.2 for x in s)
+    assert [fmd.size_class(b) for b in (2000, 8191, 8192, 40000)] == [0, 0, 1, 2]
+
+
+def test_system_bytes_budget_unit(fmd):
+    t = fmd.Topo.kg_melt(10, 100, 0.85, 1)
+    b = fmd.system_bytes(t)
+    assert 500 * 1000 < b < 3000 * 1000  # ~1-2 KB per bead including a 256-row list reserve
+
+
+def test_admission_replay_fifo_and_budget(fmd):
+    rng = np.random.default_rng(0)
+    n = 200
+    arr = np.cumsum(rng.exponential(1.0, n))
+    size = rng.integers(1, 40, n).astype(np.uint64) * 1000
+    dur = rng.exponential(5.0, n)
+    budget = 100 * 1000
+    ta, tf = fmd.admission_replay(arr, size, dur, budget, max_jobs=8)
+    assert np.all(ta >= arr - 1e-12) and np.allclose(tf, ta + dur)
+    assert np.all(np.diff(ta) >= -1e-12)  # FIFO: admitted in arrival order
+    ev = sorted([(t, +1, k) for k, t in enumerate(ta)] + [(t, -1, k) for k, t in enumerate(tf)],
+                key=lambda e: (e[0], e[1]))
+    used, live = 0, 0
+    for t, kind, k in ev:
+        used += int(size[k]) * kind
+        live += kind
+        assert used <= budget and live <= 8
+
+
+def test_admission_replay_rejects_oversize(fmd):
+    ta, tf = fmd.admission_replay([0.0, 1.0, 2.0], np.array([10, 500, 10], np.uint64), [1.0, 1.0, 1.0], 100)
+    assert ta[1] == -1.0 and ta[0] == 0.0 and ta[2] == 2.0

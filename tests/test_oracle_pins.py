@@ -1,0 +1,285 @@
+"""Pins the CPU oracle (CPU only).  Each check is independent of the formula
+it checks: published known-answer vectors, closed forms, invariants, two
+different algorithms for the same set, and finite differences of the energy."""
+import numpy as np
+import pytest
+
+from conftest import KG_AP, KG_BP, KG_LJ, pos4_of
+
+F = np.float32
+
+
+# --- Philox4x32-10 known-answer vectors (Random123 kat_vectors, Salmon et al. SC'11)
+@pytest.mark.parametrize("ctr,key,out", [
+    ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+    ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+    ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+     [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+])
+def test_philox_kat(orc, ctr, key, out):
+    assert orc.philox(ctr, key) == out
+
+
+def test_normals_are_standard(orc):
+    z = np.array([orc.normal3(7, g, s) for g in range(300) for s in range(1, 40)]).ravel()
+    assert abs(z.mean()) < 0.02 and abs(z.var() - 1.0) < 0.02
+    assert abs(np.mean(z ** 4) - 3.0) < 0.15  # Gaussian kurtosis
+    # independent streams per (gid, step)
+    assert not np.array_equal(orc.normal3(7, 1, 1), orc.normal3(7, 2, 1))
+    assert not np.array_equal(orc.normal3(7, 1, 1), orc.normal3(7, 1, 2))
+    assert np.array_equal(orc.normal3(7, 3, 5), orc.normal3(7, 3, 5))
+
+
+# --- PBC (tests/test_pbc.cu:22-93 inputs)
+def test_min_image_hand_cases(orc):
+    L = F(10.0)
+    r1 = [[1.0, 2.0, 3.0], [9.5, 0.5, 5.0], [0.5, 0.5, 0.5], [5.0, 5.0, 5.0], [0.1, 9.9, 5.0]]
+    r2 = [[2.0, 3.0, 4.0], [0.5, 9.5, 5.0], [9.5, 9.5, 9.5], [5.0, 5.0, 5.0], [9.9, 0.1, 5.0]]
+    exact = [[-1, -1, -1], [-1, 1, 0], [1, 1, 1], [0, 0, 0], [0.2, -0.2, 0.0]]
+    for a, b, e in zip(r1, r2, exact):
+        for d in range(3):
+            got = orc.min_image(F(F(a[d]) - F(b[d])), L)
+            assert abs(got - e[d]) <= 4e-7 * 10 and -5.0 <= got <= 5.0
+
+
+def test_wrap_lands_in_box(orc):
+    L = F(10.0)
+    for x in [-0.5, 10.5, 5.0, 20.3, -3.7, 0.0, 10.0, -0.01, 1e-8, -1e-8]:
+        w = orc.wrap(F(x), L)
+        assert 0.0 <= w <= L  # == L only for tiny negatives rounding up
+        assert abs(((x - w) / 10.0) - round((x - w) / 10.0)) < 1e-6
+
+
+def test_min_image_exact_is_correctly_rounded(orc):
+    rng = np.random.default_rng(0)
+    L = F(33.0)
+    a = (rng.random(2000) * L).astype(F)
+    b = (rng.random(2000) * L).astype(F)
+    for x, y in zip(a, b):
+        got = orc.lib().orc_min_image_exact(x, y, L, F(1) / L)
+        s = F(x - y)
+        m = np.rint(F(s * (F(1) / L)))
+        exact = float(x) - float(y) - float(L) * float(m)
+        assert got == F(exact)
+
+
+# --- Morton keys: decode the interleave bit by bit and compare with the quantised coordinates
+def _deinterleave(k):
+    out = [0, 0, 0]
+    for bit in range(10):
+        for a in range(3):
+            out[a] |= ((k >> (3 * bit + a)) & 1) << bit
+    return out
+
+
+def test_morton_keys_decode(orc, golden):
+    L = F(10.0)
+    for x, y, z in golden["morton64_xyz"]:
+        k = orc.morton3d(x, y, z, L)
+        q = [min(max(int(np.floor(float(F(F(c) * (F(1) / L)) * F(1024)))), 0), 1023) for c in (x, y, z)]
+        assert _deinterleave(k) == q
+    assert orc.morton3d(F(9.9999), F(9.9999), F(9.9999), L) == (1 << 30) - 1
+
+
+def test_morton_perm_is_stable_sort(orc, golden):
+    p4 = pos4_of(golden["morton64_xyz"])
+    keys = [orc.morton3d(*x, F(10.0)) for x in golden["morton64_xyz"]]
+    assert list(orc.morton_perm(p4, F(10.0))) == sorted(range(64), key=lambda i: (keys[i], i))
+
+
+# --- cells: every atom lies in the bounds of its cell
+@pytest.mark.parametrize("L,rcs", [(32.802, 2.8), (8.0, 3.0), (13.3, 2.8), (105.567, 2.8)])
+def test_cell_ids_contain_atoms(orc, L, rcs):
+    rng = np.random.default_rng(1)
+    p4 = pos4_of((rng.random((3000, 3)) * L).astype(F))
+    cid, nx = orc.cell_ids(p4, F(L), F(rcs))
+    assert nx == max(1, int(F(L) / F(rcs)))
+    cs = L / nx
+    c = np.stack([cid % nx, (cid // nx) % nx, cid // (nx * nx)], 1)
+    x = p4[:, :3].astype(np.float64)
+    assert np.all(x >= c * cs - 1e-4) and np.all(x < (c + 1) * cs + 1e-4)
+
+
+# --- pair sets: brute force and cell enumeration agree; cutoff is strict
+@pytest.mark.parametrize("n,L,rcs,bonded", [(800, 9.8, 2.8, False), (600, 7.0, 3.0, True), (300, 5.5, 2.8, True),
+                                            (900, 15.0, 2.8, True)])
+def test_pair_set_brute_equals_cells(orc, n, L, rcs, bonded):
+    rng = np.random.default_rng(n)
+    p4 = pos4_of((rng.random((n, 3)) * L).astype(F))
+    excl = None
+    if bonded:
+        bonds = np.stack([np.arange(0, n - 1, 2), np.arange(1, n, 2)], 1)
+        excl = orc.exclusions(n, bonds)
+    a = orc.pair_set(p4, F(L), F(rcs), excl, method=0)
+    b = orc.pair_set(p4, F(L), F(rcs), excl, method=1)
+    assert np.array_equal(a, b) and len(a) > 0
+    # symmetric, no self pairs, excluded pairs absent
+    s = set(map(tuple, a.tolist()))
+    assert all((j, i) in s for i, j in s) and all(i != j for i, j in s)
+    if bonded:
+        assert (0, 1) not in s and (1, 0) not in s
+
+
+def test_pair_cutoff_strict(orc):
+    # two atoms exactly rcs apart along x: not neighbors (strict <, verlet_list.cu:268)
+    p4 = pos4_of([[1.0, 5.0, 5.0], [4.0, 5.0, 5.0]])
+    assert len(orc.pair_set(p4, F(20.0), F(3.0))) == 0
+    p4 = pos4_of([[1.0, 5.0, 5.0], [3.99, 5.0, 5.0]])
+    assert len(orc.pair_set(p4, F(20.0), F(3.0))) == 2
+
+
+def test_exclusions_csr(orc):
+    off, lst = orc.exclusions(5, [[0, 1], [1, 2], [2, 1], [3, 4]])
+    assert list(off) == [0, 1, 3, 4, 5, 6] and list(lst) == [1, 0, 2, 1, 4, 3]
+
+
+# --- forces: closed forms and finite differences of the FP64 energy
+def _energy(orc, p4, L, terms, **kw):
+    f8, _ = orc.forces_f64(p4, F(L), KG_LJ, terms=terms, clamp_lj=0, clamp_fene=0, **kw)
+    return f8[:, 3].sum()
+
+
+def test_lj_closed_forms(orc):
+    rmin = 2 ** (1 / 6)
+    p4 = pos4_of([[5.0, 5.0, 5.0], [5.0 + rmin, 5.0, 5.0]])
+    f8, vir = orc.forces_f64(p4, F(20.0), KG_LJ, terms=1)
+    # force vanishes at the minimum, up to U''(rmin) = 57.1 times the FP32 rounding of r
+    dr = abs(float(p4[1, 0]) - float(p4[0, 0]) - rmin)
+    assert np.all(np.abs(f8[:, :3]) <= 57.2 * dr + 1e-9)
+    assert abs(f8[:, 3].sum() + 1.0) < 1e-6           # pair energy -eps, half to each
+    p4 = pos4_of([[5.0, 5.0, 5.0], [6.0, 5.0, 5.0]])
+    f8, vir = orc.forces_f64(p4, F(20.0), KG_LJ, terms=1)
+    assert abs(f8[0, 0] + 24.0) < 1e-9 and abs(f8[1, 0] - 24.0) < 1e-9
+    assert abs(vir[0] - 24.0) < 1e-9                  # W_xx = f.r for the pair (half from each atom)
+    assert abs(f8[:, 3].sum()) < 1e-12
+
+
+def test_fene_wca_closed_form(orc):
+    p4 = pos4_of([[5.0, 5.0, 5.0], [6.0, 5.0, 5.0]])
+    f8, vir = orc.forces_f64(p4, F(20.0), KG_LJ, bonds=[[0, 1]], bp=KG_BP, terms=2)
+    ff = -30.0 * 2.25 / (2.25 - 1.0) + 24.0
+    assert abs(f8[0, 0] - ff * -1.0) < 1e-9
+    pe = -0.5 * 30.0 * 2.25 * np.log(1 - 1 / 2.25) + 1.0
+    assert abs(f8[:, 3].sum() - pe) < 1e-9
+
+
+def test_angle_straight_chain_is_force_free(orc):
+    p4 = pos4_of([[4.0, 5.0, 5.0], [5.0, 5.0, 5.0], [6.0, 5.0, 5.0]])
+    f8, _ = orc.forces_f64(p4, F(20.0), KG_LJ, ang4=[[0, 1, 2, 0]], ap=KG_AP, terms=4)
+    assert np.all(np.abs(f8[:, :3]) < 1e-5) and abs(f8[:, 3].sum()) < 1e-12
+
+
+def _saw_chain(n, L, seed, bond=0.97, dmin=0.95):
+    """A self-avoiding random-walk chain (all non-bonded distances >= dmin)."""
+    rng = np.random.default_rng(seed)
+    x = [np.full(3, L / 2)]
+    while len(x) < n:
+        u = rng.normal(size=3)
+        c = x[-1] + bond * u / np.linalg.norm(u)
+        if len(x) >= 2:
+            d = np.array(x[:-1]) - c
+            d -= L * np.rint(d / L)
+            if np.min(np.linalg.norm(d, axis=1)) < dmin:
+                continue
+        x.append(c)
+    xyz = np.mod(np.array(x), L).astype(F)
+    bonds = np.array([[i, i + 1] for i in range(n - 1)], np.int32)
+    ang = np.array([[i, i + 1, i + 2, 0] for i in range(n - 2)], np.int32)
+    return xyz, dict(bonds=bonds, bp=KG_BP, ang4=ang, ap=KG_AP)
+
+
+@pytest.mark.parametrize("terms", [1, 2, 4, 7])
+def test_forces_are_minus_energy_gradient(orc, terms):
+    L = 7.0
+    xyz, kw = _saw_chain(60, L, seed=terms)
+    p4 = pos4_of(xyz)
+    f8, _ = orc.forces_f64(p4, F(L), KG_LJ, terms=terms, clamp_lj=0, clamp_fene=0, **kw)
+    rng = np.random.default_rng(100 + terms)
+    h = 1e-3
+    for i in rng.choice(len(xyz), 10, replace=False):
+        for d in range(3):
+            pp, pm = p4.copy(), p4.copy()
+            pp[i, d] = F(p4[i, d] + h)
+            pm[i, d] = F(p4[i, d] - h)
+            dx = float(pp[i, d]) - float(pm[i, d])
+            fd = -(_energy(orc, pp, L, terms, **kw) - _energy(orc, pm, L, terms, **kw)) / dx
+            assert abs(fd - f8[i, d]) <= 2e-3 * max(1.0, abs(f8[i, d]))
+
+
+@pytest.mark.parametrize("terms", [1, 2, 4, 7])
+def test_virial_trace_is_volume_derivative(orc, terms):
+    """W_xx+W_yy+W_zz = -dU/dlambda at lambda=1 for r -> lambda r, L -> lambda L."""
+    L = 7.0
+    xyz, kw = _saw_chain(60, L, seed=10 + terms)
+    _, vir = orc.forces_f64(pos4_of(xyz), F(L), KG_LJ, terms=terms, clamp_lj=0, clamp_fene=0, **kw)
+    if terms == 4:
+        # a bending energy depends on angles only, so it is scale-invariant: zero trace
+        assert abs(vir[0] + vir[3] + vir[5]) < 1e-9 * max(1.0, np.abs(vir).max())
+        return
+    # small h: the unshifted LJ energy jumps by 0.0163 when a pair crosses rc,
+    # and no pair of these chains crosses within +-1e-5 scaling
+    h = 1e-5
+    ep = _energy(orc, pos4_of(xyz.astype(np.float64) * (1 + h)), L * (1 + h), terms, **kw)
+    em = _energy(orc, pos4_of(xyz.astype(np.float64) * (1 - h)), L * (1 - h), terms, **kw)
+    dU = (ep - em) / (2 * h)
+    tr = vir[0] + vir[3] + vir[5]
+    assert abs(tr + dU) <= 1e-2 * max(1.0, abs(tr))
+
+
+def test_literal_lj_matches_f64(orc, golden):
+    # the literal FP32 restatement and the FP64 yardstick agree on the reference
+    # LJ input away from clamped pairs (no pair of this input is within r < 0.79)
+    p4 = pos4_of(golden["lj64_xyz"])
+    fl, vl = orc.lj_literal(p4, KG_LJ, 1, 2.5, F(8.0))
+    f8, v8 = orc.forces_f64(p4, F(8.0), KG_LJ, terms=1, clamp_lj=0)
+    err = np.linalg.norm(fl[:, :3] - f8[:, :3], axis=1) / np.maximum(np.linalg.norm(f8[:, :3], axis=1), 1.0)
+    assert err.max() < 1e-3
+
+
+def test_literal_bonded_match_f64(orc):
+    p4 = pos4_of([[2.0, 5.0, 5.0], [3.0, 5.0, 5.0], [4.2, 5.0, 5.0]])
+    fl, vl = orc.fene_literal(p4, [[0, 1], [1, 2]], [0, 0], KG_BP, F(10.0))
+    f8, v8 = orc.forces_f64(p4, F(10.0), KG_LJ, bonds=[[0, 1], [1, 2]], bp=KG_BP, terms=2)
+    assert np.allclose(fl, f8, rtol=1e-5, atol=1e-4) and np.allclose(vl, v8, rtol=1e-5, atol=1e-4)
+    ap = np.array([[100.0, F(np.pi * 120 / 180)]], F)
+    p4 = pos4_of([[3.0, 5.0, 5.0], [5.0, 5.0, 5.0], [5.0, 7.0, 5.0]])
+    fl, vl = orc.angle_literal(p4, [[0, 1, 2, 0]], ap, F(10.0))
+    f8, v8 = orc.forces_f64(p4, F(10.0), KG_LJ, ang4=[[0, 1, 2, 0]], ap=ap, terms=4)
+    assert np.allclose(fl, f8, rtol=1e-5, atol=1e-4)
+
+
+# --- Nose-Hoover chain
+def test_nh_chain_f32_literal_matches_f64(orc, golden):
+    """tests/test_nose_hoover.cu:343-439 style: 10000 KE draws, FP32 literal
+    chain (nh_expf Taylor) against the FP64 chain, state carried across trials."""
+    N, M, Td, T, dt = 100, 3, 0.5, 1.5, 0.002
+    Nf = 3 * N
+    Q1, Qr = Nf * T * Td * Td, T * Td * Td
+    xi32 = np.zeros(10); v32 = np.zeros(10)
+    xi64 = np.zeros(10); v64 = np.zeros(10)
+    for ke in golden["nhke"][:2000]:
+        s32, xi32, v32 = orc.nh_chain(0, M, Q1, Qr, xi32, v32, float(ke), T, N, dt / 2)
+        s64, xi64, v64 = orc.nh_chain(1, M, Q1, Qr, xi64, v64, float(ke), T, N, dt / 2)
+        assert abs(s32 - s64) < 1e-6
+    assert np.allclose(v32[:M], v64[:M], atol=1e-4)
+
+
+def test_nh_chain_single_thermostat_closed_form(orc):
+    # M = 1: v += dt4*G twice around exp(-dt2 v) with G recomputed from the scaled KE
+    Nf, T, KE, Q, h = 30.0, 1.0, 20.0, 3.0, 0.01
+    s, xi, v = orc.nh_chain(1, 1, Q, Q, np.zeros(10), np.zeros(10), KE, T, 10, h)
+    g0 = (2 * KE - Nf * T) / Q
+    v1 = g0 * h / 2
+    s_ref = np.exp(-h * v1)
+    g1 = (2 * KE * s_ref ** 2 - Nf * T) / Q
+    assert abs(s - s_ref) < 1e-15 and abs(v[0] - (v1 + g1 * h / 2)) < 1e-15 and abs(xi[0] - h * v1) < 1e-15
+
+
+def test_oracle_md_nve_conserves_energy(orc, golden):
+    # the oracle driver's NVE limit on tests/test_integrator.cu's lattice
+    sysd = {"pos4": pos4_of(golden["nve64_xyz"]), "vel3": golden["nve64_vel"].copy(),
+            "img3": np.zeros((64, 3), np.int32), "masses": np.ones(16, F), "lj": KG_LJ, "L": 8.0,
+            "skin": 0.5, "Tdamp": 1e10, "ensemble": 0}
+    _, _, _, _, th = orc.md_run(sysd, 300)
+    e = (th[:, 1] + th[:, 2]) * 64
+    assert np.max(np.abs(e - e[0])) < 0.01 * abs(e[0])
