@@ -89,6 +89,7 @@ class Engine {
     Replica* rep(int32_t sys);
 
   private:
+    int32_t create_impl(const fmd_system_desc* d, int32_t* sys, int kmult);
     int32_t hip_check(hipError_t e, const char* what);
     void* dalloc(Replica& r, size_t bytes, bool* ok);
     Batch* batch_for(const std::vector<int>& ids);

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "kernels.h"
@@ -93,7 +94,7 @@ static int cbits_for(int nx) {
     return cb;
 }
 
-static Sizing sizing(const fmd_system_desc* d, int bcap, int acap) {
+static Sizing sizing(const fmd_system_desc* d, int bcap, int acap, int kmult = 1) {
     Sizing s;
     s.natoms = d->natoms;
     s.nslots = ceil64(d->natoms);
@@ -103,7 +104,7 @@ static Sizing sizing(const fmd_system_desc* d, int bcap, int acap) {
     // capacity rule of src/neighbor/verlet_list.cu:4-7: max(256, 3 rho 4/3 pi rcs^3);
     // only the filled rows are streamed, so the reserve costs memory, not bandwidth
     const double expect = rho * 4.0 / 3.0 * M_PI * rcs * rcs * rcs;
-    int kc = std::max(256, (int)(3.0 * expect));
+    int kc = std::max(256, (int)(3.0 * expect)) * kmult;
     kc = (kc + 7) / 8 * 8;
     if (kc > s.nslots) kc = s.nslots;  // cannot have more than N-1 neighbors
     if (kc < 8) kc = 8;
@@ -186,12 +187,20 @@ uint64_t system_bytes(const fmd_system_desc* d) {
 // create / destroy
 // ---------------------------------------------------------------------------
 int32_t Engine::create(const fmd_system_desc* d, int32_t* sys) {
+    // an initial configuration denser than the list capacity rule allows is
+    // created again with a larger capacity instead of being truncated
+    int32_t rc = FMD_ERR_NBR_OVERFLOW;
+    for (int kmult = 1; kmult <= 4 && rc == FMD_ERR_NBR_OVERFLOW; kmult *= 2) rc = create_impl(d, sys, kmult);
+    return rc;
+}
+
+int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     std::string msg;
     if (int32_t rc = validate(d, msg)) return fail(rc, msg);
     int bcap, acap;
     incidence_caps(d, &bcap, &acap);
     if (bcap > 255 || acap > 255) return fail(FMD_ERR_ARG, "more than 255 bonds or angles on one atom");
-    Sizing S = sizing(d, bcap, acap);
+    Sizing S = sizing(d, bcap, acap, kmult);
     if (in_use_ + S.bytes > budget_)
         return fail(FMD_ERR_BUDGET, "system needs " + std::to_string(S.bytes) +
                                         " bytes; budget left " + std::to_string(budget_ - in_use_));
@@ -221,6 +230,7 @@ int32_t Engine::create(const fmd_system_desc* d, int32_t* sys) {
     D.c1 = std::exp(-(1.0f / d->Tdamp) * d->dt);  // src/integrate/langevin.cu:17-19
     D.c2 = std::sqrt(1.0f - D.c1 * D.c1);
     D.fcap = d->force_cap > 0.0f ? d->force_cap : 0.0f;
+    D.vlim = D.fcap > 0.0f ? 0.02f / d->dt : 0.0f;
     D.clamp_lj = 1;
     D.seed_lo = (uint32_t)d->seed;
     D.seed_hi = (uint32_t)(d->seed >> 32);
@@ -534,6 +544,13 @@ int32_t Engine::begin_run(const std::vector<int>& ids, int64_t nsteps) {
 int32_t Engine::launch_steps(const std::vector<int>& ids, int64_t k, bool use_graph) {
     Batch* b = batch_for(ids);
     if (!b) return FMD_ERR_ARG;
+    // FMD_NO_GRAPH=1 issues the same kernel sequence without graph capture
+    // (for tracing tools that do not follow graph launches)
+    static const bool no_graph = [] {
+        const char* v = std::getenv("FMD_NO_GRAPH");
+        return v && v[0] == '1';
+    }();
+    if (no_graph) use_graph = false;
     if (use_graph) {
         if (!b->graph || b->tune_version != tune_version_) {
             if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
@@ -656,6 +673,7 @@ int32_t Engine::set_force_cap(int32_t sys, float cap) {
     if (!r) return fail(FMD_ERR_ARG, "no such system");
     if (int32_t rc = sync()) return rc;
     r->desc.fcap = cap > 0.0f ? cap : 0.0f;
+    r->desc.vlim = r->desc.fcap > 0.0f ? 0.02f / r->desc.dt : 0.0f;
     return hip_check(hipMemcpy(d_descs_ + sys, &r->desc, sizeof(RepDesc), hipMemcpyHostToDevice), "upload desc");
 }
 
@@ -939,7 +957,8 @@ int32_t Engine::autotune(const int32_t* ids, int32_t n, int reps) {
     if (!b) return FMD_ERR_ARG;
     Launch l = launch_of(*b);
     if (reps < 1) reps = 1;
-    const int cand[4] = {64, 128, 256, 512};
+    // block shapes the kernels are compiled for (__launch_bounds__(256))
+    const int cand[3] = {64, 128, 256};
     hipEvent_t e0, e1;
     if (int32_t rc = hip_check(hipEventCreate(&e0), "event")) return rc;
     if (int32_t rc = hip_check(hipEventCreate(&e1), "event")) return rc;
@@ -949,7 +968,7 @@ int32_t Engine::autotune(const int32_t* ids, int32_t n, int reps) {
         for (int kern = 0; kern < 2 && !rc; kern++) {
             float best = 1e30f;
             int bestb = 256;
-            for (int bi = 0; bi < 4 && !rc; bi++) {
+            for (int bi = 0; bi < 3 && !rc; bi++) {
                 auto run = [&]() {
                     if (kern == 0) launch_force(l, b->cls_t0[c], b->cls_nt[c], cand[bi], FMD_TERM_ALL, 0, 1);
                     else launch_build(l, b->cls_t0[c], b->cls_nt[c], cand[bi], 1);
@@ -958,7 +977,8 @@ int32_t Engine::autotune(const int32_t* ids, int32_t n, int reps) {
                 (void)hipEventRecord(e0, stream_);
                 for (int k = 0; k < reps; k++) run();
                 (void)hipEventRecord(e1, stream_);
-                rc = hip_check(hipEventSynchronize(e1), "autotune sync");
+                rc = hip_check(hipGetLastError(), "autotune launch");
+                if (!rc) rc = hip_check(hipEventSynchronize(e1), "autotune sync");
                 float ms = 0;
                 (void)hipEventElapsedTime(&ms, e0, e1);
                 if (ms < best) { best = ms; bestb = cand[bi]; }

@@ -91,6 +91,7 @@ struct RepDesc {
     float dt, half_dt;
     float c1, c2;          // Langevin: exp(-dt/Tdamp), sqrt(1-c1^2)
     float fcap;            // LJ pair force cap (0 = off)
+    float vlim;            // push-off speed limit (0 = off): |v| dt <= 0.02 sigma
     int clamp_lj;          // force/r clamp to +-1000 (src/force/lj.cu:48)
     uint32_t seed_lo, seed_hi;
     // per-slot state (slot = position in the cell-sorted order)

@@ -366,14 +366,66 @@ void build_exclusions(int n, const std::vector<int32_t>& bonds, std::vector<int3
 
 // ---------------------------------------------------------------------------
 // Kremer-Grest melt generator.  Chains are random walks with bond length 0.97
-// from a uniform random start; each new bead is rejected and redrawn while it
-// sits closer than 1.02 to the bead two back (no immediate back-folding).
-// Positions are wrapped into [0, L) with image counts.  Uniform variates are
-// the top 53 bits of mt19937_64.
+// from a uniform random start.  Each new bead is drawn up to 64 times and kept
+// at the first draw that is at least 1.02 from the bead two back and at least
+// 0.75 from every other bead placed so far (checked on a host cell grid); when
+// no draw qualifies the last one is kept.  Positions are wrapped into [0, L)
+// with image counts.  Uniform variates are the top 53 bits of mt19937_64.
 // ---------------------------------------------------------------------------
+namespace {
+struct HostGrid {
+    int n;
+    double cs, L;
+    std::vector<std::vector<int>> cells;
+    std::vector<double> xyz;  // wrapped coordinates of placed beads
+    HostGrid(double L_, double cell) : L(L_) {
+        n = std::max(1, (int)(L_ / cell));
+        cs = L_ / n;
+        cells.resize((size_t)n * n * n);
+    }
+    int cidx(double x) const {
+        int c = (int)std::floor(x / cs);
+        return ((c % n) + n) % n;
+    }
+    static double wrap(double x, double L) { return x - L * std::floor(x / L); }
+    // min distance^2 from p to placed beads other than `skip`, capped at cut^2
+    double min_d2(const double* p, int skip, double cut) const {
+        const int cx = cidx(p[0]), cy = cidx(p[1]), cz = cidx(p[2]);
+        const int r = std::min(n, (int)std::ceil(cut / cs));
+        double best = cut * cut;
+        for (int dz = -r; dz <= r; dz++)
+            for (int dy = -r; dy <= r; dy++)
+                for (int dx = -r; dx <= r; dx++) {
+                    if (n < 2 * r + 1 && (dx != 0 || dy != 0 || dz != 0) &&
+                        (std::abs(dx) > n / 2 || std::abs(dy) > n / 2 || std::abs(dz) > n / 2))
+                        continue;
+                    const auto& c = cells[(((cz + dz) % n + n) % n * n + ((cy + dy) % n + n) % n) * n +
+                                          ((cx + dx) % n + n) % n];
+                    for (int j : c) {
+                        if (j == skip) continue;
+                        double d2 = 0;
+                        for (int a = 0; a < 3; a++) {
+                            double d = p[a] - xyz[3 * (size_t)j + a];
+                            d -= L * std::rint(d / L);
+                            d2 += d * d;
+                        }
+                        if (d2 < best) best = d2;
+                    }
+                }
+        return best;
+    }
+    void add(const double* p) {
+        int id = (int)(xyz.size() / 3);
+        for (int a = 0; a < 3; a++) xyz.push_back(p[a]);
+        cells[((size_t)cidx(p[2]) * n + cidx(p[1])) * n + cidx(p[0])].push_back(id);
+    }
+};
+}  // namespace
+
 int32_t kg_melt(int nchains, int chain_len, double rho, uint64_t seed, Topo& t) {
     if (nchains < 1 || chain_len < 1 || !(rho > 0.0))
         return fail(FMD_ERR_ARG, "kg_melt: need nchains >= 1, chain_len >= 1, rho > 0");
+    if ((int64_t)nchains * chain_len > (1 << 26)) return fail(FMD_ERR_ARG, "kg_melt: too many beads");
     std::mt19937_64 rng(seed);
     auto uni = [&]() { return (double)(rng() >> 11) * (1.0 / 9007199254740992.0); };
     const int n = nchains * chain_len;
@@ -394,26 +446,34 @@ int32_t kg_melt(int nchains, int chain_len, double rho, uint64_t seed, Topo& t) 
     t.image3.assign(3 * (size_t)n, 0);
     t.mol.resize(n);
     const float Lf = (float)L;
+    const double dmin = 0.75, dback = 1.02;
+    HostGrid grid(L, 1.0);
     std::vector<double> un(3 * (size_t)chain_len);  // unwrapped chain coordinates
     for (int c = 0; c < nchains; c++) {
         for (int b = 0; b < chain_len; b++) {
             double* p = &un[3 * b];
-            if (b == 0) {
-                for (int d = 0; d < 3; d++) p[d] = uni() * L;
-            } else {
-                const double* q = &un[3 * (b - 1)];
-                for (int tries = 0;; tries++) {
+            const int prev = c * chain_len + b - 1;
+            for (int tries = 0; tries < 64; tries++) {
+                if (b == 0) {
+                    for (int d = 0; d < 3; d++) p[d] = uni() * L;
+                } else {
+                    const double* q = &un[3 * (b - 1)];
                     double z = 2.0 * uni() - 1.0, ph = 2.0 * M_PI * uni();
                     double s = std::sqrt(std::max(0.0, 1.0 - z * z));
                     p[0] = q[0] + 0.97 * s * std::cos(ph);
                     p[1] = q[1] + 0.97 * s * std::sin(ph);
                     p[2] = q[2] + 0.97 * z;
-                    if (b < 2 || tries > 100) break;
+                }
+                if (b >= 2) {
                     const double* q2 = &un[3 * (b - 2)];
                     double dx = p[0] - q2[0], dy = p[1] - q2[1], dz = p[2] - q2[2];
-                    if (dx * dx + dy * dy + dz * dz >= 1.02 * 1.02) break;
+                    if (dx * dx + dy * dy + dz * dz < dback * dback) continue;
                 }
+                double w[3] = {HostGrid::wrap(p[0], L), HostGrid::wrap(p[1], L), HostGrid::wrap(p[2], L)};
+                if (grid.min_d2(w, b > 0 ? prev : -1, dmin) >= dmin * dmin) break;
             }
+            double w[3] = {HostGrid::wrap(p[0], L), HostGrid::wrap(p[1], L), HostGrid::wrap(p[2], L)};
+            grid.add(w);
             int i = c * chain_len + b;
             for (int d = 0; d < 3; d++) {
                 double im = std::floor(p[d] / L);

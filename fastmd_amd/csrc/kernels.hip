@@ -42,6 +42,9 @@ __device__ inline int wave_or(int x) {
     return x;
 }
 
+constexpr int kMaxBlockK = 256;  // launch bound of k_build and k_force
+constexpr int kLJChunk = 8;      // neighbors per software-pipelined chunk (kcap % 8 == 0)
+
 #pragma clang fp contract(off)
 __device__ inline float sq3(float x, float y, float z) { return x * x + y * y + z * z; }
 #pragma clang fp contract(fast)
@@ -159,6 +162,17 @@ __device__ void start_step(const RepDesc& d, RepState* st, int64_t s) {
     st->pre_ready = 1;
 }
 
+// Push-off speed limit (the displacement cap of a soft push-off): rescales v
+// so that |v| <= vlim.
+__device__ inline float4 limit_speed(float4 v, float vlim) {
+    const float s2 = v.x * v.x + v.y * v.y + v.z * v.z;
+    if (s2 > vlim * vlim) {
+        const float f = vlim * rsqrtf(s2);
+        v.x *= f; v.y *= f; v.z *= f;
+    }
+    return v;
+}
+
 // ---------------------------------------------------------------------------
 // pre-force: kick, drift, (O), wrap, image, skin displacement
 // (src/integrate/langevin.cu:34-94, src/integrate/nose_hoover.cu:606-757,
@@ -185,10 +199,12 @@ __global__ void __launch_bounds__(256) k_pre(const RepDesc* __restrict__ descs,
         const float L = st->L, invL = st->invL;
         if (d.ensemble == 0) {
             v.x += hdtm * f.x; v.y += hdtm * f.y; v.z += hdtm * f.z;
+            if (d.vlim > 0.0f) v = limit_speed(v, d.vlim);
             r.x += d.half_dt * v.x; r.y += d.half_dt * v.y; r.z += d.half_dt * v.z;
             const float3 z = normal3(d.seed_lo, d.seed_hi, (uint32_t)im.w, st->step + 1);
             const float ns = d.c2 * sqrtf(st->kT / m);
             v.x = d.c1 * v.x + ns * z.x; v.y = d.c1 * v.y + ns * z.y; v.z = d.c1 * v.z + ns * z.z;
+            if (d.vlim > 0.0f) v = limit_speed(v, d.vlim);
             r.x += d.half_dt * v.x; r.y += d.half_dt * v.y; r.z += d.half_dt * v.z;
         } else {
             const float sc = st->vscale;
@@ -403,7 +419,7 @@ __global__ void __launch_bounds__(256) k_gather(const RepDesc* __restrict__ desc
 // kernel streams one contiguous 256-byte row per neighbor index.  A count above
 // kcap is reported as ERR_NBR_OVERFLOW, never silently truncated.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_build(const RepDesc* __restrict__ descs,
+__global__ void __launch_bounds__(kMaxBlockK) k_build(const RepDesc* __restrict__ descs,
                                                const Tile* __restrict__ tiles, int t0, int ntl,
                                                int always) {
     for (int w = t0 + wave_in_grid(); w < t0 + ntl; w += waves_in_grid()) {
@@ -497,7 +513,7 @@ __global__ void __launch_bounds__(256) k_build(const RepDesc* __restrict__ descs
 // FP64 partials per wave: pe, virial[6], kinetic tensor[6] (after the kick).
 // ---------------------------------------------------------------------------
 template <int TERMS>
-__global__ void __launch_bounds__(256) k_force(const RepDesc* __restrict__ descs,
+__global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict__ descs,
                                                const Tile* __restrict__ tiles, int t0, int ntl,
                                                int kick, int always) {
     constexpr int terms = TERMS;
@@ -521,33 +537,47 @@ __global__ void __launch_bounds__(256) k_force(const RepDesc* __restrict__ descs
             const int nn = d.nnbr[i];
             const int* nl = d.nbr + (size_t)t.lt * d.kcap * kTile + lane;
             const float2* ljrow = d.lj + ti * d.ntypes;
+            const bool one_type = d.ntypes == 1;
+            const float2 p0 = ljrow[0];
             const float rc2 = d.rc2;
             const bool clamp = d.clamp_lj != 0;
             const float fcap = d.fcap;
-            for (int k = 0; k < nn; k++) {
-                const int j = nl[k * kTile];
-                const float4 pj = d.pos[j];
-                const float dx = min_image_exact(pi.x, pj.x, L, invL);
-                const float dy = min_image_exact(pi.y, pj.y, L, invL);
-                const float dz = min_image_exact(pi.z, pj.z, L, invL);
-                const float r2 = sq3(dx, dy, dz);
-                if (r2 < rc2 && r2 > 1e-10f) {
-                    const float2 p = ljrow[type_of(pj.w)];
-                    const float r2inv = 1.0f / r2;
-                    const float sr2 = p.y * p.y * r2inv;
-                    const float sr6 = sr2 * sr2 * sr2;
-                    const float sr12 = sr6 * sr6;
-                    float fr = 24.0f * p.x * r2inv * (2.0f * sr12 - sr6);
-                    if (clamp) fr = fminf(fmaxf(fr, -1000.0f), 1000.0f);
-                    if (fcap > 0.0f) {
-                        const float cap = fcap * rsqrtf(r2);
-                        fr = fminf(fmaxf(fr, -cap), cap);
+            // Software-pipelined neighbor loop: kLJChunk index loads in flight,
+            // then kLJChunk position gathers in flight, then the pair math.
+            // Rows past this atom's count read the atom itself (r^2 = 0 is
+            // rejected), so every load stays inside the kcap rows of the tile.
+            for (int k0 = 0; k0 < nn; k0 += kLJChunk) {
+                int jj[kLJChunk];
+#pragma unroll
+                for (int u = 0; u < kLJChunk; u++) jj[u] = (k0 + u < nn) ? nl[(k0 + u) * kTile] : i;
+                float4 pjs[kLJChunk];
+#pragma unroll
+                for (int u = 0; u < kLJChunk; u++) pjs[u] = d.pos[jj[u]];
+#pragma unroll
+                for (int u = 0; u < kLJChunk; u++) {
+                    const float4 pj = pjs[u];
+                    const float dx = min_image_exact(pi.x, pj.x, L, invL);
+                    const float dy = min_image_exact(pi.y, pj.y, L, invL);
+                    const float dz = min_image_exact(pi.z, pj.z, L, invL);
+                    const float r2 = sq3(dx, dy, dz);
+                    if (r2 < rc2 && r2 > 1e-10f) {
+                        const float2 p = one_type ? p0 : ljrow[type_of(pj.w)];
+                        const float r2inv = 1.0f / r2;
+                        const float sr2 = p.y * p.y * r2inv;
+                        const float sr6 = sr2 * sr2 * sr2;
+                        const float sr12 = sr6 * sr6;
+                        float fr = 24.0f * p.x * r2inv * (2.0f * sr12 - sr6);
+                        if (clamp) fr = fminf(fmaxf(fr, -1000.0f), 1000.0f);
+                        if (fcap > 0.0f) {
+                            const float cap = fcap * rsqrtf(r2);
+                            fr = fminf(fmaxf(fr, -cap), cap);
+                        }
+                        const float hx = 0.5f * fr * dx, hy = 0.5f * fr * dy, hz = 0.5f * fr * dz;
+                        fx += fr * dx; fy += fr * dy; fz += fr * dz;
+                        pe += 2.0f * p.x * (sr12 - sr6);
+                        w0 += hx * dx; w1 += hx * dy; w2 += hx * dz;
+                        w3 += hy * dy; w4 += hy * dz; w5 += hz * dz;
                     }
-                    const float hx = 0.5f * fr * dx, hy = 0.5f * fr * dy, hz = 0.5f * fr * dz;
-                    fx += fr * dx; fy += fr * dy; fz += fr * dz;
-                    pe += 2.0f * p.x * (sr12 - sr6);
-                    w0 += hx * dx; w1 += hx * dy; w2 += hx * dz;
-                    w3 += hy * dy; w4 += hy * dz; w5 += hz * dz;
                 }
             }
         }
@@ -807,8 +837,14 @@ void launch_control(const Launch& l, int op, int64_t arg) {
     if (l.nr <= 0) return;
     k_control<<<cdiv(l.nr, 64), 64, 0, l.stream>>>(l.descs, l.reps, l.nr, op, arg);
 }
+// k_build and k_force are compiled for at most kMaxBlock threads per block;
+// any other request runs at the default shape.
+static constexpr int kMaxBlock = kMaxBlockK;
+static inline int valid_block(int b) { return (b == 64 || b == 128 || b == 256) ? b : kMaxBlock; }
+
 void launch_build(const Launch& l, int t0, int nt, int block, int always) {
     if (nt <= 0) return;
+    block = valid_block(block);
     const int wpb = block / 64;
     int g = cdiv(nt, wpb);
     if (g > kStrideBlocks) g = kStrideBlocks;
@@ -827,6 +863,7 @@ void launch_rebuild(const Launch& l, int always, int build_block) {
 }
 void launch_force(const Launch& l, int t0, int nt, int block, int terms, int kick, int always) {
     if (nt <= 0) return;
+    block = valid_block(block);
     const int g = cdiv(nt, block / 64);
     switch (terms & 7) {
         case 1: k_force<1><<<g, block, 0, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;

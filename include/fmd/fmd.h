@@ -80,7 +80,8 @@ typedef struct fmd_system_desc {
     uint64_t seed;             /* Langevin noise seed                              */
     int64_t nsteps;            /* run length; denominator of the T/P ramps         */
     int32_t thermo_freq;       /* 0 = no thermo records                            */
-    float force_cap;           /* > 0: cap |F_pair| of LJ (soft push-off); 0 = off */
+    float force_cap;           /* > 0: soft push-off: cap |F_pair| of LJ and limit the */
+                               /* displacement per step to 0.02 sigma; 0 = off         */
 } fmd_system_desc;
 
 /* One thermo sample (src/analysis/thermo.cu:140-201): per-atom KE and PE,
