@@ -54,6 +54,65 @@ def cpu_baseline(engine, sids, specs, target_s=15.0):
                       f"({int(done_bs)} bead-steps, {t_used:.1f} s) with the OpenMP cell-list restatement"}
 
 
+def served_stream(eng, sids, descs, rate, tuned, window_s, steps, log_every=10.0):
+    """Poisson stream of simulation jobs through the continuous-batching
+    server: jobs start from the relaxed replicas (picked uniformly), run a
+    log-uniform number of steps, 70% Langevin / 30% Nose-Hoover NVT, T in
+    U(0.8, 1.2), at 80% of the measured batched rate.  Returns latency
+    percentiles and the aggregate rate."""
+    import numpy as np
+
+    import fastmd_amd as fmd
+    from fastmd_amd.workloads import job_stream, state_desc
+
+    pool = []
+    for s, d in zip(sids, descs):
+        p, v, im = eng.state(s)
+        pool.append((d, p, v, im, eng.box(s)))
+    lam, jobs = job_stream([eng.natoms[s] for s in sids], rate, window_s, steps=steps)
+    srv = fmd.Server(0)
+    srv.set_blocks(tuned)
+    jd = []
+    t_start = time.perf_counter()
+    next_log = log_every
+    for j in jobs:
+        d, p, v, im, L = pool[j["pool"]]
+        ens = fmd.NVT_NH if j["nvt"] else fmd.LANGEVIN
+        desc = state_desc(d, p, v, im, L, ensemble=ens, T=j["T"], Tdamp=0.1 if j["nvt"] else 1.0,
+                          seed=j["seed"], nsteps=j["steps"])
+        wait = j["t"] - (time.perf_counter() - t_start)
+        if wait > 0:
+            time.sleep(wait)
+        jd.append((srv.submit(desc, j["steps"]), len(p), j["steps"]))
+        el = time.perf_counter() - t_start
+        if el >= next_log:
+            st = srv.stats()
+            log(f"serve t={el:.0f}s submitted {len(jd)} resident {st['resident']} queued {st['queued']}")
+            next_log += log_every
+    srv.drain()
+    t_all = time.perf_counter() - t_start
+    lat, bs, failed = [], 0.0, 0
+    t_first = t_last = None
+    for jid, n, k in jd:
+        r = srv.result(jid)
+        if r["status"] != 0:
+            failed += 1
+            continue
+        lat.append(r["t_done"] - r["t_submit"])
+        bs += float(n) * k
+        t_first = r["t_submit"] if t_first is None else min(t_first, r["t_submit"])
+        t_last = r["t_done"] if t_last is None else max(t_last, r["t_done"])
+    st = srv.stats()
+    srv.close()
+    lat = np.asarray(lat)
+    return {"jobs": len(jd), "failed": failed, "rate_jobs_per_s": lam, "window_s": window_s,
+            "steps_range": list(steps), "p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
+            "p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
+            "mean_s": float(lat.mean()) if len(lat) else None,
+            "bead_steps_per_s": bs / (t_last - t_first) if lat.size else None,
+            "graph_launches": st["graph_launches"], "elapsed_s": t_all}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,6 +122,10 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--serve-window", type=float, default=60.0,
+                    help="seconds of Poisson arrivals for the served-job leg (0 = skip)")
+    ap.add_argument("--serve-steps", type=int, nargs=2, default=(250, 25000),
+                    help="log-uniform job run length range for the served-job leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: independent replicas shard across GPUs with no exchange, "
@@ -75,7 +138,8 @@ def main():
     eng = fmd.Engine(0)
     specs = workloads.uneven_replica_specs(args.replicas)
     t0 = time.perf_counter()
-    sids = workloads.relaxed_replicas(eng, specs, log=log)
+    descs = []
+    sids = workloads.relaxed_replicas(eng, specs, log=log, descs=descs)
     natoms = [eng.natoms[s] for s in sids]
     total = int(sum(natoms))
     log(f"{len(sids)} replicas, {total} beads, relaxed in {time.perf_counter() - t0:.1f} s")
@@ -106,6 +170,12 @@ def main():
     log(f"LJ kernel {ms_lj:.4f} ms, {bytes_lj / 1e6:.1f} MB alg -> {ach:.0f} GB/s; "
         f"fused force {ms_all:.4f} ms -> {ach_all:.0f} GB/s")
     rebuilds = sum(eng.rebuilds(s) for s in sids)
+
+    served = None
+    if args.serve_window > 0:
+        served = served_stream(eng, sids, descs, batched, tuned, args.serve_window, tuple(args.serve_steps))
+        log(f"served: {served['jobs']} jobs at {served['rate_jobs_per_s']:.2f}/s, p50 {served['p50_s']:.3f} s, "
+            f"p99 {served['p99_s']:.3f} s, {served['bead_steps_per_s']:.4g} bead-steps/s")
 
     cpu = None if args.no_cpu else cpu_baseline(eng, sids, specs)
     if cpu:
@@ -138,7 +208,7 @@ def main():
         "detail": {"plain_bead_steps_per_s": plain, "batched_over_plain": batched / plain,
                    "plain_steps_per_replica": args.plain_steps,
                    "fused_force_ms": ms_all, "fused_force_alg_bytes": bytes_all, "fused_force_GBs": ach_all,
-                   "neighbor_rebuilds_total": rebuilds, "autotune": {f"{k[0]}:{k[1]}": v for k, v in tuned.items()}},
+                   "neighbor_rebuilds_total": rebuilds, "served": served, "autotune": {f"{k[0]}:{k[1]}": v for k, v in tuned.items()}},
     }
     print(json.dumps(out), flush=True)
     eng.close()

@@ -118,6 +118,7 @@ SIGNATURES = [
     ("fmd_server_now", C.c_double, [C.c_void_p]),
     ("fmd_server_stats", C.c_int32, [C.c_void_p, _dp, C.POINTER(C.c_int64), _ip, _ip]),
     ("fmd_server_destroy", None, [C.c_void_p]),
+    ("fmd_server_set_block", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     ("fmd_admission_replay", C.c_int32, [C.c_int32, _dp, C.POINTER(C.c_uint64), _dp, C.c_uint64,
                                          C.c_int32, _dp, _dp]),
 ]
@@ -474,6 +475,13 @@ class Server:
         if state:
             out["pos4"], out["vel"] = p, v
         return out
+
+    def set_blocks(self, tuned):
+        """tuned: {(kernel, size_class): block}, e.g. from Engine.autotune."""
+        for (kern, cls), b in tuned.items():
+            rc = lib().fmd_server_set_block(self.h, kern, cls, b)
+            if rc:
+                raise FmdError(rc, f"server block ({kern}, {cls}) = {b}")
 
     def drain(self):
         lib().fmd_server_drain(self.h)

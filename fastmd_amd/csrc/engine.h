@@ -85,6 +85,8 @@ class Engine {
     int32_t time_force(const int32_t* ids, int32_t n, uint32_t terms, int reps, double* ms,
                        double* bytes, int64_t* pairs);
     int tuned(int kernel, int cls) const { return tuned_[kernel][cls]; }
+    // Sets one block shape (kernel 0 = force, 1 = list build); 64, 128 or 256.
+    int32_t set_tuned(int kernel, int cls, int block);
     hipStream_t stream() const { return stream_; }
     Replica* rep(int32_t sys);
 

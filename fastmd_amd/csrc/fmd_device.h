@@ -55,6 +55,7 @@ struct RepState {
     int maxdisp_bits;      // max |r - r_ref|^2 since the last rebuild (float bits)
     int err;               // ErrBits
     int pre_ready;         // start-of-step scalars for step+1 are computed
+    int want_obs;          // the current step records PE and virial (NPT or thermo step)
     int64_t nrebuilds;
     // current box and cell grid (float copies for per-atom kernels)
     float L, invL;
@@ -75,6 +76,7 @@ struct RepState {
     // last reductions (after the post-force kick and scaling)
     double pe, vir[6], kin[6];
     int64_t rec_step;      // step of the last reduction
+    int64_t obs_step;      // step of the last PE / virial evaluation
     // thermo ring
     int thermo_freq;
     int thermo_cap;
@@ -99,6 +101,7 @@ struct RepDesc {
     float4* vel;
     float4* force;
     float4* pos_ref;
+    int4* ipos;            // fixed-point copy of pos for the force path (x, y, z, type bits)
     int4* img;             // image counts x, y, z and the global id in w
     float4* tmp_pos;
     float4* tmp_vel;
@@ -150,15 +153,19 @@ __device__ inline float dist2_ref(float4 a, float4 b, float L, float invL) {
     float dz = min_image_ref(a.z - b.z, L, invL);
     return dx * dx + dy * dy + dz * dz;
 }
-// Force-path displacement: exact difference a-b (TwoSum) folded into the
-// minimum image and rounded once, so the FP32 displacement carries no error
-// from positions near opposite box faces.
-__device__ inline float min_image_exact(float a, float b, float L, float invL) {
-    float s = a - b;
-    float bb = s - a;
-    float err = (a - (s - bb)) + (-b - bb);
-    float m = rintf(s * invL);
-    return (s - L * m) + err;
+// Force-path coordinates: each axis of a position is kept as an unsigned
+// 32-bit fixed-point number on a grid of 2^32 steps over the box,
+// X = floor(x * 2^32 / L) mod 2^32 (FP64 product).  The difference of two such
+// numbers in two's complement is the minimum-image displacement, exact on the
+// grid; it is converted to FP32 and scaled by L (one rounding each).  The grid
+// step L / 2^32 is 1e-8 sigma for a 40-sigma box.
+__device__ inline unsigned fix_coord(float x, double scale) {
+    return (unsigned)(long long)floor((double)x * scale);
+}
+// ulen = L * 2^-32 is exact (power-of-two scaling), so (float)d * ulen rounds
+// once, identically to ((float)d * L) * 2^-32.
+__device__ inline float fix_diff(unsigned a, unsigned b, float ulen) {
+    return (float)(int)(a - b) * ulen;
 }
 
 // Morton interleave of 10 bits (src/core/morton.cu:4-10)

@@ -42,6 +42,17 @@ __device__ inline int wave_or(int x) {
     return x;
 }
 
+// Global address-space views of descriptor pointers, so the hot loops issue
+// global_load instead of flat_load.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FMD_GLOBAL __attribute__((address_space(1)))
+#else
+#define FMD_GLOBAL
+#endif
+template <class T> __device__ inline const FMD_GLOBAL T* gview(const T* p) {
+    return (const FMD_GLOBAL T*)p;
+}
+
 constexpr int kMaxBlockK = 256;  // launch bound of k_build and k_force
 constexpr int kLJChunk = 8;      // neighbors per software-pipelined chunk (kcap % 8 == 0)
 
@@ -143,6 +154,7 @@ __device__ void start_step(const RepDesc& d, RepState* st, int64_t s) {
     float T = (float)st->T_start + ((float)st->T_stop - (float)st->T_start) * frac;
     float P = (float)st->P_start + ((float)st->P_stop - (float)st->P_start) * frac;
     const double hdt = 0.5 * (double)d.dt, dt = (double)d.dt;
+    st->want_obs = (d.ensemble == 2) || (st->thermo_freq > 0 && (s % st->thermo_freq) == 0);
     if (d.ensemble == 0) {
         st->kT = T;
         st->vscale = 1.0f;
@@ -231,6 +243,9 @@ __global__ void __launch_bounds__(256) k_pre(const RepDesc* __restrict__ descs,
         d.pos[i] = r;
         d.vel[i] = v;
         d.img[i] = im;
+        const double sc = 4294967296.0 / (double)L;
+        d.ipos[i] = make_int4((int)fix_coord(r.x, sc), (int)fix_coord(r.y, sc), (int)fix_coord(r.z, sc),
+                              __float_as_int(r.w));
     }
     const float m = wave_max(bad ? 0.0f : d2);
     const int b = wave_or(bad);
@@ -408,6 +423,9 @@ __global__ void __launch_bounds__(256) k_gather(const RepDesc* __restrict__ desc
             d.vel[i] = d.tmp_vel[src];
             d.img[i] = im;
             d.slot_of[im.w] = i;
+            const double sc = 4294967296.0 / (double)st->L;
+            d.ipos[i] = make_int4((int)fix_coord(p.x, sc), (int)fix_coord(p.y, sc), (int)fix_coord(p.z, sc),
+                                  __float_as_int(p.w));
         }
     }
 }
@@ -498,12 +516,95 @@ __global__ void __launch_bounds__(kMaxBlockK) k_build(const RepDesc* __restrict_
                 }
             }
             if (cnt > kcap) err |= ERR_NBR_OVERFLOW;
-            d.nnbr[i] = min(cnt, kcap);
+            const int stored = min(cnt, kcap);
+            // pad to a whole chunk with the atom itself (rejected by r^2 = 0), so
+            // the force kernel loads whole chunks without per-row guards
+            for (int k = stored; k < ((stored + kLJChunk - 1) / kLJChunk) * kLJChunk; k++) out[k * kTile] = i;
+            d.nnbr[i] = stored;
         }
         err = wave_or(err);
         if (err && lane_id() == 0) atomicOr(&st->err, err);
     }
 }
+
+// LJ over this atom's Verlet rows (src/force/lj.cu:27-63).  The rows are
+// padded to whole chunks of kLJChunk with the atom itself (r^2 = 0 is
+// rejected), so the loop loads whole chunks unconditionally; the next chunk's
+// indices are in flight while the current chunk's positions are gathered and
+// used.  The cutoff is applied by masking 1/r^2, so there is no per-pair
+// branch.  OBS adds the per-atom PE and virial (thermo steps and NPT), CAP the
+// push-off pair-force cap, ONE_TYPE a single (eps, sigma) pair.
+template <bool OBS, bool CAP, bool ONE_TYPE>
+__device__ __forceinline__ void lj_loop(const RepDesc& d, int lt, int lane, int i, int4 pi, float L,
+                                        float& fx, float& fy, float& fz, float& pe,
+                                        float& w0, float& w1, float& w2, float& w3, float& w4,
+                                        float& w5) {
+    const int nn = d.nnbr[i];
+    const FMD_GLOBAL int* nl = gview(d.nbr) + (size_t)lt * d.kcap * kTile + lane;
+    const FMD_GLOBAL int4* pos = gview(d.ipos);
+    const float2* ljrow = d.lj + pi.w * d.ntypes;
+    const float2 p0 = ljrow[0];
+    const float rc2 = d.rc2;
+    const float fcap = d.fcap;
+    const float ulen = L * 2.3283064365386963e-10f;
+    unsigned nxt[kLJChunk];
+    if (nn > 0) {
+#pragma unroll
+        for (int u = 0; u < kLJChunk; u++) nxt[u] = (unsigned)nl[u * kTile];
+    }
+    for (int k0 = 0; k0 < nn; k0 += kLJChunk) {
+        int4 pjs[kLJChunk];
+#pragma unroll
+        for (int u = 0; u < kLJChunk; u++) pjs[u] = pos[nxt[u]];
+        if (k0 + kLJChunk < nn) {
+#pragma unroll
+            for (int u = 0; u < kLJChunk; u++) nxt[u] = (unsigned)nl[(k0 + kLJChunk + u) * kTile];
+        }
+#pragma unroll
+        for (int u = 0; u < kLJChunk; u++) {
+            const int4 pj = pjs[u];
+            const float dx = fix_diff(pi.x, pj.x, ulen);
+            const float dy = fix_diff(pi.y, pj.y, ulen);
+            const float dz = fix_diff(pi.z, pj.z, ulen);
+            const float r2 = sq3(dx, dy, dz);
+            const bool in = (r2 < rc2) && (r2 > 1e-10f);
+            const float2 p = ONE_TYPE ? p0 : ljrow[pj.w];
+            const float r2inv = in ? __builtin_amdgcn_rcpf(r2) : 0.0f;
+            const float sr2 = p.y * p.y * r2inv;
+            const float sr6 = sr2 * sr2 * sr2;
+            const float sr12 = sr6 * sr6;
+            float fr = 24.0f * p.x * r2inv * (2.0f * sr12 - sr6);
+            fr = fminf(fmaxf(fr, -1000.0f), 1000.0f);  // src/force/lj.cu:48
+            if (CAP) {
+                const float cap = fcap * rsqrtf(r2);
+                fr = fminf(fmaxf(fr, -cap), cap);
+            }
+            fx += fr * dx; fy += fr * dy; fz += fr * dz;
+            if (OBS) {
+                const float hx = 0.5f * fr * dx, hy = 0.5f * fr * dy, hz = 0.5f * fr * dz;
+                pe += 2.0f * p.x * (sr12 - sr6);
+                w0 += hx * dx; w1 += hx * dy; w2 += hx * dz;
+                w3 += hy * dy; w4 += hy * dz; w5 += hz * dz;
+            }
+        }
+    }
+}
+
+#define LJ_ARGS d, t.lt, lane, i, pi, L, fx, fy, fz, pe, w0, w1, w2, w3, w4, w5
+__device__ __forceinline__ void lj_dispatch(const RepDesc& d, const Tile& t, int lane, int i, int4 pi,
+                                            float L, bool obs, float& fx, float& fy,
+                                            float& fz, float& pe, float& w0, float& w1, float& w2,
+                                            float& w3, float& w4, float& w5) {
+    const bool cap = d.fcap > 0.0f, one = d.ntypes == 1;
+    if (!cap && one) {
+        if (obs) lj_loop<true, false, true>(LJ_ARGS); else lj_loop<false, false, true>(LJ_ARGS);
+    } else if (!cap) {
+        if (obs) lj_loop<true, false, false>(LJ_ARGS); else lj_loop<false, false, false>(LJ_ARGS);
+    } else {
+        if (obs) lj_loop<true, true, false>(LJ_ARGS); else lj_loop<false, true, false>(LJ_ARGS);
+    }
+}
+#undef LJ_ARGS
 
 // ---------------------------------------------------------------------------
 // Force kernel: LJ over the Verlet list (src/force/lj.cu:27-63), FENE+WCA
@@ -530,68 +631,24 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
     double k0 = 0, k1 = 0, k2 = 0, k3 = 0, k4 = 0, k5 = 0;
     int err = 0;
     if (i < d.natoms) {
-        const float4 pi = d.pos[i];
-        const int ti = type_of(pi.w);
-        const float L = st->L, invL = st->invL;
-        if (terms & 1) {
-            const int nn = d.nnbr[i];
-            const int* nl = d.nbr + (size_t)t.lt * d.kcap * kTile + lane;
-            const float2* ljrow = d.lj + ti * d.ntypes;
-            const bool one_type = d.ntypes == 1;
-            const float2 p0 = ljrow[0];
-            const float rc2 = d.rc2;
-            const bool clamp = d.clamp_lj != 0;
-            const float fcap = d.fcap;
-            // Software-pipelined neighbor loop: kLJChunk index loads in flight,
-            // then kLJChunk position gathers in flight, then the pair math.
-            // Rows past this atom's count read the atom itself (r^2 = 0 is
-            // rejected), so every load stays inside the kcap rows of the tile.
-            for (int k0 = 0; k0 < nn; k0 += kLJChunk) {
-                int jj[kLJChunk];
-#pragma unroll
-                for (int u = 0; u < kLJChunk; u++) jj[u] = (k0 + u < nn) ? nl[(k0 + u) * kTile] : i;
-                float4 pjs[kLJChunk];
-#pragma unroll
-                for (int u = 0; u < kLJChunk; u++) pjs[u] = d.pos[jj[u]];
-#pragma unroll
-                for (int u = 0; u < kLJChunk; u++) {
-                    const float4 pj = pjs[u];
-                    const float dx = min_image_exact(pi.x, pj.x, L, invL);
-                    const float dy = min_image_exact(pi.y, pj.y, L, invL);
-                    const float dz = min_image_exact(pi.z, pj.z, L, invL);
-                    const float r2 = sq3(dx, dy, dz);
-                    if (r2 < rc2 && r2 > 1e-10f) {
-                        const float2 p = one_type ? p0 : ljrow[type_of(pj.w)];
-                        const float r2inv = 1.0f / r2;
-                        const float sr2 = p.y * p.y * r2inv;
-                        const float sr6 = sr2 * sr2 * sr2;
-                        const float sr12 = sr6 * sr6;
-                        float fr = 24.0f * p.x * r2inv * (2.0f * sr12 - sr6);
-                        if (clamp) fr = fminf(fmaxf(fr, -1000.0f), 1000.0f);
-                        if (fcap > 0.0f) {
-                            const float cap = fcap * rsqrtf(r2);
-                            fr = fminf(fmaxf(fr, -cap), cap);
-                        }
-                        const float hx = 0.5f * fr * dx, hy = 0.5f * fr * dy, hz = 0.5f * fr * dz;
-                        fx += fr * dx; fy += fr * dy; fz += fr * dz;
-                        pe += 2.0f * p.x * (sr12 - sr6);
-                        w0 += hx * dx; w1 += hx * dy; w2 += hx * dz;
-                        w3 += hy * dy; w4 += hy * dz; w5 += hz * dz;
-                    }
-                }
-            }
-        }
+        const int4 pi = d.ipos[i];
+        const int ti = pi.w;
+        const float L = st->L;
+        const float ulen = L * 2.3283064365386963e-10f;  // fixed-point step, L * 2^-32
+        if (terms & 1)
+            lj_dispatch(d, t, lane, i, pi, L, always || st->want_obs, fx, fy, fz, pe, w0, w1, w2, w3, w4,
+                        w5);
         const int hdr = (terms & 6) ? d.bhdr[i] : 0;
         if (terms & 2) {
             const int nb = hdr & 0xFF;
             const int* bt = d.bnd + (size_t)t.lt * d.bcap * kTile + lane;
             for (int b = 0; b < nb; b++) {
                 const int e = bt[b * kTile];
-                const float4 pj = d.pos[e & kSlotMask];
+                const int4 pj = d.ipos[e & kSlotMask];
                 const float4 bp = d.bparm[(unsigned)e >> kSlotBits];
-                const float dx = min_image_exact(pi.x, pj.x, L, invL);
-                const float dy = min_image_exact(pi.y, pj.y, L, invL);
-                const float dz = min_image_exact(pi.z, pj.z, L, invL);
+                const float dx = fix_diff(pi.x, pj.x, ulen);
+                const float dy = fix_diff(pi.y, pj.y, ulen);
+                const float dz = fix_diff(pi.z, pj.z, ulen);
                 float r2 = sq3(dx, dy, dz);
                 const float R02 = bp.y * bp.y;
                 // a bond at or beyond R0 is a failure of the run, except during the
@@ -623,17 +680,17 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
                 const int2 e = at[a * kTile];
                 const int role = (unsigned)e.x >> kSlotBits;
                 const float2 ap = d.aparm[(unsigned)e.y >> kSlotBits];
-                const float4 q1 = d.pos[e.x & kSlotMask], q2 = d.pos[e.y & kSlotMask];
+                const int4 q1 = d.ipos[e.x & kSlotMask], q2 = d.ipos[e.y & kSlotMask];
                 // angle atoms (A, B=center, C) from this atom's role
-                const float4 pa = (role == 0) ? pi : q1;
-                const float4 pb = (role == 1) ? pi : ((role == 0) ? q1 : q2);
-                const float4 pc = (role == 2) ? pi : q2;
-                const float ax = min_image_exact(pa.x, pb.x, L, invL);
-                const float ay = min_image_exact(pa.y, pb.y, L, invL);
-                const float az = min_image_exact(pa.z, pb.z, L, invL);
-                const float bx = min_image_exact(pc.x, pb.x, L, invL);
-                const float by = min_image_exact(pc.y, pb.y, L, invL);
-                const float bz = min_image_exact(pc.z, pb.z, L, invL);
+                const int4 pa = (role == 0) ? pi : q1;
+                const int4 pb = (role == 1) ? pi : ((role == 0) ? q1 : q2);
+                const int4 pc = (role == 2) ? pi : q2;
+                const float ax = fix_diff(pa.x, pb.x, ulen);
+                const float ay = fix_diff(pa.y, pb.y, ulen);
+                const float az = fix_diff(pa.z, pb.z, ulen);
+                const float bx = fix_diff(pc.x, pb.x, ulen);
+                const float by = fix_diff(pc.y, pb.y, ulen);
+                const float bz = fix_diff(pc.z, pb.z, ulen);
                 const float ra2 = ax * ax + ay * ay + az * az;
                 const float rb2 = bx * bx + by * by + bz * bz;
                 const float cxp = ay * bz - az * by, cyp = az * bx - ax * bz, czp = ax * by - ay * bx;
@@ -727,11 +784,13 @@ __global__ void __launch_bounds__(256) k_boundary(const RepDesc* __restrict__ de
     for (int c = 0; c < 6; c++) { W[c] = red[1 + c][0]; K[c] = red[7 + c][0]; }
     const double KE = 0.5 * (K[0] + K[3] + K[5]);
     if (mode == BND_REDUCE) {
+        st->obs_step = st->step;
         st->pe = pe;
         for (int c = 0; c < 6; c++) { st->vir[c] = W[c]; st->kin[c] = K[c] * (double)st->v_pending * (double)st->v_pending; }
         return;
     }
     if (mode == BND_PRIME) {
+        st->obs_step = st->step;
         st->pe = pe;
         for (int c = 0; c < 6; c++) { st->vir[c] = W[c]; st->kin[c] = K[c]; }
         st->ke_carry = KE;  // src/main.cu:373-386
@@ -763,11 +822,15 @@ __global__ void __launch_bounds__(256) k_boundary(const RepDesc* __restrict__ de
     }
     st->v_pending = (float)ks;
     const double ks2 = (double)st->v_pending * (double)st->v_pending;
-    st->pe = pe;
-    for (int c = 0; c < 6; c++) { st->vir[c] = W[c]; st->kin[c] = K[c] * ks2; }
+    for (int c = 0; c < 6; c++) st->kin[c] = K[c] * ks2;
+    if (st->want_obs) {
+        st->pe = pe;
+        for (int c = 0; c < 6; c++) st->vir[c] = W[c];
+        st->obs_step = cur;
+    }
     st->step = cur;
     st->rec_step = cur;
-    if (st->thermo_freq > 0 && (cur % st->thermo_freq) == 0 && st->thermo_cap > 0) {
+    if (st->want_obs && st->thermo_freq > 0 && (cur % st->thermo_freq) == 0 && st->thermo_cap > 0) {
         ThermoRec r;
         const double N = (double)d.natoms, Ld = (double)st->L, V = Ld * Ld * Ld;
         r.step = cur;

@@ -131,6 +131,13 @@ class Server {
         done_cv_.wait(g, [this] { return pending_ == 0; });
         return FMD_OK;
     }
+    int32_t set_block(int kernel, int cls, int block) {
+        if (kernel < 0 || kernel > 1 || cls < 0 || cls >= kNumClasses) return FMD_ERR_ARG;
+        if (block != 64 && block != 128 && block != 256) return FMD_ERR_ARG;
+        std::lock_guard<std::mutex> g(m_);
+        blocks_.push_back({kernel, cls, block});
+        return FMD_OK;
+    }
     void stats(double* bs, int64_t* gl, int32_t* res, int32_t* q) {
         std::lock_guard<std::mutex> g(m_);
         if (bs) *bs = bead_steps_;
@@ -158,6 +165,8 @@ class Server {
                 cv_.wait(g, [&] { return stop_ || !running.empty() || aq_.queued() > 0; });
                 if (stop_) break;
                 for (const AdmitItem& it : aq_.admit()) admitted.push_back(jobs_[it.job].get());
+                for (const BlockSet& b : blocks_) eng_.set_tuned(b.kernel, b.cls, b.block);
+                blocks_.clear();
             }
             // create admitted jobs on the device (outside the lock)
             for (Job* j : admitted) {
@@ -219,8 +228,10 @@ class Server {
         }
     }
 
+    struct BlockSet { int kernel, cls, block; };
     Engine eng_;
     int graph_steps_;
+    std::vector<BlockSet> blocks_;  // block shapes to apply before the next launch
     AdmissionQueue aq_;
     std::chrono::steady_clock::time_point t0_;
     std::mutex m_;
@@ -271,6 +282,9 @@ int32_t fmd_server_stats(const fmd_server* s, double* bead_steps, int64_t* graph
     if (!s) return FMD_ERR_ARG;
     s->s->stats(bead_steps, graph_launches, resident, queued);
     return FMD_OK;
+}
+int32_t fmd_server_set_block(fmd_server* s, int32_t kernel, int32_t size_class, int32_t block) {
+    return s ? s->s->set_block(kernel, size_class, block) : FMD_ERR_ARG;
 }
 void fmd_server_destroy(fmd_server* s) {
     if (!s) return;

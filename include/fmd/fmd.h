@@ -247,6 +247,11 @@ double fmd_server_now(const fmd_server* s);
 int32_t fmd_server_stats(const fmd_server* s, double* bead_steps, int64_t* graph_launches,
                          int32_t* resident, int32_t* queued);
 void fmd_server_destroy(fmd_server* s);
+/* Block shape the server's engine uses for (kernel 0 = force, 1 = list build,
+ * size class), e.g. copied from fmd_tuned_block of an autotuned engine; takes
+ * effect at the next batched launch.  No reference counterpart (the reference
+ * hard-codes 256-thread blocks, src/force/lj.cu:116-118). */
+int32_t fmd_server_set_block(fmd_server* s, int32_t kernel, int32_t size_class, int32_t block);
 
 /* Pure admission policy (no GPU): replays arrivals (t, bytes, duration) under a
  * byte budget and a job cap with FIFO admission when memory frees, as the

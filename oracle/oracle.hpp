@@ -48,15 +48,19 @@ inline float dist2_ref(const float* a, const float* b, float L, float invL) {
 }
 inline float wrap_ref(float x, float L, float invL) { return x - L * std::floor(x * invL); }
 
-// The build's force-path displacement: the exact difference a-b (Knuth TwoSum)
-// folded into the minimum image, rounded once.  Restated here bit-for-bit so the
-// oracle takes the same cutoff decisions as the device force kernel.
-inline float min_image_exact_f(float a, float b, float L, float invL) {
-    float s = a - b;
-    float bb = s - a;
-    float err = (a - (s - bb)) + (-b - bb);
-    float m = std::rint(s * invL);
-    return (s - L * m) + err;
+// The build's force-path displacement: each coordinate as a 32-bit fixed-point
+// number X = floor(x * 2^32 / L) mod 2^32 (FP64 product), the two's-complement
+// difference of two of them converted to FP32 and scaled by L.  Restated
+// bit-for-bit so the oracle takes the same LJ cutoff decisions as the device.
+inline uint32_t fix_coord(float x, double scale) {
+    return (uint32_t)(long long)std::floor((double)x * scale);
+}
+inline float fix_diff(uint32_t a, uint32_t b, float L) {
+    return (float)(int32_t)(a - b) * (L * 2.3283064365386963e-10f);  // L * 2^-32 is exact
+}
+inline float min_image_fixed_f(float a, float b, float L) {
+    const double sc = 4294967296.0 / (double)L;
+    return fix_diff(fix_coord(a, sc), fix_coord(b, sc), L);
 }
 // Same displacement carried in FP64 (the image shift m chosen as above).
 inline double min_image_exact_d(float a, float b, float L, float invL) {
@@ -347,7 +351,7 @@ inline void angle_literal_f32(const float* pos4, int n, const int* ang4, int nan
 // terms: bit 0 = LJ (src/force/lj.cu:27-63), bit 1 = FENE+WCA
 // (src/force/fene.cu:21-65), bit 2 = angle (src/force/angle.cu:22-79).
 // LJ pairs: i != j, not excluded, rf2 < rc^2 and rf2 > 1e-10 where rf2 is the
-// FP32 squared exact-difference distance; the force/r clamp of lj.cu:48 is
+// FP32 squared fixed-point distance of the device; the force/r clamp of lj.cu:48 is
 // applied when clamp_lj is set.  FENE r^2 is clamped to [1e-6, 0.9999 R0^2]
 // as in fene.cu:33 when clamp_fene is set.  Angles use theta = atan2(|a x b|, a.b),
 // mathematically identical to acos(clamp(cos)) of angle.cu:37-45.
@@ -367,9 +371,9 @@ struct ForceParams {
 inline void lj_pair_f64(const float* pi, const float* pj, const ForceParams& P, float L,
                         float invL, double* fi, double* pe_i, double* vir) {
     float rc2 = P.rc * P.rc;
-    float dxf = min_image_exact_f(pi[0], pj[0], L, invL);
-    float dyf = min_image_exact_f(pi[1], pj[1], L, invL);
-    float dzf = min_image_exact_f(pi[2], pj[2], L, invL);
+    float dxf = min_image_fixed_f(pi[0], pj[0], L);
+    float dyf = min_image_fixed_f(pi[1], pj[1], L);
+    float dzf = min_image_fixed_f(pi[2], pj[2], L);
     float rf2 = dxf * dxf + dyf * dyf + dzf * dzf;
     if (!(rf2 < rc2 && rf2 > 1e-10f)) return;
     double dx = min_image_exact_d(pi[0], pj[0], L, invL);

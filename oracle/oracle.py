@@ -44,8 +44,8 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.orc_min_image.restype = C.c_float
         L.orc_min_image.argtypes = [C.c_float] * 3
-        L.orc_min_image_exact.restype = C.c_float
-        L.orc_min_image_exact.argtypes = [C.c_float] * 4
+        L.orc_min_image_fixed.restype = C.c_float
+        L.orc_min_image_fixed.argtypes = [C.c_float] * 3
         L.orc_wrap.restype = C.c_float
         L.orc_wrap.argtypes = [C.c_float] * 3
         L.orc_morton3d.restype = C.c_uint32

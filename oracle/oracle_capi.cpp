@@ -9,9 +9,7 @@ using namespace orc;
 extern "C" {
 
 float orc_min_image(float dr, float L, float invL) { return min_image_ref(dr, L, invL); }
-float orc_min_image_exact(float a, float b, float L, float invL) {
-    return min_image_exact_f(a, b, L, invL);
-}
+float orc_min_image_fixed(float a, float b, float L) { return min_image_fixed_f(a, b, L); }
 float orc_wrap(float x, float L, float invL) { return wrap_ref(x, L, invL); }
 uint32_t orc_morton3d(float x, float y, float z, float invL) { return morton3d(x, y, z, invL); }
 

@@ -173,6 +173,28 @@ def test_uneven_workload_specs(fmd):
     assert [fmd.size_class(b) for b in (2000, 8191, 8192, 40000)] == [0, 0, 1, 2]
 
 
+def test_served_job_stream(fmd):
+    """Poisson stream at 80% of a given rate: E[steps] of log-uniform [a, b]
+    is (b - a) / ln(b / a), so the job rate follows in closed form."""
+    from fastmd_amd import workloads
+    lam, jobs = workloads.job_stream([1000, 3000], 1e9, 200.0, steps=(1000, 100000))
+    e_steps = 99000 / np.log(100.0)
+    assert lam == pytest.approx(0.8e9 / (2000 * e_steps), rel=1e-12)
+    t = np.array([j["t"] for j in jobs])
+    assert np.all(np.diff(t) > 0) and t[-1] < 200.0
+    n = len(jobs)
+    assert abs(n - lam * 200) < 5 * np.sqrt(lam * 200)
+    st = np.array([j["steps"] for j in jobs])
+    assert st.min() >= 1000 and st.max() <= 100000
+    assert abs(np.log(st).mean() - np.log(1e4)) < 0.1  # log-uniform: mean log = midpoint
+    nvt = np.mean([j["nvt"] for j in jobs])
+    assert abs(nvt - 0.3) < 5 * np.sqrt(0.21 / n)
+    assert {j["pool"] for j in jobs} == {0, 1}
+    assert all(0.8 <= j["T"] <= 1.2 for j in jobs)
+    _, again = workloads.job_stream([1000, 3000], 1e9, 200.0, steps=(1000, 100000))
+    assert again == jobs  # seeded
+
+
 def test_system_bytes_budget_unit(fmd):
     t = fmd.Topo.kg_melt(10, 100, 0.85, 1)
     b = fmd.system_bytes(t)

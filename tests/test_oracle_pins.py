@@ -50,17 +50,26 @@ def test_wrap_lands_in_box(orc):
         assert abs(((x - w) / 10.0) - round((x - w) / 10.0)) < 1e-6
 
 
-def test_min_image_exact_is_correctly_rounded(orc):
-    rng = np.random.default_rng(0)
-    L = F(33.0)
-    a = (rng.random(2000) * L).astype(F)
-    b = (rng.random(2000) * L).astype(F)
+@pytest.mark.parametrize("L", [8.0, 33.0, 105.567])
+def test_fixed_point_displacement(orc, L):
+    """The force-path displacement is the minimum-image difference of positions
+    quantised to L/2^32, rounded twice in FP32: within one grid step plus 2 ulp
+    of the exact minimum-image difference, and antisymmetric."""
+    rng = np.random.default_rng(int(L))
+    L = F(L)
+    a = (rng.random(3000) * L).astype(F)
+    b = (rng.random(3000) * L).astype(F)
+    a[:4] = [0.0, np.nextafter(L, 0, dtype=F), F(1e-7), L / 2]
+    b[:4] = [np.nextafter(L, 0, dtype=F), 0.0, L - F(1e-7), 0.0]
+    q = float(L) / 2 ** 32
     for x, y in zip(a, b):
-        got = orc.lib().orc_min_image_exact(x, y, L, F(1) / L)
-        s = F(x - y)
-        m = np.rint(F(s * (F(1) / L)))
-        exact = float(x) - float(y) - float(L) * float(m)
-        assert got == F(exact)
+        got = orc.lib().orc_min_image_fixed(x, y, L)
+        d = float(x) - float(y)
+        d -= float(L) * np.rint(d / float(L))
+        if abs(abs(d) - float(L) / 2) < 1e-4:
+            continue  # the two images are equally near
+        assert abs(got - d) <= 2 * q + 2 * abs(d) * 2 ** -24
+        assert orc.lib().orc_min_image_fixed(y, x, L) == -got or abs(got) < 3 * q
 
 
 # --- Morton keys: decode the interleave bit by bit and compare with the quantised coordinates
