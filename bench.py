@@ -113,6 +113,20 @@ def served_stream(eng, sids, descs, rate, tuned, window_s, steps, log_every=10.0
             "graph_launches": st["graph_launches"], "elapsed_s": t_all}
 
 
+def calibrate_copy(mib):
+    """A plain copy of known size (read mib MiB, write mib MiB) for scaling
+    the HBM byte counters of a profiler against algorithmic bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read": n * 4, "bytes_written": n * 4, "launches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,7 +140,12 @@ def main():
                     help="seconds of Poisson arrivals for the served-job leg (0 = skip)")
     ap.add_argument("--serve-steps", type=int, nargs=2, default=(250, 25000),
                     help="log-uniform job run length range for the served-job leg")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy of MIB MiB (profiler byte-counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: independent replicas shard across GPUs with no exchange, "
               "so multi-GPU runs are not part of this build (use --gpus 1)", file=sys.stderr)

@@ -168,6 +168,20 @@ __device__ inline float fix_diff(unsigned a, unsigned b, float ulen) {
     return (float)(int)(a - b) * ulen;
 }
 
+// LJ cutoff test in grid units.  a = (float)(int)(Xi - Xj) per axis (the
+// fixed-point displacement in grid steps u = L * 2^-32, rounded once),
+// q = fma(az, az, fma(ay, ay, ax * ax)) = r^2 / u^2, and a pair is inside when
+// lo_g < q < rc2_g with rc2_g = (float)(rc^2 / u^2) and lo_g =
+// (float)(1e-10f / u^2), both quotients in FP64 (u^2 is exact in FP64).  The
+// lower bound is the r^2 > 1e-10 test of src/force/lj.cu:37.
+struct GridCut { float rc2g, lo_g; };
+__device__ inline GridCut grid_cut(float rc2, float u) {
+    const double u2 = (double)u * (double)u;
+    return GridCut{(float)((double)rc2 / u2), (float)((double)1e-10f / u2)};
+}
+__device__ inline float grid_d(unsigned a, unsigned b) { return (float)(int)(a - b); }
+__device__ inline float grid_q(float x, float y, float z) { return fmaf(z, z, fmaf(y, y, x * x)); }
+
 // Morton interleave of 10 bits (src/core/morton.cu:4-10)
 __host__ __device__ inline uint32_t expand_bits(uint32_t v) {
     v = (v | (v << 16)) & 0x030000FFu;

@@ -55,10 +55,14 @@ template <class T> __device__ inline const FMD_GLOBAL T* gview(const T* p) {
 
 constexpr int kMaxBlockK = 256;  // launch bound of k_build and k_force
 constexpr int kLJChunk = 8;      // neighbors per software-pipelined chunk (kcap % 8 == 0)
+constexpr int kBuildChunk = 8;   // list-build candidates whose loads are issued together
 
 #pragma clang fp contract(off)
 __device__ inline float sq3(float x, float y, float z) { return x * x + y * y + z * z; }
 #pragma clang fp contract(fast)
+
+// Word 3 of a raw buffer resource on gfx9 parts: 32-bit data format, no swizzle.
+constexpr int kBufferWord3 = 0x00020000;
 
 // ---------------------------------------------------------------------------
 // FP64 Nose-Hoover chain (src/integrate/nose_hoover.cu:86-178 / :356-447) and
@@ -484,27 +488,41 @@ __global__ void __launch_bounds__(kMaxBlockK) k_build(const RepDesc* __restrict_
             int* out = d.nbr + (size_t)t.lt * d.kcap * kTile + lane;
             const int kcap = d.kcap;
             const float rcs2 = d.rcs2;
+            const FMD_GLOBAL float4* pos = gview(d.pos);
+            const FMD_GLOBAL int* cstart = gview(d.cell_start);
+            // cell c of the 27 in (dz, dy, dx) order, each offset in -1..1
+            auto key_of = [&](int c) {
+                int x = cx + c % 3 - 1, y = cy + (c / 3) % 3 - 1, z = cz + c / 9 - 1;
+                x += (x < 0) ? nx : 0; x -= (x >= nx) ? nx : 0;
+                y += (y < 0) ? nx : 0; y -= (y >= nx) ? nx : 0;
+                z += (z < 0) ? nx : 0; z -= (z >= nx) ? nx : 0;
+                return (int)cell_key(x, y, z);
+            };
             int cnt = 0;
-            int visited[27];
-            int nvis = 0;
-            for (int dz = -1; dz <= 1; dz++) {
-                int z = cz + dz; z += (z < 0) ? nx : 0; z -= (z >= nx) ? nx : 0;
-                for (int dy = -1; dy <= 1; dy++) {
-                    int y = cy + dy; y += (y < 0) ? nx : 0; y -= (y >= nx) ? nx : 0;
-                    for (int dx = -1; dx <= 1; dx++) {
-                        int x = cx + dx; x += (x < 0) ? nx : 0; x -= (x >= nx) ? nx : 0;
-                        const int key = (int)cell_key(x, y, z);
-                        if (nx < 3) {  // small boxes visit a cell once (verlet_list.cu:154-187)
-                            bool seen = false;
-                            for (int v = 0; v < nvis; v++) seen |= (visited[v] == key);
-                            if (seen) continue;
-                            visited[nvis++] = key;
-                        }
-                        const int s = d.cell_start[key], e = d.cell_start[key + 1];
-                        for (int q = s; q < e; q++) {
-                            if (q == i) continue;
-                            const float4 pj = d.pos[q];
-                            if (!(dist2_ref(pi, pj, L, invL) < rcs2)) continue;
+            int key = key_of(0);
+            int s = cstart[key], e = cstart[key + 1];
+            for (int c = 0; c < 27; c++) {
+                // the next cell's bounds are in flight while this cell is scanned
+                int nkey = 0, ns = 0, ne = 0;
+                if (c + 1 < 27) {
+                    nkey = key_of(c + 1);
+                    ns = cstart[nkey];
+                    ne = cstart[nkey + 1];
+                }
+                bool seen = false;
+                if (nx < 3)  // small boxes visit a cell once (verlet_list.cu:154-187)
+                    for (int v = 0; v < c; v++) seen |= (key_of(v) == key);
+                if (!seen) {
+                    // candidates in chunks whose loads are issued together
+                    for (int q0 = s; q0 < e; q0 += kBuildChunk) {
+                        float4 pj[kBuildChunk];
+#pragma unroll
+                        for (int u = 0; u < kBuildChunk; u++) pj[u] = pos[min(q0 + u, e - 1)];
+#pragma unroll
+                        for (int u = 0; u < kBuildChunk; u++) {
+                            const int q = q0 + u;
+                            if (q >= e || q == i) continue;
+                            if (!(dist2_ref(pi, pj[u], L, invL) < rcs2)) continue;
                             bool exc = (q == ex[0]) | (q == ex[1]) | (q == ex[2]) | (q == ex[3]);
                             for (int b = 4; b < nb && b < d.bcap; b++)
                                 exc |= ((bt[b * kTile] & kSlotMask) == q);
@@ -514,6 +532,7 @@ __global__ void __launch_bounds__(kMaxBlockK) k_build(const RepDesc* __restrict_
                         }
                     }
                 }
+                key = nkey; s = ns; e = ne;
             }
             if (cnt > kcap) err |= ERR_NBR_OVERFLOW;
             const int stored = min(cnt, kcap);
@@ -527,12 +546,17 @@ __global__ void __launch_bounds__(kMaxBlockK) k_build(const RepDesc* __restrict_
     }
 }
 
-// LJ over this atom's Verlet rows (src/force/lj.cu:27-63).  The rows are
-// padded to whole chunks of kLJChunk with the atom itself (r^2 = 0 is
-// rejected), so the loop loads whole chunks unconditionally; the next chunk's
-// indices are in flight while the current chunk's positions are gathered and
-// used.  The cutoff is applied by masking 1/r^2, so there is no per-pair
-// branch.  OBS adds the per-atom PE and virial (thermo steps and NPT), CAP the
+// LJ over this atom's Verlet rows (src/force/lj.cu:27-63), in grid units.
+// With a = (float)(int)(Xi - Xj) the fixed-point displacement (grid step
+// u = L * 2^-32), q = a.a is r^2 / u^2: the cutoff test compares q with
+// thresholds scaled by 1/u^2, 1/q = u^2 / r^2 feeds the LJ terms with
+// sigma^2 / u^2 and 24 eps / u^2 folded in, and the force and virial sums are
+// kept in grid units and scaled by u and u^2 once per atom.  The rows are
+// padded to whole chunks of kLJChunk with the atom itself (q = 0 is rejected),
+// so whole chunks are loaded unconditionally; the next chunk's indices are in
+// flight while the current chunk's positions are gathered (buffer loads: one
+// shift per address).  The cutoff masks 1/q, so there is no per-pair branch.
+// OBS adds the per-atom PE and virial (thermo steps and NPT), CAP the
 // push-off pair-force cap, ONE_TYPE a single (eps, sigma) pair.
 template <bool OBS, bool CAP, bool ONE_TYPE>
 __device__ __forceinline__ void lj_loop(const RepDesc& d, int lt, int lane, int i, int4 pi, float L,
@@ -541,52 +565,68 @@ __device__ __forceinline__ void lj_loop(const RepDesc& d, int lt, int lane, int 
                                         float& w5) {
     const int nn = d.nnbr[i];
     const FMD_GLOBAL int* nl = gview(d.nbr) + (size_t)lt * d.kcap * kTile + lane;
-    const FMD_GLOBAL int4* pos = gview(d.ipos);
+    const __amdgpu_buffer_rsrc_t prs =
+        __builtin_amdgcn_make_buffer_rsrc((void*)d.ipos, 0, d.nslots * (int)sizeof(int4), kBufferWord3);
+    const float u = L * 2.3283064365386963e-10f;
+    const GridCut gc = grid_cut(d.rc2, u);
+    const float iu2 = (float)(1.0 / ((double)u * (double)u));
     const float2* ljrow = d.lj + pi.w * d.ntypes;
     const float2 p0 = ljrow[0];
-    const float rc2 = d.rc2;
-    const float fcap = d.fcap;
-    const float ulen = L * 2.3283064365386963e-10f;
+    const float s2g0 = p0.y * p0.y * iu2, c24g0 = 24.0f * p0.x * iu2, c2e0 = 2.0f * p0.x;
+    const float capg = CAP ? d.fcap / u : 0.0f;  // fcap / r = (fcap / u) / sqrt(q)
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+    float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
     unsigned nxt[kLJChunk];
     if (nn > 0) {
 #pragma unroll
-        for (int u = 0; u < kLJChunk; u++) nxt[u] = (unsigned)nl[u * kTile];
+        for (int k = 0; k < kLJChunk; k++) nxt[k] = (unsigned)nl[k * kTile];
     }
     for (int k0 = 0; k0 < nn; k0 += kLJChunk) {
-        int4 pjs[kLJChunk];
+        uint4 pjs[kLJChunk];
 #pragma unroll
-        for (int u = 0; u < kLJChunk; u++) pjs[u] = pos[nxt[u]];
+        for (int k = 0; k < kLJChunk; k++) {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b128(prs, nxt[k] << 4, 0, 0);
+            pjs[k] = make_uint4(v[0], v[1], v[2], v[3]);
+        }
         if (k0 + kLJChunk < nn) {
 #pragma unroll
-            for (int u = 0; u < kLJChunk; u++) nxt[u] = (unsigned)nl[(k0 + kLJChunk + u) * kTile];
+            for (int k = 0; k < kLJChunk; k++) nxt[k] = (unsigned)nl[(k0 + kLJChunk + k) * kTile];
         }
 #pragma unroll
-        for (int u = 0; u < kLJChunk; u++) {
-            const int4 pj = pjs[u];
-            const float dx = fix_diff(pi.x, pj.x, ulen);
-            const float dy = fix_diff(pi.y, pj.y, ulen);
-            const float dz = fix_diff(pi.z, pj.z, ulen);
-            const float r2 = sq3(dx, dy, dz);
-            const bool in = (r2 < rc2) && (r2 > 1e-10f);
-            const float2 p = ONE_TYPE ? p0 : ljrow[pj.w];
-            const float r2inv = in ? __builtin_amdgcn_rcpf(r2) : 0.0f;
-            const float sr2 = p.y * p.y * r2inv;
+        for (int k = 0; k < kLJChunk; k++) {
+            const uint4 pj = pjs[k];
+            const float ax = grid_d(pi.x, pj.x), ay = grid_d(pi.y, pj.y), az = grid_d(pi.z, pj.z);
+            const float q = grid_q(ax, ay, az);
+            const bool in = (q < gc.rc2g) && (q > gc.lo_g);
+            float s2g = s2g0, c24g = c24g0, c2e = c2e0;
+            if (!ONE_TYPE) {
+                const float2 p = ljrow[pj.w];
+                s2g = p.y * p.y * iu2; c24g = 24.0f * p.x * iu2; c2e = 2.0f * p.x;
+            }
+            const float qinv = in ? __builtin_amdgcn_rcpf(q) : 0.0f;
+            const float sr2 = s2g * qinv;
             const float sr6 = sr2 * sr2 * sr2;
             const float sr12 = sr6 * sr6;
-            float fr = 24.0f * p.x * r2inv * (2.0f * sr12 - sr6);
-            fr = fminf(fmaxf(fr, -1000.0f), 1000.0f);  // src/force/lj.cu:48
+            float fr = c24g * qinv * (2.0f * sr12 - sr6);  // force / r in real units
+            fr = fminf(fmaxf(fr, -1000.0f), 1000.0f);       // src/force/lj.cu:48
             if (CAP) {
-                const float cap = fcap * rsqrtf(r2);
+                const float cap = capg * __builtin_amdgcn_rsqf(q);
                 fr = fminf(fmaxf(fr, -cap), cap);
             }
-            fx += fr * dx; fy += fr * dy; fz += fr * dz;
+            gx += fr * ax; gy += fr * ay; gz += fr * az;
             if (OBS) {
-                const float hx = 0.5f * fr * dx, hy = 0.5f * fr * dy, hz = 0.5f * fr * dz;
-                pe += 2.0f * p.x * (sr12 - sr6);
-                w0 += hx * dx; w1 += hx * dy; w2 += hx * dz;
-                w3 += hy * dy; w4 += hy * dz; w5 += hz * dz;
+                const float hx = fr * ax, hy = fr * ay, hz = fr * az;
+                pe += c2e * (sr12 - sr6);
+                v0 += hx * ax; v1 += hx * ay; v2 += hx * az;
+                v3 += hy * ay; v4 += hy * az; v5 += hz * az;
             }
         }
+    }
+    fx += gx * u; fy += gy * u; fz += gz * u;
+    if (OBS) {
+        const float hu2 = 0.5f * u * u;
+        w0 += v0 * hu2; w1 += v1 * hu2; w2 += v2 * hu2;
+        w3 += v3 * hu2; w4 += v4 * hu2; w5 += v5 * hu2;
     }
 }
 

@@ -62,6 +62,22 @@ inline float min_image_fixed_f(float a, float b, float L) {
     const double sc = 4294967296.0 / (double)L;
     return fix_diff(fix_coord(a, sc), fix_coord(b, sc), L);
 }
+// The device's LJ cutoff test in grid units (grid step u = L * 2^-32):
+// a = (float)(int32)(Xi - Xj) per axis, q = fma(az, az, fma(ay, ay, ax * ax)),
+// inside when lo_g < q < rc2_g, rc2_g = (float)(rc2 / u^2), lo_g =
+// (float)(1e-10f / u^2), quotients in FP64.  Mathematically r^2 = q u^2, so
+// this is the r^2 < rc^2 && r^2 > 1e-10 test of src/force/lj.cu:37 up to one
+// FP32 rounding of each side.
+inline bool lj_inside_grid(const float* pi, const float* pj, float rc2, float L) {
+    const double sc = 4294967296.0 / (double)L;
+    const float u = L * 2.3283064365386963e-10f;
+    const double u2 = (double)u * (double)u;
+    const float rc2g = (float)((double)rc2 / u2), lo_g = (float)((double)1e-10f / u2);
+    float a[3];
+    for (int k = 0; k < 3; k++) a[k] = (float)(int32_t)(fix_coord(pi[k], sc) - fix_coord(pj[k], sc));
+    const float q = std::fma(a[2], a[2], std::fma(a[1], a[1], a[0] * a[0]));
+    return q < rc2g && q > lo_g;
+}
 // Same displacement carried in FP64 (the image shift m chosen as above).
 inline double min_image_exact_d(float a, float b, float L, float invL) {
     float s = a - b;
@@ -350,8 +366,8 @@ inline void angle_literal_f32(const float* pos4, int n, const int* ang4, int nan
 // FP64 yardstick for the device force kernel.
 // terms: bit 0 = LJ (src/force/lj.cu:27-63), bit 1 = FENE+WCA
 // (src/force/fene.cu:21-65), bit 2 = angle (src/force/angle.cu:22-79).
-// LJ pairs: i != j, not excluded, rf2 < rc^2 and rf2 > 1e-10 where rf2 is the
-// FP32 squared fixed-point distance of the device; the force/r clamp of lj.cu:48 is
+// LJ pairs: i != j, not excluded, and inside by the device's grid-unit test
+// (lj_inside_grid); the force/r clamp of lj.cu:48 is
 // applied when clamp_lj is set.  FENE r^2 is clamped to [1e-6, 0.9999 R0^2]
 // as in fene.cu:33 when clamp_fene is set.  Angles use theta = atan2(|a x b|, a.b),
 // mathematically identical to acos(clamp(cos)) of angle.cu:37-45.
@@ -371,11 +387,7 @@ struct ForceParams {
 inline void lj_pair_f64(const float* pi, const float* pj, const ForceParams& P, float L,
                         float invL, double* fi, double* pe_i, double* vir) {
     float rc2 = P.rc * P.rc;
-    float dxf = min_image_fixed_f(pi[0], pj[0], L);
-    float dyf = min_image_fixed_f(pi[1], pj[1], L);
-    float dzf = min_image_fixed_f(pi[2], pj[2], L);
-    float rf2 = dxf * dxf + dyf * dyf + dzf * dzf;
-    if (!(rf2 < rc2 && rf2 > 1e-10f)) return;
+    if (!lj_inside_grid(pi, pj, rc2, L)) return;
     double dx = min_image_exact_d(pi[0], pj[0], L, invL);
     double dy = min_image_exact_d(pi[1], pj[1], L, invL);
     double dz = min_image_exact_d(pi[2], pj[2], L, invL);

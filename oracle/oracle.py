@@ -46,6 +46,8 @@ def lib():
         L.orc_min_image.argtypes = [C.c_float] * 3
         L.orc_min_image_fixed.restype = C.c_float
         L.orc_min_image_fixed.argtypes = [C.c_float] * 3
+        L.orc_lj_inside_grid.restype = None
+        L.orc_lj_inside_grid.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_void_p]
         L.orc_wrap.restype = C.c_float
         L.orc_wrap.argtypes = [C.c_float] * 3
         L.orc_morton3d.restype = C.c_uint32
@@ -99,6 +101,15 @@ def type_bits(t):
 
 def min_image(dr, L):
     return lib().orc_min_image(dr, L, np.float32(1.0) / np.float32(L))
+
+
+def lj_inside_grid(a, b, rc2, L):
+    """Device LJ cutoff decisions (grid units) for point pairs a[k], b[k]."""
+    a = np.ascontiguousarray(a, np.float32).reshape(-1, 3)
+    b = np.ascontiguousarray(b, np.float32).reshape(-1, 3)
+    out = np.zeros(len(a), np.int32)
+    lib().orc_lj_inside_grid(len(a), a.ctypes.data, b.ctypes.data, rc2, L, out.ctypes.data)
+    return out.astype(bool)
 
 
 def wrap(x, L):

@@ -10,6 +10,14 @@ extern "C" {
 
 float orc_min_image(float dr, float L, float invL) { return min_image_ref(dr, L, invL); }
 float orc_min_image_fixed(float a, float b, float L) { return min_image_fixed_f(a, b, L); }
+// n pairs of points (xyz each): out[k] = 1 if the device's grid-unit LJ test keeps pair k
+void orc_lj_inside_grid(int n, const float* a, const float* b, float rc2, float L, int32_t* out) {
+    for (int k = 0; k < n; k++) {
+        const float pa[3] = {a[3 * k], a[3 * k + 1], a[3 * k + 2]};
+        const float pb[3] = {b[3 * k], b[3 * k + 1], b[3 * k + 2]};
+        out[k] = lj_inside_grid(pa, pb, rc2, L) ? 1 : 0;
+    }
+}
 float orc_wrap(float x, float L, float invL) { return wrap_ref(x, L, invL); }
 uint32_t orc_morton3d(float x, float y, float z, float invL) { return morton3d(x, y, z, invL); }
 

@@ -72,6 +72,37 @@ def test_fixed_point_displacement(orc, L):
         assert orc.lib().orc_min_image_fixed(y, x, L) == -got or abs(got) < 3 * q
 
 
+@pytest.mark.parametrize("L", [6.0, 31.7, 105.6])
+def test_lj_grid_cutoff_matches_exact_distance(orc, L):
+    """The device's grid-unit LJ test equals the exact minimum-image test
+    r^2 < rc^2 (and r^2 > 1e-10) outside a thin band around rc^2: a wrong
+    scale of the grid (u instead of u^2, a missing 1/L) or a dropped axis
+    flips many pairs far from the band."""
+    rng = np.random.default_rng(7)
+    L = F(L)
+    rc2 = F(2.5) * F(2.5)
+    n = 20000
+    a = (rng.random((n, 3)) * L).astype(F)
+    # offsets spread around the cutoff, and a few coincident and near pairs
+    r = rng.uniform(0.0, 3.2, n)
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    b = (a + r[:, None] * u).astype(F)
+    b[:5] = a[:5]
+    b[5:10] = a[5:10] + F(2e-5)
+    b = np.mod(b, L).astype(F)
+    got = orc.lj_inside_grid(a, b, float(rc2), float(L))
+    d = a.astype(np.float64) - b.astype(np.float64)
+    d -= float(L) * np.rint(d / float(L))
+    r2 = (d * d).sum(1)
+    want = (r2 < float(rc2)) & (r2 > 1e-10)
+    band = np.abs(r2 - float(rc2)) < 1e-5 * float(rc2)
+    assert not got[:5].any()              # coincident points (row padding) are never a pair
+    assert got[5:10].all()                # 3.5e-5 apart (r^2 = 1.2e-9): inside
+    assert np.array_equal(got[~band], want[~band])
+    assert band.sum() < 20 and want.sum() > n // 2
+
+
 # --- Morton keys: decode the interleave bit by bit and compare with the quantised coordinates
 def _deinterleave(k):
     out = [0, 0, 0]
