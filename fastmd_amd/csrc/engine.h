@@ -13,7 +13,6 @@
 
 namespace fmd {
 
-constexpr int kMaxReplicas = 4096;
 constexpr int kNumClasses = 3;
 constexpr int kGraphSteps = 20;
 
@@ -36,6 +35,9 @@ struct Batch {
     std::vector<int> ids;   // replica ids, ordered by size class
     Tile* d_tiles = nullptr;
     int* d_reps = nullptr;
+    int* d_rep_t0 = nullptr;  // first tile of each replica in d_reps order
+    int* d_rlist = nullptr;   // compact rebuild tile list (ntl entries)
+    int* d_nrt = nullptr;     // its length
     int ntl = 0, nr = 0;
     int cls_t0[kNumClasses] = {0, 0, 0}, cls_nt[kNumClasses] = {0, 0, 0};
     hipGraphExec_t graph = nullptr;

@@ -21,9 +21,21 @@ int size_class(int natoms) { return natoms < 8192 ? 0 : (natoms < 32768 ? 1 : 2)
 
 static inline int ceil64(int n) { return (n + 63) / 64 * 64; }
 
+static void free_batch(Batch* b) {
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    (void)hipFree(b->d_tiles);
+    (void)hipFree(b->d_reps);
+    (void)hipFree(b->d_rep_t0);
+    (void)hipFree(b->d_rlist);
+    (void)hipFree(b->d_nrt);
+    delete b;
+}
+
 Engine::Engine(int device, uint64_t budget) : device_(device) {
-    for (int k = 0; k < 2; k++)
-        for (int c = 0; c < kNumClasses; c++) tuned_[k][c] = 256;
+    for (int c = 0; c < kNumClasses; c++) {
+        tuned_[0][c] = 256;
+        tuned_[1][c] = 64;  // the list build runs one-wave workgroups (LDS staging per tile)
+    }
     if (hip_check(hipSetDevice(device), "hipSetDevice")) return;
     if (hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate")) return;
     size_t fr = 0, tot = 0;
@@ -37,11 +49,7 @@ Engine::Engine(int device, uint64_t budget) : device_(device) {
 Engine::~Engine() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (auto& kv : batches_) {
-        Batch* b = kv.second;
-        if (b->graph) (void)hipGraphExecDestroy(b->graph);
-        (void)hipFree(b->d_tiles);
-        (void)hipFree(b->d_reps);
-        delete b;
+        free_batch(kv.second);
     }
     for (Replica* r : reps_)
         if (r) {
@@ -424,7 +432,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
         return FMD_ERR_HIP;
     }
     Launch l = launch_of(*b);
-    launch_rebuild(l, 1, 256);
+    launch_rebuild(l, 1, 1);
     launch_force(l, 0, b->ntl, 256, FMD_TERM_ALL, 0, 1);
     launch_boundary(l, BND_PRIME);
     rc = hip_check(hipStreamSynchronize(s), "sync(prime)");
@@ -462,10 +470,7 @@ void Engine::drop_batches_with(int id) {
     for (auto it = batches_.begin(); it != batches_.end();) {
         if (std::find(it->first.begin(), it->first.end(), id) != it->first.end()) {
             Batch* b = it->second;
-            if (b->graph) (void)hipGraphExecDestroy(b->graph);
-            (void)hipFree(b->d_tiles);
-            (void)hipFree(b->d_reps);
-            delete b;
+            free_batch(b);
             it = batches_.erase(it);
         } else {
             ++it;
@@ -493,24 +498,35 @@ Batch* Engine::batch_for(const std::vector<int>& ids_in) {
     std::vector<int> ord = ids;
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int c) { return reps_[a]->cls < reps_[c]->cls; });
     std::vector<Tile> tiles;
+    std::vector<int> t0s(ord.size());
     for (int cl = 0; cl < kNumClasses; cl++) {
         b->cls_t0[cl] = (int)tiles.size();
-        for (int id : ord)
-            if (reps_[id]->cls == cl)
-                for (int t = 0; t < reps_[id]->ntiles; t++) tiles.push_back(Tile{id, t});
+        for (size_t k = 0; k < ord.size(); k++)
+            if (reps_[ord[k]]->cls == cl) {
+                t0s[k] = (int)tiles.size();
+                for (int t = 0; t < reps_[ord[k]]->ntiles; t++) tiles.push_back(Tile{ord[k], t});
+            }
         b->cls_nt[cl] = (int)tiles.size() - b->cls_t0[cl];
     }
     b->ntl = (int)tiles.size();
     b->nr = (int)ord.size();
+    if (ord.size() > (size_t)kMaxReplicas) {
+        delete b;
+        fail(FMD_ERR_ARG, "more than " + std::to_string(kMaxReplicas) + " systems in one batch");
+        return nullptr;
+    }
     bool ok = hipMalloc(&b->d_tiles, std::max<size_t>(1, tiles.size()) * sizeof(Tile)) == hipSuccess &&
-              hipMalloc(&b->d_reps, std::max<size_t>(1, ord.size()) * sizeof(int)) == hipSuccess;
+              hipMalloc(&b->d_reps, std::max<size_t>(1, ord.size()) * sizeof(int)) == hipSuccess &&
+              hipMalloc(&b->d_rep_t0, std::max<size_t>(1, ord.size()) * sizeof(int)) == hipSuccess &&
+              hipMalloc(&b->d_rlist, std::max<size_t>(1, tiles.size()) * sizeof(int)) == hipSuccess &&
+              hipMalloc(&b->d_nrt, sizeof(int)) == hipSuccess;
     if (ok)
         ok = hipMemcpy(b->d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice) == hipSuccess &&
-             hipMemcpy(b->d_reps, ord.data(), ord.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+             hipMemcpy(b->d_reps, ord.data(), ord.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(b->d_rep_t0, t0s.data(), t0s.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemset(b->d_nrt, 0, sizeof(int)) == hipSuccess;
     if (!ok) {
-        (void)hipFree(b->d_tiles);
-        (void)hipFree(b->d_reps);
-        delete b;
+        free_batch(b);
         fail(FMD_ERR_NOMEM, "batch allocation failed");
         return nullptr;
     }
@@ -519,7 +535,7 @@ Batch* Engine::batch_for(const std::vector<int>& ids_in) {
 }
 
 Launch Engine::launch_of(const Batch& b) const {
-    return Launch{d_descs_, b.d_tiles, b.ntl, b.d_reps, b.nr, stream_};
+    return Launch{d_descs_, b.d_tiles, b.ntl, b.d_reps, b.nr, b.d_rep_t0, b.d_rlist, b.d_nrt, stream_};
 }
 
 // One MD step for every replica of the batch.
@@ -527,8 +543,7 @@ void Engine::launch_step_seq(Batch& b) {
     Launch l = launch_of(b);
     launch_pre(l);
     launch_decide(l);
-    launch_rebuild(l, 0, 0);
-    for (int c = 0; c < kNumClasses; c++) launch_build(l, b.cls_t0[c], b.cls_nt[c], tuned_[1][c], 0);
+    launch_rebuild(l, 0, 1);
     for (int c = 0; c < kNumClasses; c++)
         launch_force(l, b.cls_t0[c], b.cls_nt[c], tuned_[0][c], FMD_TERM_ALL, 1, 0);
     launch_boundary(l, BND_STEP);
@@ -631,10 +646,7 @@ void Engine::trim_batches(const std::vector<int>& keep_in) {
     for (auto it = batches_.begin(); it != batches_.end();) {
         if (it->first != keep) {
             Batch* b = it->second;
-            if (b->graph) (void)hipGraphExecDestroy(b->graph);
-            (void)hipFree(b->d_tiles);
-            (void)hipFree(b->d_reps);
-            delete b;
+            free_batch(b);
             it = batches_.erase(it);
         } else {
             ++it;
@@ -705,7 +717,7 @@ int32_t Engine::build_neighbors(int32_t sys) {
     if (!b) return FMD_ERR_ARG;
     Launch l = launch_of(*b);
     launch_control(l, CTL_FORCE_REBUILD, 0);
-    launch_rebuild(l, 1, 256);
+    launch_rebuild(l, 1, 1);
     // slots were permuted: recompute the forces in the new order
     launch_force(l, 0, b->ntl, 256, FMD_TERM_ALL, 0, 1);
     launch_boundary(l, BND_REDUCE);
@@ -962,8 +974,9 @@ int32_t Engine::time_force(const int32_t* ids, int32_t n, uint32_t terms, int re
 }
 
 // ---------------------------------------------------------------------------
-// autotuner: per size class, time the force and list-build kernels over the
-// class's tiles for each candidate block shape and keep the fastest.
+// autotuner: per size class, time the force kernel over the class's tiles for
+// each candidate block shape and keep the fastest.  The list build has one
+// shape (one wave per tile, its LDS staging sized for that).
 // ---------------------------------------------------------------------------
 int32_t Engine::autotune(const int32_t* ids, int32_t n, int reps) {
     if (!ids || n <= 0) return fail(FMD_ERR_ARG, "empty system list");
@@ -980,14 +993,11 @@ int32_t Engine::autotune(const int32_t* ids, int32_t n, int reps) {
     int32_t rc = FMD_OK;
     for (int c = 0; c < kNumClasses && !rc; c++) {
         if (b->cls_nt[c] == 0) continue;
-        for (int kern = 0; kern < 2 && !rc; kern++) {
+        for (int kern = 0; kern < 1 && !rc; kern++) {
             float best = 1e30f;
             int bestb = 256;
             for (int bi = 0; bi < 3 && !rc; bi++) {
-                auto run = [&]() {
-                    if (kern == 0) launch_force(l, b->cls_t0[c], b->cls_nt[c], cand[bi], FMD_TERM_ALL, 0, 1);
-                    else launch_build(l, b->cls_t0[c], b->cls_nt[c], cand[bi], 1);
-                };
+                auto run = [&]() { launch_force(l, b->cls_t0[c], b->cls_nt[c], cand[bi], FMD_TERM_ALL, 0, 1); };
                 run();  // warm
                 (void)hipEventRecord(e0, stream_);
                 for (int k = 0; k < reps; k++) run();
@@ -1012,6 +1022,7 @@ int32_t Engine::set_tuned(int kernel, int cls, int block) {
         return fail(FMD_ERR_ARG, "kernel must be 0 or 1 and the size class in [0, 3)");
     if (block != 64 && block != 128 && block != 256)
         return fail(FMD_ERR_ARG, "block shape must be 64, 128 or 256");
+    if (kernel == 1 && block != 64) return fail(FMD_ERR_ARG, "the list build runs 64-thread workgroups");
     tuned_[kernel][cls] = block;
     tune_version_++;
     return FMD_OK;

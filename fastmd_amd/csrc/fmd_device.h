@@ -13,7 +13,8 @@
 
 namespace fmd {
 
-constexpr int kTile = 64;          // slots per tile = one wave64
+constexpr int kTile = 64;
+constexpr int kMaxReplicas = 4096;  // entries of an engine's replica table          // slots per tile = one wave64
 constexpr int kMaxTypes = 16;
 constexpr int kMaxChain = 10;
 constexpr int kNumPart = 16;       // FP64 partials per wave: pe, W[6], K[6], pad

@@ -13,6 +13,9 @@ struct Launch {
     int ntl;               // tiles in the list
     const int* reps;       // device list of replica indices
     int nr;                // replicas in the list
+    const int* rep_t0;     // first tile of each listed replica
+    int* rlist;            // compact list of rebuilding tiles (capacity ntl)
+    int* nrt;              // its length
     hipStream_t stream;
 };
 
@@ -21,11 +24,11 @@ enum ControlOp { CTL_RUN = 0, CTL_FORCE_REBUILD = 1 };
 
 void launch_pre(const Launch& l);
 void launch_decide(const Launch& l);
-// The six rebuild phases; `always` ignores the active/rebuild flags.
-void launch_rebuild(const Launch& l, int always, int build_block);
+// The rebuild phases (list of rebuilding tiles, Morton cell sort and, if
+// `build`, the Verlet lists); `always` ignores the active/rebuild flags.
+void launch_rebuild(const Launch& l, int always, int build);
 // Force kernel over tiles [t0, t0+nt) of the list with the given block size.
 void launch_force(const Launch& l, int t0, int nt, int block, int terms, int kick, int always);
-void launch_build(const Launch& l, int t0, int nt, int block, int always);
 void launch_boundary(const Launch& l, int mode);
 void launch_control(const Launch& l, int op, int64_t arg);
 // Reference cell ids / Morton keys by global id for one replica.

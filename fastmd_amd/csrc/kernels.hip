@@ -312,16 +312,77 @@ __global__ void k_control(const RepDesc* __restrict__ descs, const int* __restri
 // slot order is a Morton curve over cells, and within a cell atoms are ordered
 // by global id (deterministic).
 // ---------------------------------------------------------------------------
+// The rebuild kernels walk the batch's compact list of rebuilding tiles
+// (k_rebuild_list), so their cost follows the replicas that rebuild.
 #define REBUILD_PROLOGUE                                      \
-    const Tile t = tiles[w];                                  \
+    const Tile t = tiles[rlist[v]];                           \
     const RepDesc& d = descs[t.rep];                          \
     RepState* st = d.st;                                      \
-    if (!always && !(st->active && st->rebuild)) continue;    \
+    (void)st;                                                 \
     const int i = t.lt * kTile + lane_id();
+#define REBUILD_LOOP for (int v = wave_in_grid(), nv = *nrt; v < nv; v += waves_in_grid())
+#define REBUILD_ARGS const RepDesc* __restrict__ descs, const Tile* __restrict__ tiles, \
+                     const int* __restrict__ rlist, const int* __restrict__ nrt
 
-__global__ void __launch_bounds__(256) k_bin(const RepDesc* __restrict__ descs,
-                                             const Tile* __restrict__ tiles, int ntl, int always) {
-    for (int w = wave_in_grid(); w < ntl; w += waves_in_grid()) {
+// Compact list of the tiles of rebuilding replicas (all replicas if `always`):
+// one workgroup scans the per-replica tile counts and writes tile indices.
+__global__ void __launch_bounds__(1024) k_rebuild_list(const RepDesc* __restrict__ descs,
+                                                       const int* __restrict__ reps,
+                                                       const int* __restrict__ rep_t0, int nr,
+                                                       int always, int* __restrict__ rlist,
+                                                       int* __restrict__ nrt) {
+    constexpr int kPer = kMaxReplicas / 1024;  // replicas per thread
+    __shared__ int wsum[16];
+    __shared__ int s_off[kMaxReplicas + 1];  // exclusive prefix of the listed tile counts
+    int cnt[kPer];
+    int local = 0;
+    for (int k = 0; k < kPer; k++) {
+        const int r = threadIdx.x * kPer + k;
+        cnt[k] = 0;
+        if (r < nr) {
+            const RepDesc& d = descs[reps[r]];
+            const RepState* st = d.st;
+            if (always || (st->active && st->rebuild)) cnt[k] = d.ntiles;
+        }
+        local += cnt[k];
+    }
+    int x = local;
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < 16; k++) { const int v = wsum[k]; wsum[k] = acc; acc += v; }
+        s_off[nr] = acc;
+        *nrt = acc;
+    }
+    __syncthreads();
+    int off = wsum[wv] + x - local;
+    for (int k = 0; k < kPer; k++) {
+        const int r = threadIdx.x * kPer + k;
+        if (r < nr) s_off[r] = off;
+        off += cnt[k];
+    }
+    __syncthreads();
+    // entry v belongs to the last replica r with s_off[r] <= v
+    const int total = s_off[nr];
+    for (int v = threadIdx.x; v < total; v += blockDim.x) {
+        int lo = 0, hi = nr - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= v) lo = mid; else hi = mid - 1;
+        }
+        rlist[v] = rep_t0[lo] + (v - s_off[lo]);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_bin(REBUILD_ARGS) {
+    REBUILD_LOOP {
         REBUILD_PROLOGUE
         if (i < d.natoms) {
             const float4 r = d.pos[i];
@@ -379,10 +440,8 @@ __global__ void __launch_bounds__(1024) k_scan(const RepDesc* __restrict__ descs
     if (threadIdx.x == 1023) d.cell_start[nkeys] = run;
 }
 
-__global__ void __launch_bounds__(256) k_scatter(const RepDesc* __restrict__ descs,
-                                                 const Tile* __restrict__ tiles, int ntl,
-                                                 int always) {
-    for (int w = wave_in_grid(); w < ntl; w += waves_in_grid()) {
+__global__ void __launch_bounds__(256) k_scatter(REBUILD_ARGS) {
+    REBUILD_LOOP {
         REBUILD_PROLOGUE
         if (i < d.natoms) {
             const int dst = d.cell_start[d.cell_of[i]] + d.rank[i];
@@ -396,10 +455,8 @@ __global__ void __launch_bounds__(256) k_scatter(const RepDesc* __restrict__ des
 
 // Rank inside the cell by global id: deterministic order independent of the
 // atomic arrival order of k_bin.
-__global__ void __launch_bounds__(256) k_rank(const RepDesc* __restrict__ descs,
-                                              const Tile* __restrict__ tiles, int ntl,
-                                              int always) {
-    for (int w = wave_in_grid(); w < ntl; w += waves_in_grid()) {
+__global__ void __launch_bounds__(256) k_rank(REBUILD_ARGS) {
+    REBUILD_LOOP {
         REBUILD_PROLOGUE
         if (i < d.natoms) {
             const int src = d.order[i];
@@ -413,10 +470,8 @@ __global__ void __launch_bounds__(256) k_rank(const RepDesc* __restrict__ descs,
     }
 }
 
-__global__ void __launch_bounds__(256) k_gather(const RepDesc* __restrict__ descs,
-                                                const Tile* __restrict__ tiles, int ntl,
-                                                int always) {
-    for (int w = wave_in_grid(); w < ntl; w += waves_in_grid()) {
+__global__ void __launch_bounds__(256) k_gather(REBUILD_ARGS) {
+    REBUILD_LOOP {
         REBUILD_PROLOGUE
         if (i < d.natoms) {
             const int src = d.order2[i];
@@ -440,20 +495,72 @@ __global__ void __launch_bounds__(256) k_gather(const RepDesc* __restrict__ desc
 // bonded partners excluded; stored tile-major nbr[tile][k][lane] so the force
 // kernel streams one contiguous 256-byte row per neighbor index.  A count above
 // kcap is reported as ERR_NBR_OVERFLOW, never silently truncated.
+//
+// One wave per tile.  The tile's atoms lie in a few Morton-consecutive cells;
+// the union of those cells' 27-neighborhoods (typically 45-80 cells, about
+// 1000-1500 atoms) is staged in LDS with coalesced loads, and each lane scans
+// its own 27 cells from LDS in (dz, dy, dx) order, candidates in ascending slot
+// order, exactly as the global-memory scan that serves tiles whose union does
+// not fit.  Both produce the same rows.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kMaxBlockK) k_build(const RepDesc* __restrict__ descs,
-                                               const Tile* __restrict__ tiles, int t0, int ntl,
-                                               int always) {
-    for (int w = t0 + wave_in_grid(); w < t0 + ntl; w += waves_in_grid()) {
+constexpr int kStageAtoms = 1792;  // LDS-staged candidate atoms per tile
+constexpr int kMaxUnion = 128;     // distinct neighbor cells of one tile
+constexpr int kMaxOwn = 64;        // distinct cells holding the tile's atoms
+
+// set bits of m below this lane
+__device__ inline int lane_prefix(unsigned long long m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// cell coordinate offset by o in -1..1 with periodic wrap
+__device__ inline int wrap_cell(int c, int o, int nx) {
+    c += o;
+    c += (c < 0) ? nx : 0;
+    c -= (c >= nx) ? nx : 0;
+    return c;
+}
+
+// Scan of one chunk of kBuildChunk candidates (slots q0 + k, k < n): the
+// distance tests give a bit mask, then the set bits are appended in order,
+// skipping the atom itself and its bonded partners.
+__device__ __forceinline__ void build_chunk(const float4 (&pj)[kBuildChunk], int q0, int n, int i, float4 pi,
+                                            float L, float invL, float rcs2, const int (&ex)[4], int nb,
+                                            int bcap, const int* bt, int* out, int kcap, int& cnt) {
+    unsigned m = 0;
+#pragma unroll
+    for (int k = 0; k < kBuildChunk; k++) {
+        const float r2 = dist2_ref(pi, pj[k], L, invL);  // every k: no per-candidate branch
+        m |= ((k < n) & (r2 < rcs2)) ? (1u << k) : 0u;
+    }
+    while (m) {
+        const int q = q0 + __builtin_ctz(m);
+        m &= m - 1;
+        bool exc = (q == i) | (q == ex[0]) | (q == ex[1]) | (q == ex[2]) | (q == ex[3]);
+        for (int b = 4; b < nb && b < bcap; b++) exc |= ((bt[b * kTile] & kSlotMask) == q);
+        if (!exc) {
+            if (cnt < kcap) out[cnt * kTile] = q;
+            cnt++;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(64) k_build(REBUILD_ARGS) {
+    __shared__ float4 s_pos[kStageAtoms];
+    __shared__ int4 s_cell[kMaxUnion];             // (first slot, count, LDS offset, key)
+    __shared__ unsigned char s_map[kMaxOwn * 27];  // own cell x neighbor -> union index
+    __shared__ unsigned char s_seg[kStageAtoms];   // LDS atom -> union index
+    REBUILD_LOOP {
         REBUILD_PROLOGUE
+        const int lane = lane_id();
+        const bool valid = i < d.natoms;
         int err = 0;
-        if (i < d.natoms) {
-            const float4 pi = d.pos[i];
+        // bonded partners (slots), the exclusion set and the bonded tables
+        int ex[4] = {-1, -1, -1, -1};
+        int nb = 0;
+        float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (valid) {
+            pi = d.pos[i];
             const int g = d.img[i].w;
-            const int lane = lane_id();
-            // bonded partners (slots) and the exclusion set
-            int ex[4] = {-1, -1, -1, -1};
-            int nb = 0;
             const int b0 = d.t_boff[g], b1 = d.t_boff[g + 1];
             for (int e = b0; e < b1; e++) {
                 const int ent = d.t_blst[e];
@@ -478,62 +585,160 @@ __global__ void __launch_bounds__(kMaxBlockK) k_build(const RepDesc* __restrict_
             }
             if (na > d.acap) err |= ERR_BONDED;
             d.bhdr[i] = min(nb, d.bcap) | (min(na, d.acap) << 8);
-            const int* bt = d.bnd + (size_t)t.lt * d.bcap * kTile + lane;
-            // neighbor cells
-            const int nx = st->nx;
-            const float ics = st->inv_cs;
-            const float L = st->L, invL = st->invL;
-            const int cx = cell_coord(pi.x, ics, nx), cy = cell_coord(pi.y, ics, nx),
-                      cz = cell_coord(pi.z, ics, nx);
-            int* out = d.nbr + (size_t)t.lt * d.kcap * kTile + lane;
-            const int kcap = d.kcap;
-            const float rcs2 = d.rcs2;
-            const FMD_GLOBAL float4* pos = gview(d.pos);
-            const FMD_GLOBAL int* cstart = gview(d.cell_start);
-            // cell c of the 27 in (dz, dy, dx) order, each offset in -1..1
-            auto key_of = [&](int c) {
-                int x = cx + c % 3 - 1, y = cy + (c / 3) % 3 - 1, z = cz + c / 9 - 1;
-                x += (x < 0) ? nx : 0; x -= (x >= nx) ? nx : 0;
-                y += (y < 0) ? nx : 0; y -= (y >= nx) ? nx : 0;
-                z += (z < 0) ? nx : 0; z -= (z >= nx) ? nx : 0;
-                return (int)cell_key(x, y, z);
-            };
-            int cnt = 0;
-            int key = key_of(0);
-            int s = cstart[key], e = cstart[key + 1];
-            for (int c = 0; c < 27; c++) {
-                // the next cell's bounds are in flight while this cell is scanned
-                int nkey = 0, ns = 0, ne = 0;
-                if (c + 1 < 27) {
-                    nkey = key_of(c + 1);
-                    ns = cstart[nkey];
-                    ne = cstart[nkey + 1];
+        }
+        const int* bt = d.bnd + (size_t)t.lt * d.bcap * kTile + lane;
+        int* out = d.nbr + (size_t)t.lt * d.kcap * kTile + lane;
+        const int kcap = d.kcap;
+        const float rcs2 = d.rcs2;
+        const int nx = st->nx;
+        const float ics = st->inv_cs;
+        const float L = st->L, invL = st->invL;
+        const FMD_GLOBAL float4* gpos = gview(d.pos);
+        const FMD_GLOBAL int* cstart = gview(d.cell_start);
+        int cx = 0, cy = 0, cz = 0;
+        if (valid) {
+            cx = cell_coord(pi.x, ics, nx); cy = cell_coord(pi.y, ics, nx); cz = cell_coord(pi.z, ics, nx);
+        }
+        const int key = (int)cell_key(cx, cy, cz);
+        // distinct cells of the tile: atoms are sorted by cell key, so a cell
+        // starts where the key differs from the previous lane's
+        const int prev = __shfl_up(key, 1, 64);
+        const unsigned long long first = __ballot(valid && (lane == 0 || key != prev));
+        const int nown = __popcll(first);
+        const int own = lane_prefix(first) + ((first >> lane) & 1ull) - 1;  // this lane's cell index
+        // union of the 27-neighborhoods of the tile's cells
+        bool staged = nown <= kMaxOwn;
+        int nu = 0;
+        if (staged) {
+            unsigned long long rest = first;
+            for (int o = 0; o < nown && staged; o++) {
+                const int src = __builtin_ctzll(rest);
+                rest &= rest - 1;
+                const int ox = __shfl(cx, src, 64), oy = __shfl(cy, src, 64), oz = __shfl(cz, src, 64);
+                int k = -1;
+                bool fresh = false;
+                if (lane < 27) {
+                    k = (int)cell_key(wrap_cell(ox, lane % 3 - 1, nx), wrap_cell(oy, (lane / 3) % 3 - 1, nx),
+                                      wrap_cell(oz, lane / 9 - 1, nx));
+                    int u = -1;
+                    for (int v = 0; v < nu; v++) u = (s_cell[v].w == k) ? v : u;
+                    // duplicates among the 27 (boxes under 3 cells wide): the lowest lane adds the cell
+                    int dup = -1;
+                    for (int m = 0; m < 27; m++) {
+                        const int km = __shfl(k, m, 64);
+                        if (m < lane && km == k && dup < 0) dup = m;
+                    }
+                    fresh = (u < 0) && (dup < 0);
+                    s_map[o * 27 + lane] = (unsigned char)(u < 0 ? 255 : u);  // new cells filled below
                 }
-                bool seen = false;
-                if (nx < 3)  // small boxes visit a cell once (verlet_list.cu:154-187)
-                    for (int v = 0; v < c; v++) seen |= (key_of(v) == key);
-                if (!seen) {
-                    // candidates in chunks whose loads are issued together
-                    for (int q0 = s; q0 < e; q0 += kBuildChunk) {
-                        float4 pj[kBuildChunk];
+                const unsigned long long nm = __ballot(fresh);
+                const int slot = nu + lane_prefix(nm);
+                if (nu + __popcll(nm) > kMaxUnion) {
+                    staged = false;
+                } else {
+                    if (fresh) {
+                        s_cell[slot] = make_int4(0, 0, 0, k);
+                        s_map[o * 27 + lane] = (unsigned char)slot;
+                    }
+                    __syncthreads();
+                    if (lane < 27 && !fresh && s_map[o * 27 + lane] == 255) {
+                        // a duplicate of a cell added by a lower lane of this round
+                        int u = 0;
+                        for (int v = nu; v < nu + __popcll(nm); v++) u = (s_cell[v].w == k) ? v : u;
+                        s_map[o * 27 + lane] = (unsigned char)u;
+                    }
+                    nu += __popcll(nm);
+                }
+                __syncthreads();
+            }
+        }
+        // counts, LDS offsets and the staged positions
+        int nstage = 0;
+        if (staged) {
+            for (int u0 = 0; u0 < nu; u0 += 64) {
+                const int u = u0 + lane;
+                int s0 = 0, c = 0;
+                if (u < nu) {
+                    const int k = s_cell[u].w;
+                    s0 = cstart[k];
+                    c = cstart[k + 1] - s0;
+                }
+                int x = c;
 #pragma unroll
-                        for (int u = 0; u < kBuildChunk; u++) pj[u] = pos[min(q0 + u, e - 1)];
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int y = __shfl_up(x, o, 64);
+                    if (lane >= o) x += y;
+                }
+                const int off = nstage + x - c;
+                if (u < nu) s_cell[u] = make_int4(s0, c, off, s_cell[u].w);
+                nstage += __shfl(x, 63, 64);
+            }
+            staged = nstage <= kStageAtoms;
+        }
+        __syncthreads();
+        if (staged) {
+            for (int u = lane; u < nu; u += 64) {
+                const int4 ce = s_cell[u];
+                for (int j = 0; j < ce.y; j++) s_seg[ce.z + j] = (unsigned char)u;
+            }
+            __syncthreads();
+            for (int q0 = 0; q0 < nstage; q0 += 4 * kTile) {
+                float4 v4[4];
 #pragma unroll
-                        for (int u = 0; u < kBuildChunk; u++) {
-                            const int q = q0 + u;
-                            if (q >= e || q == i) continue;
-                            if (!(dist2_ref(pi, pj[u], L, invL) < rcs2)) continue;
-                            bool exc = (q == ex[0]) | (q == ex[1]) | (q == ex[2]) | (q == ex[3]);
-                            for (int b = 4; b < nb && b < d.bcap; b++)
-                                exc |= ((bt[b * kTile] & kSlotMask) == q);
-                            if (exc) continue;
-                            if (cnt < kcap) out[cnt * kTile] = q;
-                            cnt++;
-                        }
+                for (int k = 0; k < 4; k++) {
+                    const int q = q0 + k * kTile + lane;
+                    if (q < nstage) {
+                        const int4 ce = s_cell[s_seg[q]];
+                        v4[k] = gpos[ce.x + q - ce.z];
                     }
                 }
-                key = nkey; s = ns; e = ne;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int q = q0 + k * kTile + lane;
+                    if (q < nstage) s_pos[q] = v4[k];
+                }
             }
+            __syncthreads();
+        }
+        int cnt = 0;
+        if (valid && staged) {
+            for (int c = 0; c < 27; c++) {
+                const int u = s_map[own * 27 + c];
+                bool seen = false;
+                if (nx < 3)  // small boxes visit a cell once (verlet_list.cu:154-187)
+                    for (int v = 0; v < c; v++) seen |= (s_map[own * 27 + v] == u);
+                if (seen) continue;
+                const int4 ce = s_cell[u];
+                for (int q0 = 0; q0 < ce.y; q0 += kBuildChunk) {
+                    float4 pj[kBuildChunk];
+#pragma unroll
+                    for (int k = 0; k < kBuildChunk; k++) pj[k] = s_pos[ce.z + min(q0 + k, ce.y - 1)];
+                    build_chunk(pj, ce.x + q0, ce.y - q0, i, pi, L, invL, rcs2, ex, nb, d.bcap, bt, out, kcap,
+                                cnt);
+                }
+            }
+        } else if (valid) {
+            // global-memory scan with the same visiting order
+            auto key_of = [&](int c) {
+                return (int)cell_key(wrap_cell(cx, c % 3 - 1, nx), wrap_cell(cy, (c / 3) % 3 - 1, nx),
+                                     wrap_cell(cz, c / 9 - 1, nx));
+            };
+            for (int c = 0; c < 27; c++) {
+                const int k = key_of(c);
+                bool seen = false;
+                if (nx < 3)
+                    for (int v = 0; v < c; v++) seen |= (key_of(v) == k);
+                if (seen) continue;
+                const int s0 = cstart[k], e0 = cstart[k + 1];
+                for (int q0 = s0; q0 < e0; q0 += kBuildChunk) {
+                    float4 pj[kBuildChunk];
+#pragma unroll
+                    for (int k2 = 0; k2 < kBuildChunk; k2++) pj[k2] = gpos[min(q0 + k2, e0 - 1)];
+                    build_chunk(pj, q0, e0 - q0, i, pi, L, invL, rcs2, ex, nb, d.bcap, bt, out, kcap, cnt);
+                }
+            }
+        }
+        if (valid) {
             if (cnt > kcap) err |= ERR_NBR_OVERFLOW;
             const int stored = min(cnt, kcap);
             // pad to a whole chunk with the atom itself (rejected by r^2 = 0), so
@@ -542,7 +747,8 @@ __global__ void __launch_bounds__(kMaxBlockK) k_build(const RepDesc* __restrict_
             d.nnbr[i] = stored;
         }
         err = wave_or(err);
-        if (err && lane_id() == 0) atomicOr(&st->err, err);
+        if (err && lane == 0) atomicOr(&st->err, err);
+        __syncthreads();  // the next tile reuses the LDS
     }
 }
 
@@ -927,6 +1133,7 @@ __global__ void k_morton_keys(const RepDesc* __restrict__ dp, int natoms, uint32
 // ---------------------------------------------------------------------------
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 static constexpr int kStrideBlocks = 4096;  // grid cap of the grid-stride rebuild kernels
+static constexpr int kBuildBlocks = 8192;   // grid cap of the one-wave list-build workgroups
 
 void launch_pre(const Launch& l) {
     if (l.ntl <= 0) return;
@@ -945,24 +1152,21 @@ void launch_control(const Launch& l, int op, int64_t arg) {
 static constexpr int kMaxBlock = kMaxBlockK;
 static inline int valid_block(int b) { return (b == 64 || b == 128 || b == 256) ? b : kMaxBlock; }
 
-void launch_build(const Launch& l, int t0, int nt, int block, int always) {
-    if (nt <= 0) return;
-    block = valid_block(block);
-    const int wpb = block / 64;
-    int g = cdiv(nt, wpb);
-    if (g > kStrideBlocks) g = kStrideBlocks;
-    k_build<<<g, block, 0, l.stream>>>(l.descs, l.tiles, t0, nt, always);
-}
-void launch_rebuild(const Launch& l, int always, int build_block) {
+void launch_rebuild(const Launch& l, int always, int build) {
     if (l.ntl <= 0) return;
+    k_rebuild_list<<<1, 1024, 0, l.stream>>>(l.descs, l.reps, l.rep_t0, l.nr, always, l.rlist, l.nrt);
     int g = cdiv(l.ntl, 4);
     if (g > kStrideBlocks) g = kStrideBlocks;
-    k_bin<<<g, 256, 0, l.stream>>>(l.descs, l.tiles, l.ntl, always);
+    k_bin<<<g, 256, 0, l.stream>>>(l.descs, l.tiles, l.rlist, l.nrt);
     k_scan<<<l.nr, 1024, 0, l.stream>>>(l.descs, l.reps, l.nr, always);
-    k_scatter<<<g, 256, 0, l.stream>>>(l.descs, l.tiles, l.ntl, always);
-    k_rank<<<g, 256, 0, l.stream>>>(l.descs, l.tiles, l.ntl, always);
-    k_gather<<<g, 256, 0, l.stream>>>(l.descs, l.tiles, l.ntl, always);
-    if (build_block > 0) launch_build(l, 0, l.ntl, build_block, always);
+    k_scatter<<<g, 256, 0, l.stream>>>(l.descs, l.tiles, l.rlist, l.nrt);
+    k_rank<<<g, 256, 0, l.stream>>>(l.descs, l.tiles, l.rlist, l.nrt);
+    k_gather<<<g, 256, 0, l.stream>>>(l.descs, l.tiles, l.rlist, l.nrt);
+    if (build) {
+        int gb = l.ntl;
+        if (gb > kBuildBlocks) gb = kBuildBlocks;
+        k_build<<<gb, 64, 0, l.stream>>>(l.descs, l.tiles, l.rlist, l.nrt);
+    }
 }
 void launch_force(const Launch& l, int t0, int nt, int block, int terms, int kick, int always) {
     if (nt <= 0) return;
