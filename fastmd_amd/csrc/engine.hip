@@ -240,6 +240,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     D.fcap = d->force_cap > 0.0f ? d->force_cap : 0.0f;
     D.vlim = D.fcap > 0.0f ? 0.02f / d->dt : 0.0f;
     D.clamp_lj = 1;
+    D.idx16 = (S.nslots <= 65536) && std::getenv("FMD_IDX32") == nullptr ? 1 : 0;
     D.seed_lo = (uint32_t)d->seed;
     D.seed_hi = (uint32_t)(d->seed >> 32);
 
@@ -740,9 +741,10 @@ int32_t Engine::get_pairs(int32_t sys, int64_t* npairs, int32_t* ij, int64_t cap
     if (rc) return rc;
     std::vector<std::pair<int, int>> pairs;
     for (int i = 0; i < D.natoms; i++) {
-        int lt = i / 64, lane = i % 64;
+        const int lt = i / 64, lane = i % 64;
+        const uint16_t* row16 = (const uint16_t*)(nbr.data() + (size_t)lt * D.kcap * 64);
         for (int k = 0; k < nn[i]; k++) {
-            int j = nbr[((size_t)lt * D.kcap + k) * 64 + lane];
+            const int j = D.idx16 ? (int)row16[(size_t)k * 64 + lane] : nbr[((size_t)lt * D.kcap + k) * 64 + lane];
             if (j < 0 || j >= D.natoms) return fail(FMD_ERR_STATE, "corrupt neighbor entry");
             pairs.emplace_back(img[i].w, img[j].w);
         }
@@ -939,7 +941,9 @@ int32_t Engine::time_force(const int32_t* ids, int32_t n, uint32_t terms, int re
         int64_t p = 0, nb = 0, na = 0;
         for (int i = 0; i < r->natoms; i++) { p += nn[i]; nb += hdr[i] & 0xFF; na += (hdr[i] >> 8) & 0xFF; }
         P += p;
-        if (terms & 1) B += 36.0 * r->natoms + 4.0 * (double)p;
+        // per bead: 16 B position, 4 B row count, 16 B force; per listed pair:
+        // its 16-bit or 32-bit index (the position gathers are cache traffic)
+        if (terms & 1) B += 36.0 * r->natoms + (r->desc.idx16 ? 2.0 : 4.0) * (double)p;
         if (terms & 6) B += 4.0 * r->natoms + ((terms & 1) ? 0.0 : 32.0 * r->natoms);
         if (terms & 2) B += 4.0 * (double)nb;
         if (terms & 4) B += 8.0 * (double)na;

@@ -14,7 +14,12 @@
 namespace fmd {
 
 constexpr int kTile = 64;
-constexpr int kMaxReplicas = 4096;  // entries of an engine's replica table          // slots per tile = one wave64
+constexpr int kMaxReplicas = 4096;  // entries of an engine's replica table
+// The list build stages a tile's candidate atoms -- the union of the
+// 27-neighborhoods of the cells holding its atoms -- in LDS when they fit: at
+// most kMaxUnion cells and kStageAtoms atoms.
+constexpr int kMaxUnion = 128;
+constexpr int kStageAtoms = 1792;          // slots per tile = one wave64
 constexpr int kMaxTypes = 16;
 constexpr int kMaxChain = 10;
 constexpr int kNumPart = 16;       // FP64 partials per wave: pe, W[6], K[6], pad
@@ -96,6 +101,7 @@ struct RepDesc {
     float fcap;            // LJ pair force cap (0 = off)
     float vlim;            // push-off speed limit (0 = off): |v| dt <= 0.02 sigma
     int clamp_lj;          // force/r clamp to +-1000 (src/force/lj.cu:48)
+    int idx16;             // neighbor rows hold 16-bit slots (nslots <= 65536)
     uint32_t seed_lo, seed_hi;
     // per-slot state (slot = position in the cell-sorted order)
     float4* pos;
@@ -115,7 +121,7 @@ struct RepDesc {
     int* cell_cnt;         // [ncell_cap]
     int* cell_start;       // [ncell_cap + 1]
     int* nnbr;             // [nslots]
-    int* nbr;              // [ntiles][kcap][64]
+    int* nbr;              // [ntiles][kcap][64] slots: 16-bit if idx16, else 32-bit
     int* bhdr;             // [nslots] nbonds | nangles << 8
     int* bnd;              // [ntiles][bcap][64] partner slot | type << 26
     int2* ang;             // [ntiles][acap][64] (slot | role << 26, slot | type << 26)

@@ -503,8 +503,6 @@ __global__ void __launch_bounds__(256) k_gather(REBUILD_ARGS) {
 // order, exactly as the global-memory scan that serves tiles whose union does
 // not fit.  Both produce the same rows.
 // ---------------------------------------------------------------------------
-constexpr int kStageAtoms = 1792;  // LDS-staged candidate atoms per tile
-constexpr int kMaxUnion = 128;     // distinct neighbor cells of one tile
 constexpr int kMaxOwn = 64;        // distinct cells holding the tile's atoms
 
 // set bits of m below this lane
@@ -522,25 +520,34 @@ __device__ inline int wrap_cell(int c, int o, int nx) {
 
 // Scan of one chunk of kBuildChunk candidates (slots q0 + k, k < n): the
 // distance tests give a bit mask, then the set bits are appended in order,
-// skipping the atom itself and its bonded partners.
+// skipping the atom itself and its bonded partners.  W16 rows hold 16-bit
+// slots (replicas of at most 65536 slots), other rows 32-bit slots.
+template <bool W16>
 __device__ __forceinline__ void build_chunk(const float4 (&pj)[kBuildChunk], int q0, int n, int i, float4 pi,
                                             float L, float invL, float rcs2, const int (&ex)[4], int nb,
-                                            int bcap, const int* bt, int* out, int kcap, int& cnt) {
+                                            int bcap, const int* bt, int* out, unsigned short* out16, int kcap,
+                                            int& cnt) {
     unsigned m = 0;
 #pragma unroll
     for (int k = 0; k < kBuildChunk; k++) {
         const float r2 = dist2_ref(pi, pj[k], L, invL);  // every k: no per-candidate branch
         m |= ((k < n) & (r2 < rcs2)) ? (1u << k) : 0u;
     }
-    while (m) {
-        const int q = q0 + __builtin_ctz(m);
-        m &= m - 1;
-        bool exc = (q == i) | (q == ex[0]) | (q == ex[1]) | (q == ex[2]) | (q == ex[3]);
-        for (int b = 4; b < nb && b < bcap; b++) exc |= ((bt[b * kTile] & kSlotMask) == q);
-        if (!exc) {
-            if (cnt < kcap) out[cnt * kTile] = q;
-            cnt++;
+    // the atom itself and its bonded partners, where they fall in this chunk
+    auto drop = [&](int e) {
+        const unsigned o = (unsigned)(e - q0);
+        m &= (o < (unsigned)kBuildChunk) ? ~(1u << o) : ~0u;
+    };
+    drop(i); drop(ex[0]); drop(ex[1]); drop(ex[2]); drop(ex[3]);
+    for (int b = 4; b < nb && b < bcap; b++) drop(bt[b * kTile] & kSlotMask);
+    // append in slot order
+#pragma unroll
+    for (int k = 0; k < kBuildChunk; k++) {
+        if (((m >> k) & 1u) && cnt < kcap) {
+            if (W16) out16[cnt * kTile] = (unsigned short)(q0 + k);
+            else out[cnt * kTile] = q0 + k;
         }
+        cnt += (m >> k) & 1u;
     }
 }
 
@@ -588,6 +595,7 @@ __global__ void __launch_bounds__(64) k_build(REBUILD_ARGS) {
         }
         const int* bt = d.bnd + (size_t)t.lt * d.bcap * kTile + lane;
         int* out = d.nbr + (size_t)t.lt * d.kcap * kTile + lane;
+        unsigned short* out16 = (unsigned short*)(d.nbr + (size_t)t.lt * d.kcap * kTile) + lane;
         const int kcap = d.kcap;
         const float rcs2 = d.rcs2;
         const int nx = st->nx;
@@ -686,11 +694,9 @@ __global__ void __launch_bounds__(64) k_build(REBUILD_ARGS) {
                 float4 v4[4];
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    const int q = q0 + k * kTile + lane;
-                    if (q < nstage) {
-                        const int4 ce = s_cell[s_seg[q]];
-                        v4[k] = gpos[ce.x + q - ce.z];
-                    }
+                    const int q = min(q0 + k * kTile + lane, nstage - 1);
+                    const int4 ce = s_cell[s_seg[q]];
+                    v4[k] = gpos[ce.x + q - ce.z];
                 }
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
@@ -713,8 +719,12 @@ __global__ void __launch_bounds__(64) k_build(REBUILD_ARGS) {
                     float4 pj[kBuildChunk];
 #pragma unroll
                     for (int k = 0; k < kBuildChunk; k++) pj[k] = s_pos[ce.z + min(q0 + k, ce.y - 1)];
-                    build_chunk(pj, ce.x + q0, ce.y - q0, i, pi, L, invL, rcs2, ex, nb, d.bcap, bt, out, kcap,
-                                cnt);
+                    if (d.idx16)
+                        build_chunk<true>(pj, ce.x + q0, ce.y - q0, i, pi, L, invL, rcs2, ex, nb, d.bcap, bt, out,
+                                          out16, kcap, cnt);
+                    else
+                        build_chunk<false>(pj, ce.x + q0, ce.y - q0, i, pi, L, invL, rcs2, ex, nb, d.bcap, bt, out,
+                                           out16, kcap, cnt);
                 }
             }
         } else if (valid) {
@@ -734,7 +744,12 @@ __global__ void __launch_bounds__(64) k_build(REBUILD_ARGS) {
                     float4 pj[kBuildChunk];
 #pragma unroll
                     for (int k2 = 0; k2 < kBuildChunk; k2++) pj[k2] = gpos[min(q0 + k2, e0 - 1)];
-                    build_chunk(pj, q0, e0 - q0, i, pi, L, invL, rcs2, ex, nb, d.bcap, bt, out, kcap, cnt);
+                    if (d.idx16)
+                        build_chunk<true>(pj, q0, e0 - q0, i, pi, L, invL, rcs2, ex, nb, d.bcap, bt, out, out16,
+                                          kcap, cnt);
+                    else
+                        build_chunk<false>(pj, q0, e0 - q0, i, pi, L, invL, rcs2, ex, nb, d.bcap, bt, out, out16,
+                                           kcap, cnt);
                 }
             }
         }
@@ -743,7 +758,11 @@ __global__ void __launch_bounds__(64) k_build(REBUILD_ARGS) {
             const int stored = min(cnt, kcap);
             // pad to a whole chunk with the atom itself (rejected by r^2 = 0), so
             // the force kernel loads whole chunks without per-row guards
-            for (int k = stored; k < ((stored + kLJChunk - 1) / kLJChunk) * kLJChunk; k++) out[k * kTile] = i;
+            const int padded = ((stored + kLJChunk - 1) / kLJChunk) * kLJChunk;
+            for (int k = stored; k < padded; k++) {
+                if (d.idx16) out16[k * kTile] = (unsigned short)i;
+                else out[k * kTile] = i;
+            }
             d.nnbr[i] = stored;
         }
         err = wave_or(err);
@@ -764,13 +783,16 @@ __global__ void __launch_bounds__(64) k_build(REBUILD_ARGS) {
 // shift per address).  The cutoff masks 1/q, so there is no per-pair branch.
 // OBS adds the per-atom PE and virial (thermo steps and NPT), CAP the
 // push-off pair-force cap, ONE_TYPE a single (eps, sigma) pair.
-template <bool OBS, bool CAP, bool ONE_TYPE>
-__device__ __forceinline__ void lj_loop(const RepDesc& d, int lt, int lane, int i, int4 pi, float L,
-                                        float& fx, float& fy, float& fz, float& pe,
-                                        float& w0, float& w1, float& w2, float& w3, float& w4,
-                                        float& w5) {
+struct LJSum {
+    float fx, fy, fz, pe, w0, w1, w2, w3, w4, w5;  // force, PE and virial of one atom's rows
+};
+
+template <bool OBS, bool CAP, bool ONE_TYPE, bool W16>
+__device__ __forceinline__ LJSum lj_loop(const RepDesc& d, int lt, int lane, int i, int4 pi, float L) {
     const int nn = d.nnbr[i];
     const FMD_GLOBAL int* nl = gview(d.nbr) + (size_t)lt * d.kcap * kTile + lane;
+    const FMD_GLOBAL unsigned short* nl16 =
+        (const FMD_GLOBAL unsigned short*)(gview(d.nbr) + (size_t)lt * d.kcap * kTile) + lane;
     const __amdgpu_buffer_rsrc_t prs =
         __builtin_amdgcn_make_buffer_rsrc((void*)d.ipos, 0, d.nslots * (int)sizeof(int4), kBufferWord3);
     const float u = L * 2.3283064365386963e-10f;
@@ -780,12 +802,12 @@ __device__ __forceinline__ void lj_loop(const RepDesc& d, int lt, int lane, int 
     const float2 p0 = ljrow[0];
     const float s2g0 = p0.y * p0.y * iu2, c24g0 = 24.0f * p0.x * iu2, c2e0 = 2.0f * p0.x;
     const float capg = CAP ? d.fcap / u : 0.0f;  // fcap / r = (fcap / u) / sqrt(q)
-    float gx = 0.f, gy = 0.f, gz = 0.f;
+    float gx = 0.f, gy = 0.f, gz = 0.f, pe = 0.f;
     float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
     unsigned nxt[kLJChunk];
     if (nn > 0) {
 #pragma unroll
-        for (int k = 0; k < kLJChunk; k++) nxt[k] = (unsigned)nl[k * kTile];
+        for (int k = 0; k < kLJChunk; k++) nxt[k] = W16 ? (unsigned)nl16[k * kTile] : (unsigned)nl[k * kTile];
     }
     for (int k0 = 0; k0 < nn; k0 += kLJChunk) {
         uint4 pjs[kLJChunk];
@@ -796,7 +818,9 @@ __device__ __forceinline__ void lj_loop(const RepDesc& d, int lt, int lane, int 
         }
         if (k0 + kLJChunk < nn) {
 #pragma unroll
-            for (int k = 0; k < kLJChunk; k++) nxt[k] = (unsigned)nl[(k0 + kLJChunk + k) * kTile];
+            for (int k = 0; k < kLJChunk; k++)
+                nxt[k] = W16 ? (unsigned)nl16[(k0 + kLJChunk + k) * kTile]
+                             : (unsigned)nl[(k0 + kLJChunk + k) * kTile];
         }
 #pragma unroll
         for (int k = 0; k < kLJChunk; k++) {
@@ -828,27 +852,20 @@ __device__ __forceinline__ void lj_loop(const RepDesc& d, int lt, int lane, int 
             }
         }
     }
-    fx += gx * u; fy += gy * u; fz += gz * u;
-    if (OBS) {
-        const float hu2 = 0.5f * u * u;
-        w0 += v0 * hu2; w1 += v1 * hu2; w2 += v2 * hu2;
-        w3 += v3 * hu2; w4 += v4 * hu2; w5 += v5 * hu2;
-    }
+    const float hu2 = 0.5f * u * u;
+    return LJSum{gx * u, gy * u, gz * u, pe, v0 * hu2, v1 * hu2, v2 * hu2, v3 * hu2, v4 * hu2, v5 * hu2};
 }
 
-#define LJ_ARGS d, t.lt, lane, i, pi, L, fx, fy, fz, pe, w0, w1, w2, w3, w4, w5
-__device__ __forceinline__ void lj_dispatch(const RepDesc& d, const Tile& t, int lane, int i, int4 pi,
-                                            float L, bool obs, float& fx, float& fy,
-                                            float& fz, float& pe, float& w0, float& w1, float& w2,
-                                            float& w3, float& w4, float& w5) {
+#define LJ_ARGS d, t.lt, lane, i, pi, L
+template <bool W16>
+__device__ __forceinline__ LJSum lj_variants(const RepDesc& d, const Tile& t, int lane, int i, int4 pi,
+                                             float L, bool obs) {
     const bool cap = d.fcap > 0.0f, one = d.ntypes == 1;
-    if (!cap && one) {
-        if (obs) lj_loop<true, false, true>(LJ_ARGS); else lj_loop<false, false, true>(LJ_ARGS);
-    } else if (!cap) {
-        if (obs) lj_loop<true, false, false>(LJ_ARGS); else lj_loop<false, false, false>(LJ_ARGS);
-    } else {
-        if (obs) lj_loop<true, true, false>(LJ_ARGS); else lj_loop<false, true, false>(LJ_ARGS);
-    }
+    if (!cap && one)
+        return obs ? lj_loop<true, false, true, W16>(LJ_ARGS) : lj_loop<false, false, true, W16>(LJ_ARGS);
+    if (!cap)
+        return obs ? lj_loop<true, false, false, W16>(LJ_ARGS) : lj_loop<false, false, false, W16>(LJ_ARGS);
+    return obs ? lj_loop<true, true, false, W16>(LJ_ARGS) : lj_loop<false, true, false, W16>(LJ_ARGS);
 }
 #undef LJ_ARGS
 
@@ -881,9 +898,13 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
         const int ti = pi.w;
         const float L = st->L;
         const float ulen = L * 2.3283064365386963e-10f;  // fixed-point step, L * 2^-32
-        if (terms & 1)
-            lj_dispatch(d, t, lane, i, pi, L, always || st->want_obs, fx, fy, fz, pe, w0, w1, w2, w3, w4,
-                        w5);
+        const bool obs = always || st->want_obs;
+        if (terms & 1) {
+            const LJSum a = d.idx16 ? lj_variants<true>(d, t, lane, i, pi, L, obs)
+                                    : lj_variants<false>(d, t, lane, i, pi, L, obs);
+            fx = a.fx; fy = a.fy; fz = a.fz; pe = a.pe;
+            w0 = a.w0; w1 = a.w1; w2 = a.w2; w3 = a.w3; w4 = a.w4; w5 = a.w5;
+        }
         const int hdr = (terms & 6) ? d.bhdr[i] : 0;
         if (terms & 2) {
             const int nb = hdr & 0xFF;
@@ -1172,15 +1193,16 @@ void launch_force(const Launch& l, int t0, int nt, int block, int terms, int kic
     if (nt <= 0) return;
     block = valid_block(block);
     const int g = cdiv(nt, block / 64);
+    const size_t lds = 0;
     switch (terms & 7) {
-        case 1: k_force<1><<<g, block, 0, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
-        case 2: k_force<2><<<g, block, 0, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
-        case 3: k_force<3><<<g, block, 0, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
-        case 4: k_force<4><<<g, block, 0, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
-        case 5: k_force<5><<<g, block, 0, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
-        case 6: k_force<6><<<g, block, 0, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
-        case 7: k_force<7><<<g, block, 0, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
-        default: k_force<0><<<g, block, 0, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
+        case 1: k_force<1><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
+        case 2: k_force<2><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
+        case 3: k_force<3><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
+        case 4: k_force<4><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
+        case 5: k_force<5><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
+        case 6: k_force<6><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
+        case 7: k_force<7><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
+        default: k_force<0><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
     }
 }
 void launch_boundary(const Launch& l, int mode) {

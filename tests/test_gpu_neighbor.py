@@ -58,7 +58,9 @@ def test_pairs_reference_lj64(fmd, engine, orc, golden):
     engine.remove(sid)
 
 
-@pytest.mark.parametrize("n,L,seed", [(4000, 16.77, 4), (2000, 6.0, 5), (700, 9.3, 6)])
+# (300, 17.0): sparse cells, so a tile's candidate cells exceed the LDS
+# staging limits and the list build takes its global-memory scan
+@pytest.mark.parametrize("n,L,seed", [(4000, 16.77, 4), (2000, 6.0, 5), (700, 9.3, 6), (300, 17.0, 7)])
 def test_pairs_random(fmd, engine, orc, n, L, seed):
     rng = np.random.default_rng(seed)
     p4 = pos4_of((rng.random((n, 3)) * L).astype(np.float32))
@@ -83,3 +85,28 @@ def test_pairs_kg_melt_with_exclusions_after_steps(fmd, engine, orc):
     bset = _as_set(d.bonds) | _as_set(d.bonds[:, ::-1])
     assert not (_as_set(got) & bset)
     engine.remove(sid)
+
+
+def test_pairs_and_forces_with_32bit_rows(fmd, engine, orc, monkeypatch):
+    """Replicas above 65536 slots keep 32-bit neighbor rows; FMD_IDX32 selects
+    that layout for a small melt, whose pair set and forces must not change."""
+    d = relax_kg(fmd, engine, 10, 100, seed=9)
+    sids = {}
+    for wide in (False, True):
+        if wide:
+            monkeypatch.setenv("FMD_IDX32", "1")
+        else:
+            monkeypatch.delenv("FMD_IDX32", raising=False)
+        sids[wide] = engine.add(d)
+    monkeypatch.delenv("FMD_IDX32", raising=False)
+    p4, _, _ = engine.state(sids[False])
+    excl = orc.exclusions(len(p4), d.bonds)
+    ref = orc.pair_set(p4, np.float32(d.c.L), np.float32(2.5) + np.float32(0.3), excl)
+    f16 = engine.forces(sids[False])
+    f32 = engine.forces(sids[True])
+    assert np.array_equal(engine.pairs(sids[False]), ref)
+    assert np.array_equal(engine.pairs(sids[True]), ref)
+    for a, b in zip(f16, f32):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+    for s in sids.values():
+        engine.remove(s)
