@@ -607,6 +607,7 @@ static int32_t err_code(int e) {
     if (e & ERR_STRETCH) return FMD_ERR_STRETCH;
     if (e & ERR_CELL) return FMD_ERR_CELL;
     if (e & ERR_BONDED) return FMD_ERR_ARG;
+    if (e & ERR_INTERNAL) return FMD_ERR_STATE;
     return FMD_OK;
 }
 

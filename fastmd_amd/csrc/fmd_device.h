@@ -33,6 +33,7 @@ enum ErrBits : int {
     ERR_NAN = 4,
     ERR_CELL = 8,
     ERR_BONDED = 16,
+    ERR_INTERNAL = 32,  // an internal consistency check failed
 };
 
 struct Tile {
@@ -151,8 +152,12 @@ __host__ __device__ inline int type_of(float w) { return __builtin_bit_cast(int,
 __host__ __device__ inline float type_bits(int t) { return __builtin_bit_cast(float, t); }
 
 // min_image of src/core/pbc.cuh:4-6
+// dr - L * rint(dr * invL) (src/core/pbc.cuh:4-12).  For wrapped positions
+// |dr| <= L, so the image shift m = rint(dr * invL) is -1, 0 or 1 and L * m is
+// exact: the single-rounding fma(-L, m, dr) returns the same bits as the
+// two-rounding form, in one instruction fewer.
 __device__ inline float min_image_ref(float dr, float L, float invL) {
-    return dr - L * rintf(dr * invL);
+    return fmaf(-L, rintf(dr * invL), dr);
 }
 __device__ inline float dist2_ref(float4 a, float4 b, float L, float invL) {
     float dx = min_image_ref(a.x - b.x, L, invL);

@@ -36,6 +36,16 @@ __device__ inline double wave_sum(double x) {
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
     return x;
 }
+__device__ inline int wave_min_i(int x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o, 64));
+    return x;
+}
+__device__ inline int wave_max_i(int x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, 64));
+    return x;
+}
 __device__ inline int wave_or(int x) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o, 64);
@@ -246,7 +256,7 @@ __global__ void __launch_bounds__(256) k_pre(const RepDesc* __restrict__ descs,
                 isfinite(v.y) && isfinite(v.z));
         d.pos[i] = r;
         d.vel[i] = v;
-        d.img[i] = im;
+        if (ix | iy | iz) d.img[i] = im;  // image counts change only on a wrap
         const double sc = 4294967296.0 / (double)L;
         d.ipos[i] = make_int4((int)fix_coord(r.x, sc), (int)fix_coord(r.y, sc), (int)fix_coord(r.z, sc),
                               __float_as_int(r.w));
@@ -503,7 +513,7 @@ __global__ void __launch_bounds__(256) k_gather(REBUILD_ARGS) {
 // order, exactly as the global-memory scan that serves tiles whose union does
 // not fit.  Both produce the same rows.
 // ---------------------------------------------------------------------------
-constexpr int kMaxOwn = 64;        // distinct cells holding the tile's atoms
+constexpr int kMaxWin = 1024;      // cells of a tile's neighbor window (bitmap words <= 32)
 
 // set bits of m below this lane
 __device__ inline int lane_prefix(unsigned long long m) {
@@ -552,9 +562,10 @@ __device__ __forceinline__ void build_chunk(const float4 (&pj)[kBuildChunk], int
 }
 
 __global__ void __launch_bounds__(64) k_build(REBUILD_ARGS) {
-    __shared__ float4 s_pos[kStageAtoms];
+    __shared__ float s_px[kStageAtoms], s_py[kStageAtoms], s_pz[kStageAtoms];  // staged positions (SoA)
     __shared__ int4 s_cell[kMaxUnion];             // (first slot, count, LDS offset, key)
-    __shared__ unsigned char s_map[kMaxOwn * 27];  // own cell x neighbor -> union index
+    __shared__ unsigned int s_bits[kMaxWin / 32];   // union cells of the window
+    __shared__ unsigned char s_wmap[kMaxWin];       // window cell -> union index
     __shared__ unsigned char s_seg[kStageAtoms];   // LDS atom -> union index
     REBUILD_LOOP {
         REBUILD_PROLOGUE
@@ -607,58 +618,90 @@ __global__ void __launch_bounds__(64) k_build(REBUILD_ARGS) {
         if (valid) {
             cx = cell_coord(pi.x, ics, nx); cy = cell_coord(pi.y, ics, nx); cz = cell_coord(pi.z, ics, nx);
         }
-        const int key = (int)cell_key(cx, cy, cz);
-        // distinct cells of the tile: atoms are sorted by cell key, so a cell
-        // starts where the key differs from the previous lane's
-        const int prev = __shfl_up(key, 1, 64);
-        const unsigned long long first = __ballot(valid && (lane == 0 || key != prev));
-        const int nown = __popcll(first);
-        const int own = lane_prefix(first) + ((first >> lane) & 1ull) - 1;  // this lane's cell index
-        // union of the 27-neighborhoods of the tile's cells
-        bool staged = nown <= kMaxOwn;
+        // The tile's cells lie in a small box of cell coordinates (taken relative
+        // to lane 0's cell, modulo the box).  Its 27-neighborhoods fall in that
+        // box grown by one cell per side (the whole axis if that wraps): a window
+        // of at most kMaxWin cells.  A bitmap over the window marks the union,
+        // whose cells are numbered in window order.
+        const int h = nx >> 1;
+        auto rel = [&](int c, int c0) {
+            int r = c - c0;
+            r += (r < -h) ? nx : 0;
+            r -= (r >= nx - h) ? nx : 0;
+            return r;
+        };
+        const int c0x = __shfl(cx, 0, 64), c0y = __shfl(cy, 0, 64), c0z = __shfl(cz, 0, 64);
+        const int rx = valid ? rel(cx, c0x) : 0, ry = valid ? rel(cy, c0y) : 0, rz = valid ? rel(cz, c0z) : 0;
+        int sx, sy, sz, ex_, ey_, ez_;
+        auto window = [&](int r, int& s0, int& e0) {
+            const int lo = wave_min_i(r), hi = wave_max_i(r);
+            s0 = lo - 1;
+            e0 = min(hi - lo + 3, nx);  // the whole axis once the grown box wraps
+        };
+        window(rx, sx, ex_);
+        window(ry, sy, ey_);
+        window(rz, sz, ez_);
+        // window cell of this atom's neighbor cell c in (dz, dy, dx) order; the
+        // offset from the window start is >= 0 and, on an axis the window
+        // wraps (extent nx), taken modulo nx
+        auto wmod = [](int v, int e) {  // v in [0, e + 1]: v mod e (e >= 1)
+            v -= (v >= e) ? e : 0;
+            v -= (v >= e) ? e : 0;
+            return v;
+        };
+        auto win_of = [&](int c) {
+            const int wx = wmod(rx + c % 3 - 1 - sx, ex_), wy = wmod(ry + (c / 3) % 3 - 1 - sy, ey_),
+                      wz = wmod(rz + c / 9 - 1 - sz, ez_);
+            return (wz * ey_ + wy) * ex_ + wx;
+        };
+        const int W = ex_ * ey_ * ez_;
+        bool staged = W <= kMaxWin;
         int nu = 0;
         if (staged) {
-            unsigned long long rest = first;
-            for (int o = 0; o < nown && staged; o++) {
-                const int src = __builtin_ctzll(rest);
-                rest &= rest - 1;
-                const int ox = __shfl(cx, src, 64), oy = __shfl(cy, src, 64), oz = __shfl(cz, src, 64);
-                int k = -1;
-                bool fresh = false;
-                if (lane < 27) {
-                    k = (int)cell_key(wrap_cell(ox, lane % 3 - 1, nx), wrap_cell(oy, (lane / 3) % 3 - 1, nx),
-                                      wrap_cell(oz, lane / 9 - 1, nx));
-                    int u = -1;
-                    for (int v = 0; v < nu; v++) u = (s_cell[v].w == k) ? v : u;
-                    // duplicates among the 27 (boxes under 3 cells wide): the lowest lane adds the cell
-                    int dup = -1;
-                    for (int m = 0; m < 27; m++) {
-                        const int km = __shfl(k, m, 64);
-                        if (m < lane && km == k && dup < 0) dup = m;
-                    }
-                    fresh = (u < 0) && (dup < 0);
-                    s_map[o * 27 + lane] = (unsigned char)(u < 0 ? 255 : u);  // new cells filled below
+            for (int j = lane; j < kMaxWin / 32; j += kTile) s_bits[j] = 0u;
+            __syncthreads();
+            // each cell of the tile marks its 27 neighbors (its first lane does)
+            const int prev = __shfl_up(cx + nx * (cy + nx * cz), 1, 64);
+            if (valid && (lane == 0 || prev != cx + nx * (cy + nx * cz)))
+                for (int c = 0; c < 27; c++) {
+                    const int w = win_of(c);
+                    atomicOr(&s_bits[w >> 5], 1u << (w & 31));
                 }
-                const unsigned long long nm = __ballot(fresh);
-                const int slot = nu + lane_prefix(nm);
-                if (nu + __popcll(nm) > kMaxUnion) {
-                    staged = false;
-                } else {
-                    if (fresh) {
-                        s_cell[slot] = make_int4(0, 0, 0, k);
-                        s_map[o * 27 + lane] = (unsigned char)slot;
-                    }
-                    __syncthreads();
-                    if (lane < 27 && !fresh && s_map[o * 27 + lane] == 255) {
-                        // a duplicate of a cell added by a lower lane of this round
-                        int u = 0;
-                        for (int v = nu; v < nu + __popcll(nm); v++) u = (s_cell[v].w == k) ? v : u;
-                        s_map[o * 27 + lane] = (unsigned char)u;
-                    }
-                    nu += __popcll(nm);
-                }
-                __syncthreads();
+            __syncthreads();
+            // number the marked cells in window order
+            const int nwords = (W + 31) >> 5;  // <= kMaxWin / 32 <= 64 lanes
+            const unsigned bits = (lane < nwords) ? s_bits[lane] : 0u;
+            int x = __popc(bits);
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(x, o, 64);
+                if (lane >= o) x += y;
             }
+            nu = __shfl(x, 63, 64);
+            staged = nu <= kMaxUnion;
+            if (staged) {
+                // lane l numbers bit (l & 31) of word j + (l >> 5): two words per pass
+                const int base = x - __popc(bits);  // marked cells in words below this lane's word
+                auto wrapn = [&](int p) {           // p in [-nx, 2 nx)
+                    p += (p < 0) ? nx : 0;
+                    p -= (p >= nx) ? nx : 0;
+                    return p;
+                };
+                for (int j = 0; j < nwords; j += 2) {
+                    const int word = j + (lane >> 5), bit = lane & 31;
+                    const unsigned wb = __shfl(bits, word & 63, 64);
+                    const int wbase = __shfl(base, word & 63, 64);
+                    if (word < nwords && ((wb >> bit) & 1u)) {
+                        const int w = word * 32 + bit;
+                        const int u = wbase + __popc(wb & ((1u << bit) - 1u));
+                        const int wx = w % ex_, wy = (w / ex_) % ey_, wz = w / (ex_ * ey_);
+                        s_wmap[w] = (unsigned char)u;
+                        s_cell[u] = make_int4(0, 0, 0, (int)cell_key(wrapn(c0x + sx + wx), wrapn(c0y + sy + wy),
+                                                                     wrapn(c0z + sz + wz)));
+                    }
+                }
+            }
+            __syncthreads();
         }
         // counts, LDS offsets and the staged positions
         int nstage = 0;
@@ -690,18 +733,22 @@ __global__ void __launch_bounds__(64) k_build(REBUILD_ARGS) {
                 for (int j = 0; j < ce.y; j++) s_seg[ce.z + j] = (unsigned char)u;
             }
             __syncthreads();
-            for (int q0 = 0; q0 < nstage; q0 += 4 * kTile) {
-                float4 v4[4];
+            for (int q0 = 0; q0 < nstage; q0 += 8 * kTile) {
+                float4 v4[8];
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
+                for (int k = 0; k < 8; k++) {
                     const int q = min(q0 + k * kTile + lane, nstage - 1);
                     const int4 ce = s_cell[s_seg[q]];
                     v4[k] = gpos[ce.x + q - ce.z];
                 }
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
+                for (int k = 0; k < 8; k++) {
                     const int q = q0 + k * kTile + lane;
-                    if (q < nstage) s_pos[q] = v4[k];
+                    if (q < nstage) {
+                        s_px[q] = v4[k].x;
+                        s_py[q] = v4[k].y;
+                        s_pz[q] = v4[k].z;
+                    }
                 }
             }
             __syncthreads();
@@ -709,16 +756,24 @@ __global__ void __launch_bounds__(64) k_build(REBUILD_ARGS) {
         int cnt = 0;
         if (valid && staged) {
             for (int c = 0; c < 27; c++) {
-                const int u = s_map[own * 27 + c];
+                const int w = win_of(c);
                 bool seen = false;
                 if (nx < 3)  // small boxes visit a cell once (verlet_list.cu:154-187)
-                    for (int v = 0; v < c; v++) seen |= (s_map[own * 27 + v] == u);
+                    for (int v = 0; v < c; v++) seen |= (win_of(v) == w);
                 if (seen) continue;
+                const int u = s_wmap[w];
+                if (u >= nu) {  // every neighbor cell was marked; guard the LDS lookups regardless
+                    err |= ERR_INTERNAL;
+                    continue;
+                }
                 const int4 ce = s_cell[u];
                 for (int q0 = 0; q0 < ce.y; q0 += kBuildChunk) {
                     float4 pj[kBuildChunk];
 #pragma unroll
-                    for (int k = 0; k < kBuildChunk; k++) pj[k] = s_pos[ce.z + min(q0 + k, ce.y - 1)];
+                    for (int k = 0; k < kBuildChunk; k++) {
+                        const int q = ce.z + min(q0 + k, ce.y - 1);
+                        pj[k] = make_float4(s_px[q], s_py[q], s_pz[q], 0.0f);
+                    }
                     if (d.idx16)
                         build_chunk<true>(pj, ce.x + q0, ce.y - q0, i, pi, L, invL, rcs2, ex, nb, d.bcap, bt, out,
                                           out16, kcap, cnt);

@@ -72,6 +72,25 @@ def test_fixed_point_displacement(orc, L):
         assert orc.lib().orc_min_image_fixed(y, x, L) == -got or abs(got) < 3 * q
 
 
+def test_min_image_single_rounding_form_is_exact():
+    """The list build evaluates dr - L * rint(dr / L) as fma(-L, m, dr): for
+    wrapped positions m is -1, 0 or 1, L * m is exact, so one rounding of the
+    exact dr - L m equals the reference's two-step FP32 result.  Checked with
+    FP64 (where dr - L m is exact) rounded once to FP32."""
+    rng = np.random.default_rng(3)
+    for L in (F(6.0), F(13.3), F(33.6), F(105.567)):
+        a = (rng.random(200000) * L).astype(F)
+        b = (rng.random(200000) * L).astype(F)
+        a[:4] = [0.0, np.nextafter(L, 0, dtype=F), L, F(1e-7)]
+        b[:4] = [np.nextafter(L, 0, dtype=F), 0.0, 0.0, L]
+        dr = (a - b).astype(F)
+        m = np.rint(dr * (F(1) / L)).astype(F)
+        assert set(np.unique(m)) <= {F(-1), F(0), F(1)}
+        two_step = (dr - (L * m).astype(F)).astype(F)
+        fused = (dr.astype(np.float64) - np.float64(L) * m.astype(np.float64)).astype(F)
+        assert np.array_equal(two_step.view(np.uint32), fused.view(np.uint32))
+
+
 @pytest.mark.parametrize("L", [6.0, 31.7, 105.6])
 def test_lj_grid_cutoff_matches_exact_distance(orc, L):
     """The device's grid-unit LJ test equals the exact minimum-image test
