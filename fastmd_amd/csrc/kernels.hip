@@ -24,6 +24,18 @@ __device__ inline int wave_in_grid() {
     return __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
 }
 __device__ inline int waves_in_grid() { return gridDim.x * (blockDim.x >> 6); }
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Renumbering block b to (b % 8) * (G / 8) + b / 8 (for the first G - G % 8
+// blocks) gives each XCD one contiguous range of tiles, so a replica's
+// positions are gathered through one L2 instead of eight.
+constexpr int kXcds = 8;
+__device__ inline int xcd_block() {
+    const int b = blockIdx.x, g = gridDim.x, g8 = g - g % kXcds;
+    return b < g8 ? (b % kXcds) * (g8 / kXcds) + b / kXcds : b;
+}
+__device__ inline int xcd_wave_in_grid() {
+    return __builtin_amdgcn_readfirstlane(xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6));
+}
 __device__ inline int lane_id() { return threadIdx.x & 63; }
 
 __device__ inline float wave_max(float x) {
@@ -936,7 +948,7 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
                                                const Tile* __restrict__ tiles, int t0, int ntl,
                                                int kick, int always) {
     constexpr int terms = TERMS;
-    const int w = t0 + wave_in_grid();
+    const int w = t0 + xcd_wave_in_grid();
     if (w >= t0 + ntl) return;
     const Tile t = tiles[w];
     const RepDesc& d = descs[t.rep];
