@@ -19,6 +19,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the LJ kernel on the default workload
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_force_lj_kg256_hbm_bytes.txt")
+
+
+def profiled_traffic():
+    """HBM bytes per LJ launch from the committed counter summary (default workload only)."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
 
 
 def log(msg):
@@ -196,6 +210,8 @@ def main():
         log(f"served: {served['jobs']} jobs at {served['rate_jobs_per_s']:.2f}/s, p50 {served['p50_s']:.3f} s, "
             f"p99 {served['p99_s']:.3f} s, {served['bead_steps_per_s']:.4g} bead-steps/s")
 
+    default_workload = args.replicas == 256
+    traffic = profiled_traffic() if default_workload else None
     cpu = None if args.no_cpu else cpu_baseline(eng, sids, specs)
     if cpu:
         log(f"cpu baseline: {cpu['value']:.4g} bead-steps/s on {cpu['cores']} threads")
@@ -220,7 +236,8 @@ def main():
             "replicas": len(sids), "beads": total, "rc": 2.5, "skin": 0.3,
         },
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
                      "kernel": "k_force<1> (LJ over the Verlet list)", "ms_per_launch": ms_lj,
                      "alg_bytes_per_launch": bytes_lj, "listed_pairs": pairs},
         "cpu_baseline": cpu,
