@@ -147,6 +147,9 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--workload", choices=("replicas", "single"), default="replicas",
+                    help="replicas: the uneven-replica batch (default); single: one 10,000 x 100-bead "
+                         "melt (1,000,000 beads), Langevin T=1")
     ap.add_argument("--plain-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
@@ -169,7 +172,12 @@ def main():
     from fastmd_amd import workloads
 
     eng = fmd.Engine(0)
-    specs = workloads.uneven_replica_specs(args.replicas)
+    single = args.workload == "single"
+    if single:
+        specs = [{"nchains": 10000, "chain_len": 100, "T": 1.0, "seed": 42, "gen_seed": 2024}]
+        args.serve_window = 0.0
+    else:
+        specs = workloads.uneven_replica_specs(args.replicas)
     t0 = time.perf_counter()
     descs = []
     sids = workloads.relaxed_replicas(eng, specs, log=log, descs=descs)
@@ -210,7 +218,7 @@ def main():
         log(f"served: {served['jobs']} jobs at {served['rate_jobs_per_s']:.2f}/s, p50 {served['p50_s']:.3f} s, "
             f"p99 {served['p99_s']:.3f} s, {served['bead_steps_per_s']:.4g} bead-steps/s")
 
-    default_workload = args.replicas == 256
+    default_workload = args.replicas == 256 and not single
     traffic = profiled_traffic() if default_workload else None
     cpu = None if args.no_cpu else cpu_baseline(eng, sids, specs)
     if cpu:
@@ -230,9 +238,12 @@ def main():
         "dtype": "f32 (FP64 reductions)",
         "data": "synthetic: generated KG melts relaxed by a capped soft push-off on the GPU",
         "config": {
-            "workload": f"{len(sids)} uneven Kremer-Grest melt replicas, chain length log-uniform [20,200], "
-                        f"beads log-uniform [2000,64000], rho 0.85, Langevin T U[0.8,1.2], dt 0.001, "
-                        f"batched hipGraph step",
+            "workload": (f"one Kremer-Grest melt of 10,000 chains x 100 beads ({total} beads), rho 0.85, "
+                         f"Langevin T=1, dt 0.001, hipGraph step; plain = one launch per kernel per step")
+            if single else
+            (f"{len(sids)} uneven Kremer-Grest melt replicas, chain length log-uniform [20,200], "
+             f"beads log-uniform [2000,64000], rho 0.85, Langevin T U[0.8,1.2], dt 0.001, "
+             f"batched hipGraph step"),
             "replicas": len(sids), "beads": total, "rc": 2.5, "skin": 0.3,
         },
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",

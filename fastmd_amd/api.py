@@ -101,6 +101,7 @@ SIGNATURES = [
     ("fmd_system_step", C.c_int64, [C.c_void_p, C.c_int32]),
     ("fmd_get_state", C.c_int32, [C.c_void_p, C.c_int32, _fp, _fp, _ip]),
     ("fmd_get_box", C.c_int32, [C.c_void_p, C.c_int32, _dp]),
+    ("fmd_get_state_box", C.c_int32, [C.c_void_p, C.c_int32, _dp]),
     ("fmd_get_thermo", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(Thermo)]),
     ("fmd_drain_thermo", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(Thermo), C.c_int32, _ip,
                                      C.POINTER(C.c_int64)]),
@@ -382,6 +383,12 @@ class Engine:
     def box(self, sid):
         L = C.c_double()
         self._chk(lib().fmd_get_box(self.h, sid, C.byref(L)))
+        return L.value
+
+    def state_box(self, sid):
+        """Box side of the positions state() returns (differs from box() under NPT between steps)."""
+        L = C.c_double()
+        self._chk(lib().fmd_get_state_box(self.h, sid, C.byref(L)))
         return L.value
 
     def thermo(self, sid):

@@ -145,6 +145,9 @@ int32_t fmd_get_state(fmd_engine* e, int32_t sys, float* pos4, float* vel3, int3
     return e ? e->e->get_state(sys, pos4, vel3, image3) : FMD_ERR_ARG;
 }
 int32_t fmd_get_box(fmd_engine* e, int32_t sys, double* L) { return e ? e->e->get_box(sys, L) : FMD_ERR_ARG; }
+int32_t fmd_get_state_box(fmd_engine* e, int32_t sys, double* L) {
+    return e ? e->e->get_box(sys, L, true) : FMD_ERR_ARG;
+}
 int32_t fmd_get_thermo(fmd_engine* e, int32_t sys, fmd_thermo* out) {
     return e ? e->e->get_thermo(sys, out) : FMD_ERR_ARG;
 }

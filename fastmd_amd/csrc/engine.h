@@ -38,6 +38,8 @@ struct Batch {
     int* d_rep_t0 = nullptr;  // first tile of each replica in d_reps order
     int* d_rlist = nullptr;   // compact rebuild tile list (ntl entries)
     int* d_nrt = nullptr;     // its length
+    Tile* d_chunks = nullptr; // (replica, chunk of kChunkTiles tiles) list
+    int nch = 0;
     int ntl = 0, nr = 0;
     int cls_t0[kNumClasses] = {0, 0, 0}, cls_nt[kNumClasses] = {0, 0, 0};
     hipGraphExec_t graph = nullptr;
@@ -79,7 +81,7 @@ class Engine {
     int64_t step_count(int32_t sys);
     int64_t rebuilds(int32_t sys);
     int32_t get_state(int32_t sys, float* pos4, float* vel3, int32_t* img3);
-    int32_t get_box(int32_t sys, double* L);
+    int32_t get_box(int32_t sys, double* L, bool of_positions = false);
     int32_t get_thermo(int32_t sys, fmd_thermo* out);
     int32_t drain_thermo(int32_t sys, fmd_thermo* out, int32_t cap, int32_t* n, int64_t* lost);
 

@@ -28,6 +28,7 @@ static void free_batch(Batch* b) {
     (void)hipFree(b->d_rep_t0);
     (void)hipFree(b->d_rlist);
     (void)hipFree(b->d_nrt);
+    (void)hipFree(b->d_chunks);
     delete b;
 }
 
@@ -130,7 +131,7 @@ static Sizing sizing(const fmd_system_desc* d, int bcap, int acap, int kmult = 1
     b += (uint64_t)s.natoms * 4;
     b += (uint64_t)s.ncell_cap * 8 + 4;
     b += (uint64_t)s.ntiles * 64 * ((uint64_t)s.kcap * 4 + (uint64_t)s.bcap * 4 + (uint64_t)s.acap * 8);
-    b += (uint64_t)s.ntiles * kNumPart * 8;
+    b += (uint64_t)s.ntiles * (kNumPart * 8 + 4) + (uint64_t)((s.ntiles + kChunkTiles - 1) / kChunkTiles) * kNumPart * 8;
     b += (uint64_t)(d->natoms + 1) * 8 + (uint64_t)d->nbonds * 8 + (uint64_t)d->nangles * 48;
     b += sizeof(RepState) + (uint64_t)s.thermo_cap * sizeof(ThermoRec) + 4096;
     s.bytes = b;
@@ -268,6 +269,9 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     D.bnd = (int*)dalloc(r, nt * S.bcap * 64 * 4, &ok);
     D.ang = (int2*)dalloc(r, nt * S.acap * 64 * 8, &ok);
     D.wpart = (double*)dalloc(r, nt * kNumPart * 8, &ok);
+    D.tdisp = (float*)dalloc(r, nt * 4, &ok);
+    D.nchunks = (int)((nt + kChunkTiles - 1) / kChunkTiles);
+    D.cpart = (double*)dalloc(r, (size_t)D.nchunks * kNumPart * 8, &ok);
     r.st = (RepState*)dalloc(r, sizeof(RepState), &ok);
     D.st = r.st;
     D.thermo = (ThermoRec*)dalloc(r, (size_t)std::max(1, S.thermo_cap) * sizeof(ThermoRec), &ok);
@@ -364,6 +368,8 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     S0.nrebuilds = 1;
     S0.L = d->L;
     S0.invL = 1.0f / d->L;
+    S0.Lp = d->L;
+    S0.invLp = 1.0f / d->L;
     {
         int n = (int)(d->L / D.rcs);
         int nx = n < 1 ? 1 : n;
@@ -408,6 +414,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     zero(D.nnbr, ns * 4);
     zero(D.bhdr, ns * 4);
     zero(D.wpart, nt * kNumPart * 8);
+    zero(D.tdisp, nt * 4);
     up(t_boff, boff.data(), (na + 1) * 4);
     up(t_blst, blst.data(), blst.size() * 4);
     up(t_aoff, aoff.data(), (na + 1) * 4);
@@ -511,6 +518,10 @@ Batch* Engine::batch_for(const std::vector<int>& ids_in) {
     }
     b->ntl = (int)tiles.size();
     b->nr = (int)ord.size();
+    std::vector<Tile> chunks;
+    for (int id : ord)
+        for (int j = 0; j < reps_[id]->desc.nchunks; j++) chunks.push_back(Tile{id, j});
+    b->nch = (int)chunks.size();
     if (ord.size() > (size_t)kMaxReplicas) {
         delete b;
         fail(FMD_ERR_ARG, "more than " + std::to_string(kMaxReplicas) + " systems in one batch");
@@ -520,12 +531,14 @@ Batch* Engine::batch_for(const std::vector<int>& ids_in) {
               hipMalloc(&b->d_reps, std::max<size_t>(1, ord.size()) * sizeof(int)) == hipSuccess &&
               hipMalloc(&b->d_rep_t0, std::max<size_t>(1, ord.size()) * sizeof(int)) == hipSuccess &&
               hipMalloc(&b->d_rlist, std::max<size_t>(1, tiles.size()) * sizeof(int)) == hipSuccess &&
-              hipMalloc(&b->d_nrt, sizeof(int)) == hipSuccess;
+              hipMalloc(&b->d_nrt, sizeof(int)) == hipSuccess &&
+              hipMalloc(&b->d_chunks, std::max<size_t>(1, chunks.size()) * sizeof(Tile)) == hipSuccess;
     if (ok)
         ok = hipMemcpy(b->d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice) == hipSuccess &&
              hipMemcpy(b->d_reps, ord.data(), ord.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
              hipMemcpy(b->d_rep_t0, t0s.data(), t0s.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
-             hipMemset(b->d_nrt, 0, sizeof(int)) == hipSuccess;
+             hipMemset(b->d_nrt, 0, sizeof(int)) == hipSuccess &&
+             hipMemcpy(b->d_chunks, chunks.data(), chunks.size() * sizeof(Tile), hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) {
         free_batch(b);
         fail(FMD_ERR_NOMEM, "batch allocation failed");
@@ -536,7 +549,8 @@ Batch* Engine::batch_for(const std::vector<int>& ids_in) {
 }
 
 Launch Engine::launch_of(const Batch& b) const {
-    return Launch{d_descs_, b.d_tiles, b.ntl, b.d_reps, b.nr, b.d_rep_t0, b.d_rlist, b.d_nrt, stream_};
+    return Launch{d_descs_, b.d_tiles, b.ntl, b.d_reps, b.nr, b.d_rep_t0, b.d_rlist, b.d_nrt, b.d_chunks, b.nch,
+                  stream_};
 }
 
 // One MD step for every replica of the batch.
@@ -854,12 +868,12 @@ int32_t Engine::get_state(int32_t sys, float* pos4, float* vel3, int32_t* img3) 
     return FMD_OK;
 }
 
-int32_t Engine::get_box(int32_t sys, double* L) {
+int32_t Engine::get_box(int32_t sys, double* L, bool of_positions) {
     Replica* r = rep(sys);
     if (!r || !L) return fail(FMD_ERR_ARG, "bad arguments");
     RepState s;
     int32_t rc = read_state(*r, s);
-    if (!rc) *L = s.L;
+    if (!rc) *L = of_positions ? s.Lp : s.L;
     return rc;
 }
 

@@ -13,16 +13,17 @@
 
 namespace fmd {
 
-constexpr int kTile = 64;
+constexpr int kTile = 64;           // slots per tile = one wave64
 constexpr int kMaxReplicas = 4096;  // entries of an engine's replica table
 // The list build stages a tile's candidate atoms -- the union of the
 // 27-neighborhoods of the cells holding its atoms -- in LDS when they fit: at
 // most kMaxUnion cells and kStageAtoms atoms.
 constexpr int kMaxUnion = 128;
-constexpr int kStageAtoms = 1792;          // slots per tile = one wave64
+constexpr int kStageAtoms = 1792;
 constexpr int kMaxTypes = 16;
 constexpr int kMaxChain = 10;
-constexpr int kNumPart = 16;       // FP64 partials per wave: pe, W[6], K[6], pad
+constexpr int kNumPart = 16;   // FP64 partials per tile: pe, W[6], K[6], pad
+constexpr int kChunkTiles = 256;  // tiles summed by one workgroup in the first reduction level
 constexpr int kRebuildEvery = 20;  // forced rebuild period (src/main.cu:401)
 constexpr int kSlotBits = 26;      // slot / gid field of packed bonded entries
 constexpr int kSlotMask = (1 << kSlotBits) - 1;
@@ -65,7 +66,8 @@ struct RepState {
     int want_obs;          // the current step records PE and virial (NPT or thermo step)
     int64_t nrebuilds;
     // current box and cell grid (float copies for per-atom kernels)
-    float L, invL;
+    float L, invL;      // box of the next position update (the barostat advances it after the forces)
+    float Lp, invLp;    // box the current positions were placed in
     int nx;                // cells per axis
     int cbits;             // Morton bits per axis of the cell key
     float inv_cs;
@@ -138,6 +140,9 @@ struct RepDesc {
     const float* mass;     // per type
     // reductions and records
     double* wpart;         // [ntiles][kNumPart]
+    float* tdisp;          // [ntiles] largest squared displacement since the last rebuild (-1: non-finite)
+    double* cpart;         // [nchunks][kNumPart] partials summed over chunks of kChunkTiles tiles
+    int nchunks;
     RepState* st;
     ThermoRec* thermo;     // [thermo_cap]
 };

@@ -16,6 +16,8 @@ struct Launch {
     const int* rep_t0;     // first tile of each listed replica
     int* rlist;            // compact list of rebuilding tiles (capacity ntl)
     int* nrt;              // its length
+    const Tile* chunks;    // (replica, chunk) list for the first reduction level
+    int nch;
     hipStream_t stream;
 };
 

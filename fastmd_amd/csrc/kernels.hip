@@ -78,6 +78,7 @@ template <class T> __device__ inline const FMD_GLOBAL T* gview(const T* p) {
 constexpr int kMaxBlockK = 256;  // launch bound of k_build and k_force
 constexpr int kLJChunk = 8;      // neighbors per software-pipelined chunk (kcap % 8 == 0)
 constexpr int kBuildChunk = 8;   // list-build candidates whose loads are issued together
+constexpr int kBndThreads = 256;  // workgroup of the per-replica boundary reduction
 
 #pragma clang fp contract(off)
 __device__ inline float sq3(float x, float y, float z) { return x * x + y * y + z * z; }
@@ -159,9 +160,9 @@ __device__ inline double sinch(double x) {
 // Cell grid of the current box (src/neighbor/verlet_list.cu:8-13), recomputed
 // at every rebuild so the grid follows the box under NPT.
 __device__ void set_grid(const RepDesc& d, RepState* st) {
-    int n = (int)(st->L / d.rcs);
+    int n = (int)(st->Lp / d.rcs);
     int nx = n < 1 ? 1 : n;
-    float cs = st->L / (float)nx;
+    float cs = st->Lp / (float)nx;
     int cb = 0;
     while ((1 << cb) < nx) cb++;
     if ((1 << (3 * cb)) > d.ncell_cap) {
@@ -273,29 +274,56 @@ __global__ void __launch_bounds__(256) k_pre(const RepDesc* __restrict__ descs,
         d.ipos[i] = make_int4((int)fix_coord(r.x, sc), (int)fix_coord(r.y, sc), (int)fix_coord(r.z, sc),
                               __float_as_int(r.w));
     }
+    // the tile's largest squared displacement since the last rebuild (-1: a
+    // non-finite position or velocity), reduced per replica by k_decide; a
+    // per-tile record instead of an atomic on one word per replica, which
+    // serializes a large replica's waves
     const float m = wave_max(bad ? 0.0f : d2);
     const int b = wave_or(bad);
-    if (lane_id() == 0) {
-        if (m > 0.0f) atomicMax(&st->maxdisp_bits, __float_as_int(m));
-        if (b) atomicOr(&st->err, ERR_NAN);
-    }
+    if (lane_id() == 0) d.tdisp[t.lt] = b ? -1.0f : m;
 }
 
 // ---------------------------------------------------------------------------
 // per-replica rebuild decision (src/main.cu:486-496, skin_trigger.cuh:18-33)
 // ---------------------------------------------------------------------------
-__global__ void k_decide(const RepDesc* __restrict__ descs, const int* __restrict__ reps, int nr) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nr) return;
-    const RepDesc& d = descs[reps[k]];
+__global__ void __launch_bounds__(1024) k_decide(const RepDesc* __restrict__ descs,
+                                                 const int* __restrict__ reps, int nr) {
+    __shared__ float s_max[16];
+    __shared__ int s_bad[16];
+    const RepDesc& d = descs[reps[blockIdx.x]];
     RepState* st = d.st;
     if (!st->active) return;
+    float m = 0.0f;
+    int bad = 0;
+    for (int tl = threadIdx.x; tl < d.ntiles; tl += blockDim.x) {
+        const float x = d.tdisp[tl];
+        bad |= (x < 0.0f);
+        m = fmaxf(m, x);
+    }
+    m = wave_max(m);
+    bad = wave_or(bad);
+    if (lane_id() == 0) {
+        s_max[threadIdx.x >> 6] = m;
+        s_bad[threadIdx.x >> 6] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+        m = fmaxf(m, s_max[w]);
+        bad |= s_bad[w];
+    }
+    m = fmaxf(m, s_max[0]);
+    bad |= s_bad[0];
+    if (bad) st->err |= ERR_NAN;
     if (st->err) {
         st->active = 0;
         st->rebuild = 0;
         return;
     }
-    const float md = __int_as_float(st->maxdisp_bits);
+    // k_pre of this step placed the positions in the box st->L
+    st->Lp = st->L;
+    st->invLp = st->invL;
+    const float md = fmaxf(__int_as_float(st->maxdisp_bits), m);  // largest since the last rebuild
     const int since = st->since_rebuild + 1;
     const int rb = (md > d.half_skin2) || (since >= kRebuildEvery);
     st->rebuild = rb;
@@ -306,6 +334,7 @@ __global__ void k_decide(const RepDesc* __restrict__ descs, const int* __restric
         set_grid(d, st);
     } else {
         st->since_rebuild = since;
+        st->maxdisp_bits = __float_as_int(md);
     }
 }
 
@@ -422,44 +451,55 @@ __global__ void __launch_bounds__(256) k_bin(REBUILD_ARGS) {
 // the counts are cleared for the next rebuild.
 __global__ void __launch_bounds__(1024) k_scan(const RepDesc* __restrict__ descs,
                                                const int* __restrict__ reps, int nr, int always) {
+    // chunks of 1024 x 4 keys: thread t holds keys 4t..4t+3 of the chunk
+    // (one 16-byte load; a wave covers 1 KB contiguous), block-wide scan,
+    // then the running offset carries to the next chunk
     __shared__ int wsum[16];
+    __shared__ int carry;
     const RepDesc& d = descs[reps[blockIdx.x]];
     RepState* st = d.st;
     if (!always && !(st->active && st->rebuild)) return;
     const int nkeys = 1 << (3 * st->cbits);
-    const int per = (nkeys + 1023) / 1024;
-    const int b0 = threadIdx.x * per;
-    int local = 0;
-    for (int k = 0; k < per; k++) {
-        const int c = b0 + k;
-        if (c < nkeys) local += d.cell_cnt[c];
-    }
-    // inclusive scan of `local` across the workgroup
-    int x = local;
     const int lane = lane_id(), wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) wsum[wv] = x;
+    if (threadIdx.x == 0) carry = 0;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int k = 0; k < 16; k++) { int s = wsum[k]; wsum[k] = acc; acc += s; }
-    }
-    __syncthreads();
-    int run = wsum[wv] + x - local;  // exclusive prefix of this thread's chunk
-    for (int k = 0; k < per; k++) {
-        const int c = b0 + k;
-        if (c < nkeys) {
-            const int n = d.cell_cnt[c];
-            d.cell_start[c] = run;
-            d.cell_cnt[c] = 0;
-            run += n;
+    for (int base = 0; base < nkeys; base += 4 * 1024) {
+        const int c0 = base + 4 * threadIdx.x;
+        int4 v = make_int4(0, 0, 0, 0);
+        if (c0 + 3 < nkeys) {
+            v = *(const int4*)(d.cell_cnt + c0);
+        } else {
+            if (c0 < nkeys) v.x = d.cell_cnt[c0];
+            if (c0 + 1 < nkeys) v.y = d.cell_cnt[c0 + 1];
+            if (c0 + 2 < nkeys) v.z = d.cell_cnt[c0 + 2];
         }
+        const int local = v.x + v.y + v.z + v.w;
+        int x = local;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wv] = x;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int acc = carry;
+            for (int k = 0; k < 16; k++) { const int t = wsum[k]; wsum[k] = acc; acc += t; }
+            carry = acc;
+        }
+        __syncthreads();
+        const int run = wsum[wv] + x - local;  // exclusive prefix of this thread's keys
+        if (c0 + 3 < nkeys) {
+            *(int4*)(d.cell_start + c0) = make_int4(run, run + v.x, run + v.x + v.y, run + v.x + v.y + v.z);
+            *(int4*)(d.cell_cnt + c0) = make_int4(0, 0, 0, 0);
+        } else {
+            if (c0 < nkeys) { d.cell_start[c0] = run; d.cell_cnt[c0] = 0; }
+            if (c0 + 1 < nkeys) { d.cell_start[c0 + 1] = run + v.x; d.cell_cnt[c0 + 1] = 0; }
+            if (c0 + 2 < nkeys) { d.cell_start[c0 + 2] = run + v.x + v.y; d.cell_cnt[c0 + 2] = 0; }
+        }
+        __syncthreads();  // wsum is rewritten by the next chunk
     }
-    if (threadIdx.x == 1023) d.cell_start[nkeys] = run;
+    if (threadIdx.x == 0) d.cell_start[nkeys] = carry;
 }
 
 __global__ void __launch_bounds__(256) k_scatter(REBUILD_ARGS) {
@@ -504,7 +544,7 @@ __global__ void __launch_bounds__(256) k_gather(REBUILD_ARGS) {
             d.vel[i] = d.tmp_vel[src];
             d.img[i] = im;
             d.slot_of[im.w] = i;
-            const double sc = 4294967296.0 / (double)st->L;
+            const double sc = 4294967296.0 / (double)st->Lp;
             d.ipos[i] = make_int4((int)fix_coord(p.x, sc), (int)fix_coord(p.y, sc), (int)fix_coord(p.z, sc),
                                   __float_as_int(p.w));
         }
@@ -623,7 +663,7 @@ __global__ void __launch_bounds__(64) k_build(REBUILD_ARGS) {
         const float rcs2 = d.rcs2;
         const int nx = st->nx;
         const float ics = st->inv_cs;
-        const float L = st->L, invL = st->invL;
+        const float L = st->Lp, invL = st->invLp;
         const FMD_GLOBAL float4* gpos = gview(d.pos);
         const FMD_GLOBAL int* cstart = gview(d.cell_start);
         int cx = 0, cy = 0, cz = 0;
@@ -963,7 +1003,7 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
     if (i < d.natoms) {
         const int4 pi = d.ipos[i];
         const int ti = pi.w;
-        const float L = st->L;
+        const float L = st->Lp;
         const float ulen = L * 2.3283064365386963e-10f;  // fixed-point step, L * 2^-32
         const bool obs = always || st->want_obs;
         if (terms & 1) {
@@ -1082,14 +1122,43 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
     if (err && lane == 0) atomicOr(&st->err, err);
 }
 
+// First reduction level: one workgroup sums the FP64 partials of kChunkTiles
+// tiles of one replica in a fixed order (thread t takes tile t, then wave
+// shuffles, then the four waves in order), so a large replica's sum is spread
+// over many CUs.
+__global__ void __launch_bounds__(256) k_chunk(const RepDesc* __restrict__ descs,
+                                              const Tile* __restrict__ chunks, int nch, int mode) {
+    __shared__ double red[13][4];
+    const Tile ch = chunks[blockIdx.x];
+    const RepDesc& d = descs[ch.rep];
+    if (mode == BND_STEP && !d.st->active) return;
+    const int tl = ch.lt * kChunkTiles + threadIdx.x;
+    double a[13];
+#pragma unroll
+    for (int c = 0; c < 13; c++) a[c] = 0.0;
+    if (tl < d.ntiles)
+#pragma unroll
+        for (int c = 0; c < 13; c++) a[c] = d.wpart[(size_t)tl * kNumPart + c];
+#pragma unroll
+    for (int c = 0; c < 13; c++) a[c] = wave_sum(a[c]);
+    if (lane_id() == 0)
+#pragma unroll
+        for (int c = 0; c < 13; c++) red[c][threadIdx.x >> 6] = a[c];
+    __syncthreads();
+    if (threadIdx.x < 13) {
+        const int c = threadIdx.x;
+        d.cpart[(size_t)ch.lt * kNumPart + c] = ((red[c][0] + red[c][1]) + red[c][2]) + red[c][3];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Per-replica boundary: fixed-order FP64 reduction of the wave partials, then
 // the scalar integrator work (post-force chain / barostat of this step,
 // start-of-step scalars of the next), thermo record, step counter.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_boundary(const RepDesc* __restrict__ descs,
-                                                  const int* __restrict__ reps, int nr, int mode) {
-    __shared__ double red[13][256];
+__global__ void __launch_bounds__(kBndThreads) k_boundary(const RepDesc* __restrict__ descs,
+                                                          const int* __restrict__ reps, int nr, int mode) {
+    __shared__ double red[13][kBndThreads / 64];
     const RepDesc& d = descs[reps[blockIdx.x]];
     RepState* st = d.st;
     if (mode == BND_START) {
@@ -1097,25 +1166,29 @@ __global__ void __launch_bounds__(256) k_boundary(const RepDesc* __restrict__ de
         return;
     }
     if (mode == BND_STEP && !st->active) return;
-    double acc[13];
-#pragma unroll
-    for (int c = 0; c < 13; c++) acc[c] = 0.0;
-    for (int tl = threadIdx.x; tl < d.ntiles; tl += 256)
-#pragma unroll
-        for (int c = 0; c < 13; c++) acc[c] += d.wpart[(size_t)tl * kNumPart + c];
-#pragma unroll
-    for (int c = 0; c < 13; c++) red[c][threadIdx.x] = acc[c];
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
-#pragma unroll
-            for (int c = 0; c < 13; c++) red[c][threadIdx.x] += red[c][threadIdx.x + s];
-        __syncthreads();
+    // fixed-order reduction: per-thread strided sums, wave shuffles, then the
+    // waves in order (the same order whatever else is in the batch)
+    // second level: lane l of a wave reads component l % 16 of chunk 4 w + l / 16
+    // (+ 4 x waves per pass), the chunks in a fixed order
+    {
+        const int lane = lane_id(), wv = threadIdx.x >> 6, c = lane & 15;
+        double a = 0.0;
+        for (int ck = 4 * wv + (lane >> 4); ck < d.nchunks; ck += 4 * (kBndThreads / 64))
+            a += d.cpart[(size_t)ck * kNumPart + c];
+        a += __shfl_xor(a, 16, 64);
+        a += __shfl_xor(a, 32, 64);
+        if (lane < 13) red[lane][wv] = a;
     }
+    __syncthreads();
     if (threadIdx.x != 0) return;
-    const double pe = red[0][0];
+    double tot[13];
+    for (int c = 0; c < 13; c++) {
+        tot[c] = 0.0;
+        for (int w = 0; w < kBndThreads / 64; w++) tot[c] += red[c][w];
+    }
+    const double pe = tot[0];
     double W[6], K[6];
-    for (int c = 0; c < 6; c++) { W[c] = red[1 + c][0]; K[c] = red[7 + c][0]; }
+    for (int c = 0; c < 6; c++) { W[c] = tot[1 + c]; K[c] = tot[7 + c]; }
     const double KE = 0.5 * (K[0] + K[3] + K[5]);
     if (mode == BND_REDUCE) {
         st->obs_step = st->step;
@@ -1190,9 +1263,9 @@ __global__ void k_ref_cells(const RepDesc* __restrict__ dp, int natoms, int* __r
     if (i >= natoms) return;
     const RepDesc& d = *dp;
     const RepState* st = d.st;
-    int n = (int)(st->L / d.rcs);
+    int n = (int)(st->Lp / d.rcs);
     const int nx = n < 1 ? 1 : n;
-    const float ics = 1.0f / (st->L / (float)nx);
+    const float ics = 1.0f / (st->Lp / (float)nx);
     const float4 r = d.pos[i];
     out[d.img[i].w] = (cell_coord(r.z, ics, nx) * nx + cell_coord(r.y, ics, nx)) * nx +
                       cell_coord(r.x, ics, nx);
@@ -1210,7 +1283,7 @@ __global__ void k_morton_keys(const RepDesc* __restrict__ dp, int natoms, uint32
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= natoms) return;
     const RepDesc& d = *dp;
-    const float invL = d.st->invL;
+    const float invL = d.st->invLp;
     const float4 r = d.pos[i];
     out[d.img[i].w] = (expand_bits(quant1024(r.z, invL)) << 2) |
                       (expand_bits(quant1024(r.y, invL)) << 1) | expand_bits(quant1024(r.x, invL));
@@ -1229,7 +1302,7 @@ void launch_pre(const Launch& l) {
 }
 void launch_decide(const Launch& l) {
     if (l.nr <= 0) return;
-    k_decide<<<cdiv(l.nr, 64), 64, 0, l.stream>>>(l.descs, l.reps, l.nr);
+    k_decide<<<l.nr, 1024, 0, l.stream>>>(l.descs, l.reps, l.nr);
 }
 void launch_control(const Launch& l, int op, int64_t arg) {
     if (l.nr <= 0) return;
@@ -1274,7 +1347,8 @@ void launch_force(const Launch& l, int t0, int nt, int block, int terms, int kic
 }
 void launch_boundary(const Launch& l, int mode) {
     if (l.nr <= 0) return;
-    k_boundary<<<l.nr, 256, 0, l.stream>>>(l.descs, l.reps, l.nr, mode);
+    if (mode != BND_START && l.nch > 0) k_chunk<<<l.nch, 256, 0, l.stream>>>(l.descs, l.chunks, l.nch, mode);
+    k_boundary<<<l.nr, kBndThreads, 0, l.stream>>>(l.descs, l.reps, l.nr, mode);
 }
 void launch_ref_cells(const RepDesc* d, int natoms, int* out, hipStream_t s) {
     if (natoms <= 0) return;

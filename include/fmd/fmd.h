@@ -188,8 +188,14 @@ int32_t fmd_step_plain(fmd_engine* e, int32_t sys, int64_t nsteps);
 int64_t fmd_system_step(fmd_engine* e, int32_t sys);
 /* State by global id: pos4 (x, y, z, type bits), vel3, image3; any may be NULL. */
 int32_t fmd_get_state(fmd_engine* e, int32_t sys, float* pos4, float* vel3, int32_t* image3);
-/* Current box side (changes under NPT). */
+/* Current box side (changes under NPT): as in the reference, the barostat
+ * advances it after the forces of a step (src/integrate/nose_hoover.cu:185-232),
+ * and the positions follow at the next step's position update. */
 int32_t fmd_get_box(fmd_engine* e, int32_t sys, double* L);
+/* Box side the positions returned by fmd_get_state were placed in, which the
+ * parity entry points (fmd_compute_forces, fmd_build_neighbors) use; equal to
+ * fmd_get_box except under NPT between steps.  No reference counterpart. */
+int32_t fmd_get_state_box(fmd_engine* e, int32_t sys, double* L);
 /* Latest reductions (thermo of the last completed step). */
 int32_t fmd_get_thermo(fmd_engine* e, int32_t sys, fmd_thermo* out);
 /* Drains the thermo records written every thermo_freq steps since the last
