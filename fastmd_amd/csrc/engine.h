@@ -24,7 +24,8 @@ struct Replica {
     int ensemble = 0;
     RepDesc desc{};         // host copy of the device descriptor
     RepState* st = nullptr; // device
-    std::vector<void*> allocs;
+    void* block = nullptr;  // the replica's device arrays, carved from one block
+    size_t block_bytes = 0;
     uint64_t bytes = 0;
     int64_t steps_done = 0; // host mirror of st->step
     int64_t thermo_read = 0;
@@ -40,6 +41,10 @@ struct Batch {
     int* d_nrt = nullptr;     // its length
     Tile* d_chunks = nullptr; // (replica, chunk of kChunkTiles tiles) list
     int nch = 0;
+    int4* d_status = nullptr; // per replica (err, active, step lo, step hi), d_reps order
+    std::vector<int> ord;     // host copy of d_reps
+    void* block = nullptr;    // the arrays above, carved from one device block
+    size_t block_bytes = 0;
     int ntl = 0, nr = 0;
     int cls_t0[kNumClasses] = {0, 0, 0}, cls_nt[kNumClasses] = {0, 0, 0};
     hipGraphExec_t graph = nullptr;
@@ -97,7 +102,10 @@ class Engine {
   private:
     int32_t create_impl(const fmd_system_desc* d, int32_t* sys, int kmult);
     int32_t hip_check(hipError_t e, const char* what);
-    void* dalloc(Replica& r, size_t bytes, bool* ok);
+    void* acquire_block(size_t bytes, size_t* got);
+    void release_block(void* p, size_t bytes);
+    void flush_cache(uint64_t keep);
+    void free_batch(Batch* b);
     Batch* batch_for(const std::vector<int>& ids);
     void drop_batches_with(int id);
     void launch_step_seq(Batch& b);
@@ -114,6 +122,8 @@ class Engine {
     std::map<std::vector<int>, Batch*> batches_;
     int tuned_[2][kNumClasses];
     int tune_version_ = 0;
+    std::multimap<size_t, void*> cache_;  // released replica blocks by size
+    uint64_t cached_bytes_ = 0;
 };
 
 }  // namespace fmd

@@ -21,16 +21,6 @@ int size_class(int natoms) { return natoms < 8192 ? 0 : (natoms < 32768 ? 1 : 2)
 
 static inline int ceil64(int n) { return (n + 63) / 64 * 64; }
 
-static void free_batch(Batch* b) {
-    if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    (void)hipFree(b->d_tiles);
-    (void)hipFree(b->d_reps);
-    (void)hipFree(b->d_rep_t0);
-    (void)hipFree(b->d_rlist);
-    (void)hipFree(b->d_nrt);
-    (void)hipFree(b->d_chunks);
-    delete b;
-}
 
 Engine::Engine(int device, uint64_t budget) : device_(device) {
     for (int c = 0; c < kNumClasses; c++) {
@@ -54,9 +44,10 @@ Engine::~Engine() {
     }
     for (Replica* r : reps_)
         if (r) {
-            for (void* p : r->allocs) (void)hipFree(p);
+            (void)hipFree(r->block);
             delete r;
         }
+    flush_cache(0);
     if (d_descs_) (void)hipFree(d_descs_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -77,16 +68,45 @@ Replica* Engine::rep(int32_t sys) {
     return reps_[sys];
 }
 
-void* Engine::dalloc(Replica& r, size_t bytes, bool* ok) {
-    void* p = nullptr;
-    if (bytes == 0) bytes = 16;
-    if (hipMalloc(&p, bytes) != hipSuccess) {
-        *ok = false;
-        return nullptr;
+// Replica blocks are cached when a replica is destroyed and reused for a new
+// one of similar size, so serving does not call hipMalloc / hipFree per job.
+void* Engine::acquire_block(size_t bytes, size_t* got) {
+    auto it = cache_.lower_bound(bytes);
+    if (it != cache_.end() && it->first <= bytes + bytes / 4) {
+        void* p = it->second;
+        *got = it->first;
+        cached_bytes_ -= it->first;
+        cache_.erase(it);
+        return p;
     }
-    r.allocs.push_back(p);
-    r.bytes += bytes;
+    const size_t want = (bytes + (2u << 20) - 1) & ~(size_t)((2u << 20) - 1);  // 2 MiB granules
+    void* p = nullptr;
+    if (hipMalloc(&p, want) != hipSuccess) {
+        (void)hipGetLastError();
+        flush_cache(0);
+        if (hipMalloc(&p, want) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    *got = want;
     return p;
+}
+
+void Engine::release_block(void* p, size_t bytes) {
+    if (!p) return;
+    cache_.emplace(bytes, p);
+    cached_bytes_ += bytes;
+    flush_cache(std::min<uint64_t>(budget_ / 8, 8ull << 30));
+}
+
+void Engine::flush_cache(uint64_t keep) {
+    while (cached_bytes_ > keep && !cache_.empty()) {
+        auto it = std::prev(cache_.end());  // largest first
+        (void)hipFree(it->second);
+        cached_bytes_ -= it->first;
+        cache_.erase(it);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -245,37 +265,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     D.seed_lo = (uint32_t)d->seed;
     D.seed_hi = (uint32_t)(d->seed >> 32);
 
-    bool ok = true;
     const size_t ns = S.nslots, nt = S.ntiles, na = S.natoms;
-    D.pos = (float4*)dalloc(r, ns * 16, &ok);
-    D.vel = (float4*)dalloc(r, ns * 16, &ok);
-    D.force = (float4*)dalloc(r, ns * 16, &ok);
-    D.pos_ref = (float4*)dalloc(r, ns * 16, &ok);
-    D.ipos = (int4*)dalloc(r, ns * 16, &ok);
-    D.img = (int4*)dalloc(r, ns * 16, &ok);
-    D.tmp_pos = (float4*)dalloc(r, ns * 16, &ok);
-    D.tmp_vel = (float4*)dalloc(r, ns * 16, &ok);
-    D.tmp_img = (int4*)dalloc(r, ns * 16, &ok);
-    D.cell_of = (int*)dalloc(r, ns * 4, &ok);
-    D.rank = (int*)dalloc(r, ns * 4, &ok);
-    D.order = (int*)dalloc(r, ns * 4, &ok);
-    D.order2 = (int*)dalloc(r, ns * 4, &ok);
-    D.slot_of = (int*)dalloc(r, na * 4, &ok);
-    D.cell_cnt = (int*)dalloc(r, (size_t)S.ncell_cap * 4, &ok);
-    D.cell_start = (int*)dalloc(r, ((size_t)S.ncell_cap + 1) * 4, &ok);
-    D.nnbr = (int*)dalloc(r, ns * 4, &ok);
-    D.nbr = (int*)dalloc(r, nt * S.kcap * 64 * 4, &ok);
-    D.bhdr = (int*)dalloc(r, ns * 4, &ok);
-    D.bnd = (int*)dalloc(r, nt * S.bcap * 64 * 4, &ok);
-    D.ang = (int2*)dalloc(r, nt * S.acap * 64 * 8, &ok);
-    D.wpart = (double*)dalloc(r, nt * kNumPart * 8, &ok);
-    D.tdisp = (float*)dalloc(r, nt * 4, &ok);
-    D.nchunks = (int)((nt + kChunkTiles - 1) / kChunkTiles);
-    D.cpart = (double*)dalloc(r, (size_t)D.nchunks * kNumPart * 8, &ok);
-    r.st = (RepState*)dalloc(r, sizeof(RepState), &ok);
-    D.st = r.st;
-    D.thermo = (ThermoRec*)dalloc(r, (size_t)std::max(1, S.thermo_cap) * sizeof(ThermoRec), &ok);
-
     // topology by global id
     std::vector<int> boff(na + 1, 0), aoff(na + 1, 0);
     for (int b = 0; b < d->nbonds; b++) { boff[d->bonds[2 * b] + 1]++; boff[d->bonds[2 * b + 1] + 1]++; }
@@ -314,21 +304,72 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
         aoff.swap(aoff2);
         alst.swap(alst2);
     }
-    int* t_boff = (int*)dalloc(r, (na + 1) * 4, &ok);
-    int* t_blst = (int*)dalloc(r, blst.size() * 4, &ok);
-    int* t_aoff = (int*)dalloc(r, (na + 1) * 4, &ok);
-    int4* t_alst = (int4*)dalloc(r, alst.size() * 16, &ok);
-    float2* lj = (float2*)dalloc(r, (size_t)d->ntypes * d->ntypes * 8, &ok);
-    float4* bp = (float4*)dalloc(r, (size_t)std::max(1, d->nbond_types) * 16, &ok);
-    float2* ap = (float2*)dalloc(r, (size_t)std::max(1, d->nangle_types) * 8, &ok);
-    float* ms = (float*)dalloc(r, kMaxTypes * 4, &ok);
+    // one device block per replica, carved into its arrays (256-byte aligned):
+    // a first pass sizes the block, the second places the arrays
+    int* t_boff = nullptr; int* t_blst = nullptr; int* t_aoff = nullptr; int4* t_alst = nullptr;
+    float2* lj = nullptr; float4* bp = nullptr; float2* ap = nullptr; float* ms = nullptr;
+    char* base = nullptr;
+    size_t used = 0;
+    auto carve = [&](size_t bytes) -> void* {
+        void* p = base ? base + used : nullptr;
+        used += (std::max<size_t>(bytes, 16) + 255) & ~(size_t)255;
+        return p;
+    };
+    auto layout = [&]() {
+    D.pos = (float4*)carve(ns * 16);
+    D.vel = (float4*)carve(ns * 16);
+    D.force = (float4*)carve(ns * 16);
+    D.pos_ref = (float4*)carve(ns * 16);
+    D.ipos = (int4*)carve(ns * 16);
+    D.img = (int4*)carve(ns * 16);
+    D.tmp_pos = (float4*)carve(ns * 16);
+    D.tmp_vel = (float4*)carve(ns * 16);
+    D.tmp_img = (int4*)carve(ns * 16);
+    D.cell_of = (int*)carve(ns * 4);
+    D.rank = (int*)carve(ns * 4);
+    D.order = (int*)carve(ns * 4);
+    D.order2 = (int*)carve(ns * 4);
+    D.slot_of = (int*)carve(na * 4);
+    D.cell_cnt = (int*)carve((size_t)S.ncell_cap * 4);
+    D.cell_start = (int*)carve(((size_t)S.ncell_cap + 1) * 4);
+    D.nnbr = (int*)carve(ns * 4);
+    D.nbr = (int*)carve(nt * S.kcap * 64 * 4);
+    D.bhdr = (int*)carve(ns * 4);
+    D.bnd = (int*)carve(nt * S.bcap * 64 * 4);
+    D.ang = (int2*)carve(nt * S.acap * 64 * 8);
+    D.wpart = (double*)carve(nt * kNumPart * 8);
+    D.tdisp = (float*)carve(nt * 4);
+    D.nchunks = (int)((nt + kChunkTiles - 1) / kChunkTiles);
+    D.cpart = (double*)carve((size_t)D.nchunks * kNumPart * 8);
+    r.st = (RepState*)carve(sizeof(RepState));
+    D.st = r.st;
+    D.thermo = (ThermoRec*)carve((size_t)std::max(1, S.thermo_cap) * sizeof(ThermoRec));
+
+    t_boff = (int*)carve((na + 1) * 4);
+    t_blst = (int*)carve(blst.size() * 4);
+    t_aoff = (int*)carve((na + 1) * 4);
+    t_alst = (int4*)carve(alst.size() * 16);
+    lj = (float2*)carve((size_t)d->ntypes * d->ntypes * 8);
+    bp = (float4*)carve((size_t)std::max(1, d->nbond_types) * 16);
+    ap = (float2*)carve((size_t)std::max(1, d->nangle_types) * 8);
+    ms = (float*)carve(kMaxTypes * 4);
     D.t_boff = t_boff; D.t_blst = t_blst; D.t_aoff = t_aoff; D.t_alst = t_alst;
     D.lj = lj; D.bparm = bp; D.aparm = ap; D.mass = ms;
-    if (!ok) {
-        for (void* p : r.allocs) (void)hipFree(p);
+    };
+    layout();
+    const size_t block_bytes = used;
+    size_t got = 0;
+    base = (char*)acquire_block(block_bytes, &got);
+    if (!base) {
         delete R;
         return fail(FMD_ERR_NOMEM, "device allocation failed");
     }
+    r.block = base;
+    r.block_bytes = got;
+    r.bytes = got;
+    used = 0;
+    layout();
+
 
     // host staging of the per-slot state
     std::vector<float4> hpos(ns), hvel(ns);
@@ -353,7 +394,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
         for (int t = 0; t < d->ntypes; t++) mass[t] = d->masses[t];
     for (int t = 0; t < d->ntypes; t++)
         if (!(mass[t] > 0.0f)) {
-            for (void* p : r.allocs) (void)hipFree(p);
+            release_block(r.block, r.block_bytes);
             delete R;
             return fail(FMD_ERR_ARG, "masses must be > 0");
         }
@@ -427,7 +468,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     up(d_descs_ + id, &D, sizeof(RepDesc));
     if (!rc) rc = hip_check(hipStreamSynchronize(s), "sync(create)");
     if (rc) {
-        for (void* p : r.allocs) (void)hipFree(p);
+        release_block(r.block, r.block_bytes);
         delete R;
         return rc;
     }
@@ -467,7 +508,7 @@ int32_t Engine::destroy(int32_t sys) {
     if (!r) return fail(FMD_ERR_ARG, "no such system");
     (void)hipStreamSynchronize(stream_);
     drop_batches_with(sys);
-    for (void* p : r->allocs) (void)hipFree(p);
+    release_block(r->block, r->block_bytes);
     in_use_ -= r->bytes;
     reps_[sys] = nullptr;
     delete r;
@@ -527,25 +568,42 @@ Batch* Engine::batch_for(const std::vector<int>& ids_in) {
         fail(FMD_ERR_ARG, "more than " + std::to_string(kMaxReplicas) + " systems in one batch");
         return nullptr;
     }
-    bool ok = hipMalloc(&b->d_tiles, std::max<size_t>(1, tiles.size()) * sizeof(Tile)) == hipSuccess &&
-              hipMalloc(&b->d_reps, std::max<size_t>(1, ord.size()) * sizeof(int)) == hipSuccess &&
-              hipMalloc(&b->d_rep_t0, std::max<size_t>(1, ord.size()) * sizeof(int)) == hipSuccess &&
-              hipMalloc(&b->d_rlist, std::max<size_t>(1, tiles.size()) * sizeof(int)) == hipSuccess &&
-              hipMalloc(&b->d_nrt, sizeof(int)) == hipSuccess &&
-              hipMalloc(&b->d_chunks, std::max<size_t>(1, chunks.size()) * sizeof(Tile)) == hipSuccess;
-    if (ok)
-        ok = hipMemcpy(b->d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice) == hipSuccess &&
-             hipMemcpy(b->d_reps, ord.data(), ord.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
-             hipMemcpy(b->d_rep_t0, t0s.data(), t0s.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
-             hipMemset(b->d_nrt, 0, sizeof(int)) == hipSuccess &&
-             hipMemcpy(b->d_chunks, chunks.data(), chunks.size() * sizeof(Tile), hipMemcpyHostToDevice) == hipSuccess;
-    if (!ok) {
-        free_batch(b);
+    // one device block (from the replica block cache), filled by one copy
+    auto al = [](size_t n) { return (std::max<size_t>(n, 16) + 255) & ~(size_t)255; };
+    const size_t o_tiles = 0, o_reps = o_tiles + al(tiles.size() * sizeof(Tile)),
+                 o_t0 = o_reps + al(ord.size() * 4), o_chunks = o_t0 + al(t0s.size() * 4),
+                 o_nrt = o_chunks + al(chunks.size() * sizeof(Tile)), o_rlist = o_nrt + al(4),
+                 o_status = o_rlist + al(tiles.size() * 4), total = o_status + al(ord.size() * 16);
+    std::vector<char> host(o_rlist, 0);
+    std::memcpy(host.data() + o_tiles, tiles.data(), tiles.size() * sizeof(Tile));
+    std::memcpy(host.data() + o_reps, ord.data(), ord.size() * 4);
+    std::memcpy(host.data() + o_t0, t0s.data(), t0s.size() * 4);
+    std::memcpy(host.data() + o_chunks, chunks.data(), chunks.size() * sizeof(Tile));
+    char* blk = (char*)acquire_block(total, &b->block_bytes);
+    if (!blk || hipMemcpy(blk, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        if (blk) release_block(blk, b->block_bytes);
+        delete b;
         fail(FMD_ERR_NOMEM, "batch allocation failed");
         return nullptr;
     }
+    b->block = blk;
+    b->d_tiles = (Tile*)(blk + o_tiles);
+    b->d_reps = (int*)(blk + o_reps);
+    b->d_rep_t0 = (int*)(blk + o_t0);
+    b->d_chunks = (Tile*)(blk + o_chunks);
+    b->d_nrt = (int*)(blk + o_nrt);
+    b->d_rlist = (int*)(blk + o_rlist);
+    b->d_status = (int4*)(blk + o_status);
+    b->ord = ord;
     batches_[ids] = b;
     return b;
+}
+
+// Only after the stream has drained: the block returns to the cache.
+void Engine::free_batch(Batch* b) {
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    release_block(b->block, b->block_bytes);
+    delete b;
 }
 
 Launch Engine::launch_of(const Batch& b) const {
@@ -626,21 +684,27 @@ static int32_t err_code(int e) {
 }
 
 int32_t Engine::check_errors(const std::vector<int>& ids, std::vector<int>* failed) {
+    // one status kernel and one copy for the whole batch
+    Batch* b = batch_for(ids);
+    if (!b) return FMD_ERR_ARG;
+    std::vector<int4> stat(b->nr);
+    launch_status(launch_of(*b), b->d_status);
+    int32_t rc = hip_check(hipMemcpyAsync(stat.data(), b->d_status, stat.size() * sizeof(int4),
+                                          hipMemcpyDeviceToHost, stream_), "copy status");
+    if (!rc) rc = sync();
+    if (rc) return rc;
     int32_t first = FMD_OK;
-    for (int id : ids) {
+    for (int k = 0; k < b->nr; k++) {
+        const int id = b->ord[k];
         Replica* r = rep(id);
-        if (!r) continue;
-        RepState s;
-        int32_t rc = read_state(*r, s);
-        if (rc) return rc;
-        r->steps_done = s.step;
-        if (s.err) {
-            int32_t c = err_code(s.err);
+        const int4 st = stat[k];
+        r->steps_done = (int64_t)(((uint64_t)(uint32_t)st.w << 32) | (uint32_t)st.z);
+        if (st.x) {
             if (failed) failed->push_back(id);
             if (!first) {
-                first = c;
-                err_ = "system " + std::to_string(id) + " stopped at step " + std::to_string(s.step) +
-                       " (error bits " + std::to_string(s.err) + ")";
+                first = err_code(st.x);
+                err_ = "system " + std::to_string(id) + " stopped at step " + std::to_string(r->steps_done) +
+                       " (error bits " + std::to_string(st.x) + ")";
             }
         }
     }

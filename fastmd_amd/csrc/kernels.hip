@@ -181,7 +181,9 @@ __device__ void start_step(const RepDesc& d, RepState* st, int64_t s) {
     float T = (float)st->T_start + ((float)st->T_stop - (float)st->T_start) * frac;
     float P = (float)st->P_start + ((float)st->P_stop - (float)st->P_start) * frac;
     const double hdt = 0.5 * (double)d.dt, dt = (double)d.dt;
-    st->want_obs = (d.ensemble == 2) || (st->thermo_freq > 0 && (s % st->thermo_freq) == 0);
+    // observables on NPT steps (the barostat needs the virial), thermo steps and
+    // the last step of a run (its result is read back)
+    st->want_obs = (d.ensemble == 2) || (st->thermo_freq > 0 && (s % st->thermo_freq) == 0) || (s == st->stop);
     if (d.ensemble == 0) {
         st->kT = T;
         st->vscale = 1.0f;
@@ -336,6 +338,16 @@ __global__ void __launch_bounds__(1024) k_decide(const RepDesc* __restrict__ des
         st->since_rebuild = since;
         st->maxdisp_bits = __float_as_int(md);
     }
+}
+
+// Status of each listed replica for one host read per batch.
+__global__ void k_status(const RepDesc* __restrict__ descs, const int* __restrict__ reps, int nr,
+                         int4* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nr) return;
+    const RepState* st = descs[reps[k]].st;
+    const uint64_t s = (uint64_t)st->step;
+    out[k] = make_int4(st->err, st->active, (int)(uint32_t)s, (int)(uint32_t)(s >> 32));
 }
 
 // Host-requested state changes.
@@ -1303,6 +1315,10 @@ void launch_pre(const Launch& l) {
 void launch_decide(const Launch& l) {
     if (l.nr <= 0) return;
     k_decide<<<l.nr, 1024, 0, l.stream>>>(l.descs, l.reps, l.nr);
+}
+void launch_status(const Launch& l, int4* out) {
+    if (l.nr <= 0) return;
+    k_status<<<cdiv(l.nr, 256), 256, 0, l.stream>>>(l.descs, l.reps, l.nr, out);
 }
 void launch_control(const Launch& l, int op, int64_t arg) {
     if (l.nr <= 0) return;

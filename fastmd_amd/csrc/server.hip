@@ -8,6 +8,8 @@
 // the very next launch.  Each replica stops on the device at its own step
 // count, so jobs of different lengths share launches.
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -77,6 +79,10 @@ class Server {
         }
         cv_.notify_all();
         if (worker_.joinable()) worker_.join();
+        if (const char* v = std::getenv("FMD_SERVER_PROFILE"))
+            if (v[0] == '1')
+                std::fprintf(stderr, "[server] create %.3f s, enqueue %.3f s, device wait %.3f s, finish %.3f s\n",
+                             prof_[0], prof_[1], prof_[2], prof_[3]);
     }
     bool ok() const { return eng_.ok(); }
     double now() const {
@@ -170,6 +176,7 @@ class Server {
                 blocks_.clear();
             }
             // create admitted jobs on the device (outside the lock)
+            double tc0 = now();
             for (Job* j : admitted) {
                 int32_t sys = -1;
                 int32_t rc = eng_.create(&j->desc->d, &sys);
@@ -186,12 +193,23 @@ class Server {
                 j->t_admit = now();
                 running.push_back(j);
             }
+            prof_[0] += now() - tc0;
             if (running.empty()) continue;
             std::vector<int> ids;
             for (Job* j : running) ids.push_back(j->sys);
             int64_t k = graph_steps_;
-            int32_t rc = eng_.launch_steps(ids, k, true);
+            // a hipGraph pays off once the resident set is stable: right after an
+            // admission or a completion the set is new and likely to change
+            // again, so its steps are issued directly instead of captured
+            const bool stable = (ids == prev_ids_);
+            prev_ids_ = ids;
+            const double ts0 = now();
+            int32_t rc = eng_.launch_steps(ids, k, stable);
+            const double ts1 = now();
             if (!rc) rc = eng_.sync();
+            const double ts2 = now();
+            prof_[1] += ts1 - ts0;
+            prof_[2] += ts2 - ts1;
             double bs = 0;
             for (Job* j : running) bs += (double)j->desc->d.natoms * (double)std::min<int64_t>(k, j->remaining);
             std::vector<int> failed;
@@ -226,6 +244,7 @@ class Server {
             std::vector<int> rids;
             for (Job* j : running) rids.push_back(j->sys);
             eng_.trim_batches(rids);
+            prof_[3] += now() - ts2;
         }
     }
 
@@ -233,6 +252,8 @@ class Server {
     Engine eng_;
     int graph_steps_;
     std::vector<BlockSet> blocks_;  // block shapes to apply before the next launch
+    std::vector<int> prev_ids_;     // resident set of the previous launch
+    double prof_[4] = {0, 0, 0, 0}; // seconds in create, enqueue, device wait, finish
     AdmissionQueue aq_;
     std::chrono::steady_clock::time_point t0_;
     std::mutex m_;
