@@ -53,7 +53,7 @@ def cpu_baseline(engine, sids, specs, target_s=15.0):
     while t_used < target_s and k < len(sids):
         p, v, im = engine.state(sids[k])
         d = melt_desc(specs[k])
-        nst = 50
+        nst = 1000 if (len(sids) == 1 and len(p) <= 100000) else 50  # a lone small melt: its full run
         t0 = time.perf_counter()
         orc.cpu_md_langevin(p, v, im, np.ones(16, np.float32), float(d.c.L), nst, d.bonds, d.bond_params,
                             d.angles, d.angle_params, T=specs[k]["T"], Tdamp=1.0, seed=specs[k]["seed"],
@@ -147,9 +147,10 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--replicas", type=int, default=256)
-    ap.add_argument("--workload", choices=("replicas", "single"), default="replicas",
+    ap.add_argument("--workload", choices=("replicas", "single", "small"), default="replicas",
                     help="replicas: the uneven-replica batch (default); single: one 10,000 x 100-bead "
-                         "melt (1,000,000 beads), Langevin T=1")
+                         "melt (1,000,000 beads); small: one 300 x 100-bead melt (30,000 beads); "
+                         "both Langevin T=1")
     ap.add_argument("--plain-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
@@ -172,9 +173,10 @@ def main():
     from fastmd_amd import workloads
 
     eng = fmd.Engine(0)
-    single = args.workload == "single"
+    single = args.workload in ("single", "small")
     if single:
-        specs = [{"nchains": 10000, "chain_len": 100, "T": 1.0, "seed": 42, "gen_seed": 2024}]
+        nch = 10000 if args.workload == "single" else 300
+        specs = [{"nchains": nch, "chain_len": 100, "T": 1.0, "seed": 42, "gen_seed": 2024}]
         args.serve_window = 0.0
     else:
         specs = workloads.uneven_replica_specs(args.replicas)
@@ -238,8 +240,8 @@ def main():
         "dtype": "f32 (FP64 reductions)",
         "data": "synthetic: generated KG melts relaxed by a capped soft push-off on the GPU",
         "config": {
-            "workload": (f"one Kremer-Grest melt of 10,000 chains x 100 beads ({total} beads), rho 0.85, "
-                         f"Langevin T=1, dt 0.001, hipGraph step; plain = one launch per kernel per step")
+            "workload": (f"one Kremer-Grest melt of {specs[0]['nchains']} chains x 100 beads ({total} beads), "
+                         f"rho 0.85, Langevin T=1, dt 0.001, hipGraph step; plain = one launch per kernel per step")
             if single else
             (f"{len(sids)} uneven Kremer-Grest melt replicas, chain length log-uniform [20,200], "
              f"beads log-uniform [2000,64000], rho 0.85, Langevin T U[0.8,1.2], dt 0.001, "
