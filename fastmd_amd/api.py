@@ -102,6 +102,7 @@ SIGNATURES = [
     ("fmd_get_state", C.c_int32, [C.c_void_p, C.c_int32, _fp, _fp, _ip]),
     ("fmd_get_box", C.c_int32, [C.c_void_p, C.c_int32, _dp]),
     ("fmd_get_state_box", C.c_int32, [C.c_void_p, C.c_int32, _dp]),
+    ("fmd_get_rg", C.c_int32, [C.c_void_p, C.c_int32, _dp, _dp, C.c_int32, _ip]),
     ("fmd_get_thermo", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(Thermo)]),
     ("fmd_drain_thermo", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(Thermo), C.c_int32, _ip,
                                      C.POINTER(C.c_int64)]),
@@ -384,6 +385,16 @@ class Engine:
         L = C.c_double()
         self._chk(lib().fmd_get_box(self.h, sid, C.byref(L)))
         return L.value
+
+    def rg(self, sid, per_chain=False):
+        """Chain-averaged radius of gyration (and the per-chain values)."""
+        rg, n = C.c_double(), C.c_int32()
+        self._chk(lib().fmd_get_rg(self.h, sid, C.byref(rg), None, 0, C.byref(n)))
+        if not per_chain:
+            return rg.value
+        out = np.zeros(n.value, np.float64)
+        self._chk(lib().fmd_get_rg(self.h, sid, C.byref(rg), _ptr(out, C.c_double), n.value, C.byref(n)))
+        return rg.value, out
 
     def state_box(self, sid):
         """Box side of the positions state() returns (differs from box() under NPT between steps)."""

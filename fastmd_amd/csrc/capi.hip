@@ -145,6 +145,9 @@ int32_t fmd_get_state(fmd_engine* e, int32_t sys, float* pos4, float* vel3, int3
     return e ? e->e->get_state(sys, pos4, vel3, image3) : FMD_ERR_ARG;
 }
 int32_t fmd_get_box(fmd_engine* e, int32_t sys, double* L) { return e ? e->e->get_box(sys, L) : FMD_ERR_ARG; }
+int32_t fmd_get_rg(fmd_engine* e, int32_t sys, double* rg, double* per_chain, int32_t cap, int32_t* nchains) {
+    return e ? e->e->get_rg(sys, rg, per_chain, cap, nchains) : FMD_ERR_ARG;
+}
 int32_t fmd_get_state_box(fmd_engine* e, int32_t sys, double* L) {
     return e ? e->e->get_box(sys, L, true) : FMD_ERR_ARG;
 }

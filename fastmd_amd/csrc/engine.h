@@ -88,6 +88,7 @@ class Engine {
     int32_t get_state(int32_t sys, float* pos4, float* vel3, int32_t* img3);
     int32_t get_box(int32_t sys, double* L, bool of_positions = false);
     int32_t get_thermo(int32_t sys, fmd_thermo* out);
+    int32_t get_rg(int32_t sys, double* rg, double* per_chain, int32_t cap, int32_t* nchains);
     int32_t drain_thermo(int32_t sys, fmd_thermo* out, int32_t cap, int32_t* n, int64_t* lost);
 
     int32_t autotune(const int32_t* ids, int32_t n, int reps);

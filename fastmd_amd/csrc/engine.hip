@@ -304,9 +304,19 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
         aoff.swap(aoff2);
         alst.swap(alst2);
     }
+    // chains for Rg: runs of equal molecule id in global-id order (src/main.cu:270-286)
+    std::vector<int> choff;
+    if (d->mol_id) {
+        for (size_t i = 0; i < na; i++)
+            if (i == 0 || d->mol_id[i] != d->mol_id[i - 1]) choff.push_back((int)i);
+        choff.push_back((int)na);
+    } else {
+        choff.push_back(0);
+    }
     // one device block per replica, carved into its arrays (256-byte aligned):
     // a first pass sizes the block, the second places the arrays
     int* t_boff = nullptr; int* t_blst = nullptr; int* t_aoff = nullptr; int4* t_alst = nullptr;
+    int* t_choff = nullptr;
     float2* lj = nullptr; float4* bp = nullptr; float2* ap = nullptr; float* ms = nullptr;
     char* base = nullptr;
     size_t used = 0;
@@ -353,7 +363,10 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     bp = (float4*)carve((size_t)std::max(1, d->nbond_types) * 16);
     ap = (float2*)carve((size_t)std::max(1, d->nangle_types) * 8);
     ms = (float*)carve(kMaxTypes * 4);
+    t_choff = (int*)carve(choff.size() * 4);
     D.t_boff = t_boff; D.t_blst = t_blst; D.t_aoff = t_aoff; D.t_alst = t_alst;
+    D.t_choff = t_choff;
+    D.nchains = (int)choff.size() - 1;
     D.lj = lj; D.bparm = bp; D.aparm = ap; D.mass = ms;
     };
     layout();
@@ -464,6 +477,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     if (d->nbond_types > 0) up(bp, d->bond_params, (size_t)d->nbond_types * 16);
     if (d->nangle_types > 0) up(ap, d->angle_params, (size_t)d->nangle_types * 8);
     up(ms, mass.data(), kMaxTypes * 4);
+    up(t_choff, choff.data(), choff.size() * 4);
     up(r.st, &S0, sizeof(S0));
     up(d_descs_ + id, &D, sizeof(RepDesc));
     if (!rc) rc = hip_check(hipStreamSynchronize(s), "sync(create)");
@@ -939,6 +953,32 @@ int32_t Engine::get_box(int32_t sys, double* L, bool of_positions) {
     int32_t rc = read_state(*r, s);
     if (!rc) *L = of_positions ? s.Lp : s.L;
     return rc;
+}
+
+// Ensemble average of the chains' Rg (src/analysis/rg.cu:106-129), summed on the
+// host in chain order.
+int32_t Engine::get_rg(int32_t sys, double* rg, double* per_chain, int32_t cap, int32_t* nchains) {
+    Replica* r = rep(sys);
+    if (!r || !rg) return fail(FMD_ERR_ARG, "bad arguments");
+    const int nc = r->desc.nchains;
+    if (nchains) *nchains = nc;
+    *rg = 0.0;
+    if (nc <= 0) return FMD_OK;
+    double* dout = nullptr;
+    int32_t rc = hip_check(hipMalloc(&dout, (size_t)nc * 8), "hipMalloc(rg)");
+    if (rc) return rc;
+    launch_rg(d_descs_ + sys, nc, dout, stream_);
+    std::vector<double> h(nc);
+    rc = hip_check(hipMemcpyAsync(h.data(), dout, (size_t)nc * 8, hipMemcpyDeviceToHost, stream_), "copy rg");
+    if (!rc) rc = sync();
+    (void)hipFree(dout);
+    if (rc) return rc;
+    double s = 0.0;
+    for (int c = 0; c < nc; c++) s += h[c];
+    *rg = s / nc;
+    if (per_chain)
+        for (int c = 0; c < nc && c < cap; c++) per_chain[c] = h[c];
+    return FMD_OK;
 }
 
 static void thermo_of(const RepState& s, int natoms, fmd_thermo* o) {

@@ -83,6 +83,13 @@ int main(int argc, char** argv) {
         std::fprintf(tf, "# step  KE  PE  T  Pxx  Pyy  Pzz  Pxy  Pxz  Pyz\n");
         std::fflush(tf);
     }
+    // Rg file (src/analysis/rg.cu:71-96, :106-129): "# step  Rg", "%d  %.4f"
+    FILE* rgf = nullptr;
+    if (ex.rg_on && ex.rg_freq > 0) {
+        rgf = std::fopen(ex.rg_file, "w");
+        if (!rgf) return die("rg", ex.rg_file);
+        std::fprintf(rgf, "# step  Rg\n");
+    }
     std::vector<fmd_thermo> rec(4096);
     std::vector<float> pos(4 * (size_t)n), vel(3 * (size_t)n);
     std::vector<int32_t> img(3 * (size_t)n);
@@ -127,10 +134,20 @@ int main(int argc, char** argv) {
             int64_t to_restart = ex.restart_freq - (done % ex.restart_freq);
             if (chunk > to_restart) chunk = to_restart;
         }
+        if (rgf) {
+            int64_t to_rg = ex.rg_freq - (done % ex.rg_freq);
+            if (chunk > to_rg) chunk = to_rg;
+        }
         int32_t rc = fmd_step(eng, &sys, 1, chunk);
         if (drain()) return 1;
         if (rc) return die("step", fmd_last_error(eng));
         done += chunk;
+        if (rgf && done % ex.rg_freq == 0) {
+            double rg = 0.0;
+            if (fmd_get_rg(eng, sys, &rg, nullptr, 0, nullptr)) return die("rg", fmd_last_error(eng));
+            std::fprintf(rgf, "%ld  %.4f\n", (long)done, rg);
+            std::fflush(rgf);
+        }
         if (ex.restart_freq > 0 && done % ex.restart_freq == 0 && done != nsteps)
             if (write_restart(fmd::restart_filename(ex.restart_file, done), done)) return 1;
         std::printf("Step %ld/%ld\n", (long)done, (long)nsteps);
@@ -142,6 +159,7 @@ int main(int argc, char** argv) {
     std::printf("Wall time: %.3f s  |  Steps/s: %.1f  |  ns/day: %.3f\n", el,
                 el > 0 ? nsteps / el : 0.0, el > 0 ? (nsteps * (double)d.dt) / (el / 86400.0) : 0.0);
     if (tf) std::fclose(tf);
+    if (rgf) std::fclose(rgf);
     fmd_engine_destroy(eng);
     fmd_topo_free(topo);
     std::printf("Simulation complete.\n");

@@ -133,6 +133,8 @@ struct RepDesc {
     const int* t_blst;     // partner gid | type << 26
     const int* t_aoff;     // [natoms + 1]
     const int4* t_alst;    // angle (i, j, k, type) incident on the gid
+    const int* t_choff;    // [nchains + 1] chains: runs of equal molecule id in global-id order
+    int nchains;
     // parameters
     const float2* lj;      // [ntypes^2] (eps, sig)
     const float4* bparm;   // per bond type (k, R0, eps, sig)

@@ -33,6 +33,8 @@ void launch_rebuild(const Launch& l, int always, int build);
 void launch_force(const Launch& l, int t0, int nt, int block, int terms, int kick, int always);
 void launch_boundary(const Launch& l, int mode);
 void launch_control(const Launch& l, int op, int64_t arg);
+// Radius of gyration of each of a replica's chains.
+void launch_rg(const RepDesc* d, int nchains, double* out, hipStream_t s);
 // Per replica of the list: (err, active, step lo, step hi).
 void launch_status(const Launch& l, int4* out);
 // Reference cell ids / Morton keys by global id for one replica.

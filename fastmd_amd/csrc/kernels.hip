@@ -340,6 +340,36 @@ __global__ void __launch_bounds__(1024) k_decide(const RepDesc* __restrict__ des
     }
 }
 
+// Radius of gyration of each chain (src/analysis/rg.cu:3-69): unwrapped
+// positions x + image * L over the chain's beads (global-id order, through
+// slot_of), centre of mass, then sqrt of the mean squared distance from it.
+// One wave per chain, FP64 sums by wave shuffles.
+__global__ void __launch_bounds__(64) k_rg(const RepDesc* __restrict__ d, double* __restrict__ out) {
+    const int c = blockIdx.x;
+    if (c >= d->nchains) return;
+    const int g0 = d->t_choff[c], g1 = d->t_choff[c + 1], len = g1 - g0;
+    const double L = (double)d->st->Lp;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int g = g0 + lane_id(); g < g1; g += 64) {
+        const int q = d->slot_of[g];
+        const float4 p = d->pos[q];
+        const int4 im = d->img[q];
+        sx += (double)p.x + im.x * L; sy += (double)p.y + im.y * L; sz += (double)p.z + im.z * L;
+    }
+    const double cx = wave_sum(sx) / len, cy = wave_sum(sy) / len, cz = wave_sum(sz) / len;
+    double r2 = 0.0;
+    for (int g = g0 + lane_id(); g < g1; g += 64) {
+        const int q = d->slot_of[g];
+        const float4 p = d->pos[q];
+        const int4 im = d->img[q];
+        const double dx = (double)p.x + im.x * L - cx, dy = (double)p.y + im.y * L - cy,
+                     dz = (double)p.z + im.z * L - cz;
+        r2 += dx * dx + dy * dy + dz * dz;
+    }
+    r2 = wave_sum(r2);
+    if (lane_id() == 0) out[c] = sqrt(r2 / len);
+}
+
 // Status of each listed replica for one host read per batch.
 __global__ void k_status(const RepDesc* __restrict__ descs, const int* __restrict__ reps, int nr,
                          int4* __restrict__ out) {
@@ -1315,6 +1345,10 @@ void launch_pre(const Launch& l) {
 void launch_decide(const Launch& l) {
     if (l.nr <= 0) return;
     k_decide<<<l.nr, 1024, 0, l.stream>>>(l.descs, l.reps, l.nr);
+}
+void launch_rg(const RepDesc* d, int nchains, double* out, hipStream_t s) {
+    if (nchains <= 0) return;
+    k_rg<<<nchains, 64, 0, s>>>(d, out);
 }
 void launch_status(const Launch& l, int4* out) {
     if (l.nr <= 0) return;

@@ -196,6 +196,12 @@ int32_t fmd_get_box(fmd_engine* e, int32_t sys, double* L);
  * parity entry points (fmd_compute_forces, fmd_build_neighbors) use; equal to
  * fmd_get_box except under NPT between steps.  No reference counterpart. */
 int32_t fmd_get_state_box(fmd_engine* e, int32_t sys, double* L);
+/* Radius of gyration, averaged over chains (runs of equal molecule id in
+ * global-id order), from unwrapped positions; replaces compute_rg
+ * (src/analysis/rg.cuh:28-31).  per_chain (if non-NULL) receives up to cap
+ * chain values; nchains (if non-NULL) the chain count. */
+int32_t fmd_get_rg(fmd_engine* e, int32_t sys, double* rg, double* per_chain, int32_t cap,
+                   int32_t* nchains);
 /* Latest reductions (thermo of the last completed step). */
 int32_t fmd_get_thermo(fmd_engine* e, int32_t sys, fmd_thermo* out);
 /* Drains the thermo records written every thermo_freq steps since the last

@@ -648,4 +648,33 @@ template <typename Real> Real sinch(Real x) {
     return std::sinh(x) / x;
 }
 
+
+// ---------------------------------------------------------------------------
+// Radius of gyration (src/analysis/rg.cu:3-69, :106-129): chains are runs of
+// equal molecule id in atom order; each chain's Rg is sqrt(mean |x - com|^2)
+// over unwrapped positions x + image * L; the result is the chain average.
+// FP64 throughout.
+// ---------------------------------------------------------------------------
+inline double rg_ensemble(int n, const float* pos4, const int* image, const int* mol, float L) {
+    double sum = 0.0;
+    int nch = 0;
+    for (int s = 0; s < n;) {
+        int e = s + 1;
+        while (e < n && mol[e] == mol[s]) e++;
+        double c[3] = {0.0, 0.0, 0.0};
+        for (int i = s; i < e; i++)
+            for (int k = 0; k < 3; k++) c[k] += (double)pos4[4 * i + k] + (double)image[3 * i + k] * (double)L;
+        for (int k = 0; k < 3; k++) c[k] /= (double)(e - s);
+        double r2 = 0.0;
+        for (int i = s; i < e; i++)
+            for (int k = 0; k < 3; k++) {
+                const double x = (double)pos4[4 * i + k] + (double)image[3 * i + k] * (double)L - c[k];
+                r2 += x * x;
+            }
+        sum += std::sqrt(r2 / (double)(e - s));
+        nch++;
+        s = e;
+    }
+    return nch ? sum / nch : 0.0;
+}
 }  // namespace orc

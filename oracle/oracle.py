@@ -229,6 +229,17 @@ def nh_chain(prec, M, Q1, Qr, xi, vxi, KE, kT, natoms, half_dt):
     return s, xi, vxi
 
 
+def rg(pos4, image, mol, L):
+    """Chain-averaged radius of gyration, FP64 (chains = runs of equal mol id)."""
+    p = np.ascontiguousarray(pos4, np.float32).reshape(-1, 4)
+    im = np.ascontiguousarray(image, np.int32).reshape(-1, 3)
+    m = np.ascontiguousarray(mol, np.int32).reshape(-1)
+    f = lib().orc_rg
+    f.restype = C.c_double
+    f.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float]
+    return f(len(p), p.ctypes.data, im.ctypes.data, m.ctypes.data, float(L))
+
+
 def md_run(sysd, nsteps):
     """Runs the oracle MD driver on a dict describing the system (see tests)."""
     n = sysd["pos4"].shape[0]

@@ -306,4 +306,8 @@ void orc_thermo(int n, const float* pos4, const float* vel3, const float* masses
     for (int c = 0; c < 6; c++) out[3 + c] = (k6[c] + vir[c]) / V;
 }
 
+double orc_rg(int n, const float* pos4, const int32_t* image, const int32_t* mol, float L) {
+    return rg_ensemble(n, pos4, image, mol, L);
+}
+
 }  // extern "C"

@@ -1,0 +1,66 @@
+"""End-to-end drop-in: the fastmd binary run on a run.conf in the reference's
+key format (tests/integration/bonded_ring/fastmd_template.conf), writing the
+reference's thermo, Rg and restart files and stdout lines
+(src/main.cu:403-811; benchmarks/run.py:44-57 parses the stdout)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, relax_kg
+
+pytestmark = pytest.mark.gpu
+
+BIN = os.path.join(ROOT, "fastmd_amd", "bin", "fastmd")
+
+
+def test_cli_run_conf_outputs(fmd, engine, tmp_path):
+    d = relax_kg(fmd, engine, 40, 50, seed=41)
+    sid = engine.add(d)
+    p4, v, im = engine.state(sid)
+    L = engine.box(sid)
+    engine.remove(sid)
+    t = fmd.Topo.kg_melt(40, 50, 0.85, 41)
+    data = tmp_path / "melt.data"
+    t.write_data(str(data), pos4=p4, vel=v, image=im, L=L, step=0)
+    conf = f"""natoms 2000
+ntypes 1
+rc 2.5
+skin 0.3
+dt 0.001
+nvt_langevin 1.0 1.0 1.0 42
+nsteps 400
+dump_freq 0
+thermo 1 50 thermo.dat
+rg 1 100 rg.dat
+lj 0 0 1.0 1.0
+bond_type 0 30.0 1.5 1.0 1.0
+angle_type 0 5.0 180.0
+lammps_data_file {data}
+restart 200 restart.data
+"""
+    (tmp_path / "run.conf").write_text(conf)
+    print(f"[cli] running {BIN} run.conf in {tmp_path}", flush=True)
+    r = subprocess.run([BIN, "run.conf"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    out = r.stdout
+    # stdout contract parsed by benchmarks/run.py
+    assert re.search(r"Wall time: [0-9.]+ s\s+\|\s+Steps/s: [0-9.]+\s+\|\s+ns/day: [0-9.]+", out)
+    assert len(re.findall(r"^Step\s+\d+\s+T=", out, re.M)) == 8
+    # thermo file: header and 10 columns, every 50 steps
+    rows = [l for l in (tmp_path / "thermo.dat").read_text().splitlines()]
+    assert rows[0] == "# step  KE  PE  T  Pxx  Pyy  Pzz  Pxy  Pxz  Pyz"
+    vals = np.array([list(map(float, l.split())) for l in rows[1:]])
+    assert vals.shape == (8, 10) and list(vals[:, 0]) == [50 * k for k in range(1, 9)]
+    assert np.all((vals[:, 3] > 0.7) & (vals[:, 3] < 1.3))
+    # Rg file every 100 steps
+    rg = (tmp_path / "rg.dat").read_text().splitlines()
+    assert rg[0] == "# step  Rg" and len(rg) == 5
+    assert [int(l.split()[0]) for l in rg[1:]] == [100, 200, 300, 400]
+    assert all(2.0 < float(l.split()[1]) < 6.0 for l in rg[1:])
+    # restart files read back with the same topology
+    for name in ("restart_200.data", "restart_final.data"):
+        t2 = fmd.Topo.from_data(str(tmp_path / name))
+        assert t2.natoms == 2000 and len(t2.arrays()["bonds"]) == 40 * 49

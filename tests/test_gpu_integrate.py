@@ -136,3 +136,31 @@ def test_npt_pressure_control(fmd, engine, melt):
     T = np.array([r["temp"] for r in late])
     assert abs(P.mean() - 1.0) < 0.25
     assert abs(T.mean() - 1.0) < 0.05
+
+
+def test_rg_reference_cases_and_melt(fmd, engine, orc):
+    """Chain-averaged Rg on the device (src/analysis/rg.cu) against the
+    reference's own cases (tests/test_rg.cu: 2.23607 and 1.63299) and against
+    the FP64 oracle on a KG melt after Morton re-sorts and box wraps."""
+    p = np.zeros((8, 4), np.float32)
+    p[:, 0] = [1, 3, 5, 7, 2, 4, 6, 8]
+    sid = engine.add(fmd.Desc(pos4_of(p[:, :3]), 10.0, KG_LJ, mol=[0, 0, 0, 0, 1, 1, 1, 1], force_cap=10.0))
+    assert abs(engine.rg(sid) - 2.23607) < 1e-5
+    engine.remove(sid)
+    p = np.zeros((3, 4), np.float32)
+    p[:, 0] = [9, 1, 3]
+    im = np.zeros((3, 3), np.int32)
+    im[1:, 0] = 1
+    sid = engine.add(fmd.Desc(pos4_of(p[:, :3]), 10.0, KG_LJ, image=im, mol=[0, 0, 0], force_cap=10.0))
+    assert abs(engine.rg(sid) - 1.63299) < 1e-5
+    engine.remove(sid)
+    d = relax_kg(fmd, engine, 20, 50, seed=31)
+    sid = engine.add(d)
+    engine.step(sid, 100)
+    p4, _, im = engine.state(sid)
+    rg, per = engine.rg(sid, per_chain=True)
+    assert len(per) == 20
+    ref = orc.rg(p4, im, d.mol, np.float32(engine.state_box(sid)))
+    assert abs(rg - ref) <= 1e-6 * ref
+    assert 2.0 < rg < 6.0  # ~ sqrt(N/6) b for N = 50 beads
+    engine.remove(sid)

@@ -342,3 +342,41 @@ def test_oracle_md_nve_conserves_energy(orc, golden):
     _, _, _, _, th = orc.md_run(sysd, 300)
     e = (th[:, 1] + th[:, 2]) * 64
     assert np.max(np.abs(e - e[0])) < 0.01 * abs(e[0])
+
+
+# --- radius of gyration (tests/test_rg.cu:5-77)
+def test_rg_reference_cases(orc):
+    p = np.zeros((8, 4), F)
+    p[:, 0] = [1, 3, 5, 7, 2, 4, 6, 8]
+    assert abs(orc.rg(p, np.zeros((8, 3), np.int32), [0, 0, 0, 0, 1, 1, 1, 1], 10.0) - 2.23607) < 1e-5
+    p = np.zeros((3, 4), F)
+    p[:, 0] = [9, 1, 3]
+    im = np.zeros((3, 3), np.int32)
+    im[1:, 0] = 1
+    assert abs(orc.rg(p, im, [0, 0, 0], 10.0) - 1.63299) < 1e-5
+
+
+def test_rg_equals_pairwise_form(orc):
+    """Rg^2 = sum_ij |r_i - r_j|^2 / (2 N^2) per chain, an independent form;
+    and Rg is unchanged when a bead is shifted by a box vector with its image
+    count adjusted."""
+    rng = np.random.default_rng(4)
+    L = 12.0
+    n, nch = 60, 3
+    x = np.cumsum(rng.normal(size=(n, 3)) * 0.5, axis=0) + 3.0
+    mol = np.repeat(np.arange(nch), n // nch)
+    img = np.floor(x / L).astype(np.int32)
+    p = np.zeros((n, 4), F)
+    p[:, :3] = (x - img * L).astype(F)
+    unw = p[:, :3].astype(np.float64) + img * L
+    want = []
+    for c in range(nch):
+        y = unw[mol == c]
+        d = y[:, None, :] - y[None, :, :]
+        want.append(np.sqrt((d ** 2).sum() / (2 * len(y) ** 2)))
+    assert abs(orc.rg(p, img, mol, L) - np.mean(want)) < 1e-9 * np.mean(want) + 1e-6
+    img2 = img.copy()
+    img2[5, 1] += 1
+    p2 = p.copy()
+    p2[5, 1] = F(p[5, 1] - L)
+    assert abs(orc.rg(p2, img2, mol, L) - orc.rg(p, img, mol, L)) < 1e-5
