@@ -39,6 +39,8 @@ class SystemDesc(C.Structure):
         ("P_start", C.c_float), ("P_stop", C.c_float), ("Pdamp", C.c_float),
         ("chain_length", C.c_int32), ("seed", C.c_uint64), ("nsteps", C.c_int64),
         ("thermo_freq", C.c_int32), ("force_cap", C.c_float),
+        ("ntables", C.c_int32), ("table_idx", _ip), ("table_n", _ip), ("table_range", _fp),
+        ("table_data", _fp),
     ]
 
 
@@ -164,7 +166,9 @@ class Desc:
                  bond_types=None, bond_params=None, angles=None, angle_params=None, masses=None,
                  rc=2.5, skin=0.3, dt=0.001, ensemble=LANGEVIN, T=(1.0, 1.0), Tdamp=1.0,
                  P=(0.0, 0.0), Pdamp=1.0, chain=3, seed=42, nsteps=1000, thermo_freq=0,
-                 force_cap=0.0):
+                 force_cap=0.0, tables=None):
+        """tables = (table_idx, [(rmin, rmax, rows), ...]): table_idx holds
+        ntypes*ntypes entries (-1 = none), rows is (n, 3) of r, energy, force/r."""
         self.pos4 = np.ascontiguousarray(pos4, dtype=np.float32).reshape(-1, 4)
         n = self.pos4.shape[0]
         self.vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(n, 3)
@@ -199,6 +203,17 @@ class Desc:
         d.P_start, d.P_stop, d.Pdamp = P[0], P[1], Pdamp
         d.chain_length, d.seed, d.nsteps = chain, seed, nsteps
         d.thermo_freq, d.force_cap = thermo_freq, force_cap
+        d.ntables = 0
+        if tables is not None:
+            tidx, tabs = tables
+            self.table_idx = np.ascontiguousarray(tidx, dtype=np.int32).reshape(-1)
+            self.table_n = np.array([np.asarray(t[2]).shape[0] for t in tabs], np.int32)
+            self.table_range = np.array([[t[0], t[1]] for t in tabs], np.float32).reshape(-1)
+            self.table_data = np.ascontiguousarray(
+                np.concatenate([np.asarray(t[2], np.float32).reshape(-1, 3) for t in tabs]), dtype=np.float32)
+            d.ntables = len(tabs)
+            d.table_idx, d.table_n = _ptr(self.table_idx, C.c_int32), _ptr(self.table_n, C.c_int32)
+            d.table_range, d.table_data = _ptr(self.table_range, C.c_float), _ptr(self.table_data, C.c_float)
         self.c = d
 
     @property
@@ -254,6 +269,9 @@ class Topo:
 
     def arrays(self):
         d, n = self.c, self.c.natoms
+        nti = d.ntypes * d.ntypes if d.ntables else 0
+        tn = self._arr(d.table_n, d.ntables, np.int32, (d.ntables,))
+        npts = int(tn.sum())
         return {
             "pos4": self._arr(d.pos, 4 * n, np.float32, (n, 4)),
             "vel": self._arr(d.vel, 3 * n, np.float32, (n, 3)),
@@ -266,6 +284,10 @@ class Topo:
             "lj": self._arr(d.lj, 2 * d.ntypes * d.ntypes, np.float32, (d.ntypes * d.ntypes, 2)),
             "bond_params": self._arr(d.bond_params, 4 * d.nbond_types, np.float32, (d.nbond_types, 4)),
             "angle_params": self._arr(d.angle_params, 2 * d.nangle_types, np.float32, (d.nangle_types, 2)),
+            "table_idx": self._arr(d.table_idx, nti, np.int32, (nti,)),
+            "table_n": tn,
+            "table_range": self._arr(d.table_range, 2 * d.ntables, np.float32, (d.ntables, 2)),
+            "table_data": self._arr(d.table_data, 3 * npts, np.float32, (npts, 3)),
         }
 
     def extra(self):

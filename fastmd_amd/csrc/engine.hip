@@ -154,6 +154,7 @@ static Sizing sizing(const fmd_system_desc* d, int bcap, int acap, int kmult = 1
     b += (uint64_t)s.ntiles * (kNumPart * 8 + 4) + (uint64_t)((s.ntiles + kChunkTiles - 1) / kChunkTiles) * kNumPart * 8;
     b += (uint64_t)(d->natoms + 1) * 8 + (uint64_t)d->nbonds * 8 + (uint64_t)d->nangles * 48;
     b += sizeof(RepState) + (uint64_t)s.thermo_cap * sizeof(ThermoRec) + 4096;
+    for (int k = 0; k < d->ntables; k++) b += 24 + (uint64_t)d->table_n[k] * 8 + 1024;
     s.bytes = b;
     return s;
 }
@@ -186,6 +187,27 @@ static int32_t validate(const fmd_system_desc* d, std::string& msg) {
         if (t < 0 || t >= d->ntypes) { msg = "atom type out of range"; return FMD_ERR_ARG; }
     }
     if (d->nbonds < 0 || d->nangles < 0) { msg = "negative counts"; return FMD_ERR_ARG; }
+    if (d->ntables < 0 || d->ntables > 4096) { msg = "ntables out of range"; return FMD_ERR_ARG; }
+    if (d->ntables > 0) {
+        if (!d->table_idx || !d->table_n || !d->table_range || !d->table_data) {
+            msg = "tables need table_idx, table_n, table_range, table_data";
+            return FMD_ERR_ARG;
+        }
+        for (int k = 0; k < d->ntables; k++) {
+            if (d->table_n[k] < 2 || d->table_n[k] > (1 << 24)) { msg = "a table needs 2 to 2^24 points"; return FMD_ERR_ARG; }
+            const float lo = d->table_range[2 * k], hi = d->table_range[2 * k + 1];
+            if (!(hi > lo) || !std::isfinite(lo) || !std::isfinite(hi)) { msg = "table range must have rmax > rmin"; return FMD_ERR_ARG; }
+        }
+        for (int p = 0; p < d->ntypes * d->ntypes; p++) {
+            const int ti = p / d->ntypes, tj = p % d->ntypes, k = d->table_idx[p];
+            if (k < -1 || k >= d->ntables) { msg = "table index out of range"; return FMD_ERR_ARG; }
+            if (k != d->table_idx[tj * d->ntypes + ti]) { msg = "table_idx must be symmetric"; return FMD_ERR_ARG; }
+            if (k >= 0 && (d->lj[2 * p] != 0.0f || d->lj[2 * p + 1] != 0.0f)) {
+                msg = "both lj and table defined for a type pair";
+                return FMD_ERR_ARG;
+            }
+        }
+    }
     if (d->nbonds > 0 && (!d->bonds || !d->bond_types || !d->bond_params)) { msg = "bonds need bonds, bond_types, bond_params"; return FMD_ERR_ARG; }
     if (d->nangles > 0 && (!d->angles || !d->angle_params)) { msg = "angles need angles and angle_params"; return FMD_ERR_ARG; }
     for (int b = 0; b < d->nbonds; b++) {
@@ -318,6 +340,24 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     int* t_boff = nullptr; int* t_blst = nullptr; int* t_aoff = nullptr; int4* t_alst = nullptr;
     int* t_choff = nullptr;
     float2* lj = nullptr; float4* bp = nullptr; float2* ap = nullptr; float* ms = nullptr;
+    int* tidx = nullptr; float4* tparm = nullptr; int2* tno = nullptr; float2* tfe = nullptr;
+    // tables: per-table (rmin, rmax, dr, 1/dr) with dr in FP32 as
+    // src/io/table_parser.cpp:79-80, (points, offset), and (force / r, energy) rows
+    const int ntab = std::max(0, d->ntables);
+    std::vector<float4> h_tparm(ntab);
+    std::vector<int2> h_tno(ntab);
+    std::vector<float2> h_tfe;
+    for (int k = 0, off = 0; k < ntab; k++) {
+        const int n = d->table_n[k];
+        const float lo = d->table_range[2 * k], hi = d->table_range[2 * k + 1];
+        const float dr = (hi - lo) / (float)(n - 1);
+        h_tparm[k] = make_float4(lo, hi, dr, 1.0f / dr);
+        h_tno[k] = make_int2(n, off);
+        for (int p = 0; p < n; p++)
+            h_tfe.push_back(make_float2(d->table_data[3 * ((size_t)off + p) + 2],
+                                        d->table_data[3 * ((size_t)off + p) + 1]));
+        off += n;
+    }
     char* base = nullptr;
     size_t used = 0;
     auto carve = [&](size_t bytes) -> void* {
@@ -364,6 +404,14 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     ap = (float2*)carve((size_t)std::max(1, d->nangle_types) * 8);
     ms = (float*)carve(kMaxTypes * 4);
     t_choff = (int*)carve(choff.size() * 4);
+    if (ntab) {
+        tidx = (int*)carve((size_t)d->ntypes * d->ntypes * 4);
+        tparm = (float4*)carve((size_t)ntab * 16);
+        tno = (int2*)carve((size_t)ntab * 8);
+        tfe = (float2*)carve(h_tfe.size() * 8);
+    }
+    D.ntables = ntab;
+    D.tab_idx = tidx; D.tab_parm = tparm; D.tab_no = tno; D.tab_fe = tfe;
     D.t_boff = t_boff; D.t_blst = t_blst; D.t_aoff = t_aoff; D.t_alst = t_alst;
     D.t_choff = t_choff;
     D.nchains = (int)choff.size() - 1;
@@ -478,6 +526,12 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     if (d->nangle_types > 0) up(ap, d->angle_params, (size_t)d->nangle_types * 8);
     up(ms, mass.data(), kMaxTypes * 4);
     up(t_choff, choff.data(), choff.size() * 4);
+    if (ntab) {
+        up(tidx, d->table_idx, (size_t)d->ntypes * d->ntypes * 4);
+        up(tparm, h_tparm.data(), (size_t)ntab * 16);
+        up(tno, h_tno.data(), (size_t)ntab * 8);
+        up(tfe, h_tfe.data(), h_tfe.size() * 8);
+    }
     up(r.st, &S0, sizeof(S0));
     up(d_descs_ + id, &D, sizeof(RepDesc));
     if (!rc) rc = hip_check(hipStreamSynchronize(s), "sync(create)");

@@ -90,6 +90,17 @@ int main(int argc, char** argv) {
         if (!rgf) return die("rg", ex.rg_file);
         std::fprintf(rgf, "# step  Rg\n");
     }
+    // trajectory (src/io/dump.cu:12-63): traj.bin is always created; header
+    // int32 magic 0x4D444247, natoms, ntypes; per frame int64 step, int32 n,
+    // float L, then n x (x, y, z) float32, here in global-id order
+    FILE* traj = std::fopen("traj.bin", "wb");
+    if (!traj) return die("dump", "traj.bin");
+    {
+        const int32_t hdr[3] = {0x4D444247, (int32_t)n, (int32_t)d.ntypes};
+        std::fwrite(hdr, sizeof(int32_t), 3, traj);
+        std::fflush(traj);
+    }
+    std::vector<float> frame(3 * (size_t)n);
     std::vector<fmd_thermo> rec(4096);
     std::vector<float> pos(4 * (size_t)n), vel(3 * (size_t)n);
     std::vector<int32_t> img(3 * (size_t)n);
@@ -138,10 +149,29 @@ int main(int argc, char** argv) {
             int64_t to_rg = ex.rg_freq - (done % ex.rg_freq);
             if (chunk > to_rg) chunk = to_rg;
         }
+        if (ex.dump_freq > 0) {
+            int64_t to_dump = ex.dump_freq - (done % ex.dump_freq);
+            if (chunk > to_dump) chunk = to_dump;
+        }
         int32_t rc = fmd_step(eng, &sys, 1, chunk);
         if (drain()) return 1;
         if (rc) return die("step", fmd_last_error(eng));
         done += chunk;
+        if (ex.dump_freq > 0 && done % ex.dump_freq == 0) {
+            double L = 0.0;
+            fmd_get_state_box(eng, sys, &L);
+            if (fmd_get_state(eng, sys, pos.data(), nullptr, nullptr)) return die("dump", fmd_last_error(eng));
+            for (int i = 0; i < n; i++)
+                for (int k = 0; k < 3; k++) frame[3 * (size_t)i + k] = pos[4 * (size_t)i + k];
+            const int64_t st = done;
+            const int32_t nn = n;
+            const float Lf = (float)L;
+            std::fwrite(&st, sizeof(int64_t), 1, traj);
+            std::fwrite(&nn, sizeof(int32_t), 1, traj);
+            std::fwrite(&Lf, sizeof(float), 1, traj);
+            std::fwrite(frame.data(), sizeof(float), frame.size(), traj);
+            std::fflush(traj);
+        }
         if (rgf && done % ex.rg_freq == 0) {
             double rg = 0.0;
             if (fmd_get_rg(eng, sys, &rg, nullptr, 0, nullptr)) return die("rg", fmd_last_error(eng));
@@ -160,6 +190,7 @@ int main(int argc, char** argv) {
                 el > 0 ? nsteps / el : 0.0, el > 0 ? (nsteps * (double)d.dt) / (el / 86400.0) : 0.0);
     if (tf) std::fclose(tf);
     if (rgf) std::fclose(rgf);
+    std::fclose(traj);
     fmd_engine_destroy(eng);
     fmd_topo_free(topo);
     std::printf("Simulation complete.\n");

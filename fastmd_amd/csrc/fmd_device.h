@@ -140,6 +140,13 @@ struct RepDesc {
     const float4* bparm;   // per bond type (k, R0, eps, sig)
     const float2* aparm;   // per angle type (k, theta0)
     const float* mass;     // per type
+    // tabulated pair potentials (src/force/table.cu:6-182): a type pair has an
+    // LJ entry or a table; rows are interpolated linearly in r
+    int ntables;
+    const int* tab_idx;    // [ntypes^2] table of the pair, -1 = none
+    const float4* tab_parm;  // per table: rmin, rmax, dr, 1/dr
+    const int2* tab_no;    // per table: points, offset into tab_fe
+    const float2* tab_fe;  // per point: force / r, energy
     // reductions and records
     double* wpart;         // [ntiles][kNumPart]
     float* tdisp;          // [ntiles] largest squared displacement since the last rebuild (-1: non-finite)

@@ -35,6 +35,71 @@ static void copy_name(char* dst, const std::string& s) {
 }
 
 // ---------------------------------------------------------------------------
+// Tabulated potential file (src/io/table_parser.cpp:7-87): blank lines and
+// lines starting with '#' are skipped; the table starts at the line whose first
+// word is the keyword; the next line is "N n" optionally followed by
+// "R rlo rhi"; then n lines "index r energy force".  The r column must
+// increase; without R the range is the first and last r.
+// ---------------------------------------------------------------------------
+int32_t parse_table_file(const std::string& path, const std::string& keyword, TableRead& out) {
+    std::ifstream in(path);
+    if (!in.is_open()) return fail(FMD_ERR_IO, "Cannot open table file: " + path);
+    std::string line;
+    bool found = false, has_r = false;
+    int n = 0;
+    float rlo = 0.0f, rhi = 0.0f;
+    while (std::getline(in, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream iss(line);
+        std::string first;
+        iss >> first;
+        if (first != keyword) continue;
+        found = true;
+        while (std::getline(in, line)) {
+            if (line.empty() || line[0] == '#') continue;
+            std::istringstream hs(line);
+            std::string k;
+            hs >> k;
+            if (k == "N") {
+                hs >> n;
+                std::string rk;
+                if (hs >> rk && rk == "R") {
+                    hs >> rlo >> rhi;
+                    has_r = true;
+                }
+            }
+            break;
+        }
+        break;
+    }
+    if (!found) return fail(FMD_ERR_PARSE, "Table keyword not found: " + keyword);
+    if (n < 2) return fail(FMD_ERR_PARSE, "table needs N >= 2 points: " + keyword);
+    out.rows.clear();
+    while ((int)(out.rows.size() / 3) < n && std::getline(in, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream iss(line);
+        int idx;
+        float r, e, f;
+        if (!(iss >> idx >> r >> e >> f)) return fail(FMD_ERR_PARSE, "bad table line in " + path);
+        out.rows.push_back(r);
+        out.rows.push_back(e);
+        out.rows.push_back(f);
+    }
+    if ((int)(out.rows.size() / 3) != n) return fail(FMD_ERR_PARSE, "Table data line count mismatch in " + path);
+    for (int i = 1; i < n; i++)
+        if (!(out.rows[3 * i] > out.rows[3 * (i - 1)]))
+            return fail(FMD_ERR_PARSE, "Table r values not monotonically increasing");
+    if (!has_r) {
+        rlo = out.rows[0];
+        rhi = out.rows[3 * (n - 1)];
+    }
+    out.n = n;
+    out.rmin = rlo;
+    out.rmax = rhi;
+    return FMD_OK;
+}
+
+// ---------------------------------------------------------------------------
 // run.conf: one "key values..." per line, '#' in column 0 starts a comment,
 // unknown keys are ignored (src/io/config.cpp:26-128).
 // ---------------------------------------------------------------------------
@@ -46,7 +111,8 @@ int32_t parse_conf(const std::string& path, Topo& t) {
     std::vector<LJE> lj;
     std::vector<std::array<float, 4>> btyp;
     std::vector<std::array<float, 2>> atyp;
-    bool have_table = false;
+    struct TabE { int ti, tj; std::string file, keyword; };
+    std::vector<TabE> tabs;
     std::string line;
     while (std::getline(in, line)) {
         if (line.empty() || line[0] == '#') continue;
@@ -101,7 +167,9 @@ int32_t parse_conf(const std::string& path, Topo& t) {
             iss >> e.ti >> e.tj >> e.eps >> e.sig;
             lj.push_back(e);
         } else if (key == "table") {
-            have_table = true;
+            TabE e{0, 0, "", ""};
+            iss >> e.ti >> e.tj >> e.file >> e.keyword;
+            tabs.push_back(e);
         } else if (key == "bond_type") {
             int ty = 0; float k = 0, R0 = 0, eps = 0, sig = 0;
             iss >> ty >> k >> R0 >> eps >> sig;
@@ -121,8 +189,6 @@ int32_t parse_conf(const std::string& path, Topo& t) {
             copy_name(t.extra.forces_dump_file, f);
         }
     }
-    if (have_table)
-        return fail(FMD_ERR_PARSE, "tabulated pair potentials ('table') are not supported by this engine");
     if (t.ntypes < 0 || t.ntypes > 16) return fail(FMD_ERR_PARSE, "ntypes must be in [0, 16]");
     t.lj.assign(2 * (size_t)t.ntypes * t.ntypes, 0.0f);
     for (auto& e : lj) {
@@ -130,6 +196,32 @@ int32_t parse_conf(const std::string& path, Topo& t) {
             return fail(FMD_ERR_PARSE, "lj type index out of range");
         t.lj[2 * (e.ti * t.ntypes + e.tj)] = e.eps; t.lj[2 * (e.ti * t.ntypes + e.tj) + 1] = e.sig;
         t.lj[2 * (e.tj * t.ntypes + e.ti)] = e.eps; t.lj[2 * (e.tj * t.ntypes + e.ti) + 1] = e.sig;
+    }
+    // tables per type pair; one table per (file, keyword) (src/io/config.cpp:136-162)
+    t.table_idx.assign((size_t)t.ntypes * t.ntypes, -1);
+    t.table_n.clear(); t.table_range.clear(); t.table_data.clear();
+    std::vector<std::pair<std::string, std::string>> seen;
+    for (auto& e : tabs) {
+        if (e.ti < 0 || e.tj < 0 || e.ti >= t.ntypes || e.tj >= t.ntypes)
+            return fail(FMD_ERR_PARSE, "table type index out of range");
+        if (t.lj[2 * (e.ti * t.ntypes + e.tj)] != 0.0f || t.lj[2 * (e.ti * t.ntypes + e.tj) + 1] != 0.0f)
+            return fail(FMD_ERR_PARSE, "Both lj and table defined for pair (" + std::to_string(e.ti) + "," +
+                                           std::to_string(e.tj) + ")");
+        int idx = -1;
+        for (size_t k = 0; k < seen.size(); k++)
+            if (seen[k].first == e.file && seen[k].second == e.keyword) idx = (int)k;
+        if (idx < 0) {
+            TableRead tr;
+            if (int32_t rc = parse_table_file(e.file, e.keyword, tr)) return rc;
+            idx = (int)seen.size();
+            seen.emplace_back(e.file, e.keyword);
+            t.table_n.push_back(tr.n);
+            t.table_range.push_back(tr.rmin);
+            t.table_range.push_back(tr.rmax);
+            t.table_data.insert(t.table_data.end(), tr.rows.begin(), tr.rows.end());
+        }
+        t.table_idx[e.ti * t.ntypes + e.tj] = idx;
+        t.table_idx[e.tj * t.ntypes + e.ti] = idx;
     }
     t.bparams.clear();
     for (auto& b : btyp) t.bparams.insert(t.bparams.end(), b.begin(), b.end());
@@ -592,6 +684,11 @@ void fill_desc(Topo& t, fmd_system_desc& d) {
     d.nsteps = t.nsteps;
     d.thermo_freq = t.extra.thermo_on ? t.extra.thermo_freq : 0;
     d.force_cap = 0.0f;
+    d.ntables = (int32_t)t.table_n.size();
+    d.table_idx = d.ntables ? t.table_idx.data() : nullptr;
+    d.table_n = d.ntables ? t.table_n.data() : nullptr;
+    d.table_range = d.ntables ? t.table_range.data() : nullptr;
+    d.table_data = d.ntables ? t.table_data.data() : nullptr;
 }
 
 }  // namespace fmd

@@ -35,9 +35,21 @@ struct Topo {
     std::vector<float> bparams;          // 4 per bond type
     std::vector<float> aparams;          // 2 per angle type
     std::vector<float> masses;           // per type
+    std::vector<int32_t> table_idx;      // ntypes^2, -1 = no table
+    std::vector<int32_t> table_n;        // points per table
+    std::vector<float> table_range;      // rmin, rmax per table
+    std::vector<float> table_data;       // r, e, f per point
 
     int natoms() const { return (int)(pos4.size() / 4); }
 };
+
+// One table of a tabulated potential file (src/io/table_parser.cpp:7-87).
+struct TableRead {
+    int n = 0;
+    float rmin = 0.0f, rmax = 0.0f;
+    std::vector<float> rows;  // r, e, f per point
+};
+int32_t parse_table_file(const std::string& path, const std::string& keyword, TableRead& out);
 
 // Errors are reported as (status, message); the message is kept per thread.
 void set_host_error(const std::string& msg);

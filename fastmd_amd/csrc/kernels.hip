@@ -1018,6 +1018,47 @@ __device__ __forceinline__ LJSum lj_variants(const RepDesc& d, const Tile& t, in
 }
 #undef LJ_ARGS
 
+// Tabulated pairs over the same Verlet rows (src/force/table.cu:58-118).  A
+// pair inside the cutoff (the grid-unit test of the LJ loop) whose types have a
+// table gets the linear interpolant of force / r and energy at r clamped to
+// [rmin, rmax]: node k sits at rmin + k dr, k = (int)((r - rmin) / dr) clamped
+// to [0, n - 2], weight t = (r - (rmin + k dr)) / dr.  Half the pair energy and
+// half the pair virial go to this atom, as in the LJ loop.
+template <bool W16>
+__device__ LJSum table_loop(const RepDesc& d, int lt, int lane, int i, int4 pi, float L) {
+    const int nn = d.nnbr[i];
+    const int* nl = d.nbr + (size_t)lt * d.kcap * kTile + lane;
+    const unsigned short* nl16 = (const unsigned short*)(d.nbr + (size_t)lt * d.kcap * kTile) + lane;
+    const float u = L * 2.3283064365386963e-10f;
+    const GridCut gc = grid_cut(d.rc2, u);
+    const int* trow = d.tab_idx + pi.w * d.ntypes;
+    LJSum s{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < nn; k++) {
+        const int j = W16 ? (int)nl16[k * kTile] : nl[k * kTile];
+        const int4 pj = d.ipos[j];
+        const int tb = trow[pj.w];
+        if (tb < 0) continue;
+        const float ax = grid_d(pi.x, pj.x), ay = grid_d(pi.y, pj.y), az = grid_d(pi.z, pj.z);
+        const float q = grid_q(ax, ay, az);
+        if (!(q < gc.rc2g && q > gc.lo_g)) continue;
+        const float dx = ax * u, dy = ay * u, dz = az * u;
+        const float4 tp = d.tab_parm[tb];
+        const int2 tn = d.tab_no[tb];
+        const float r = fminf(fmaxf(sqrtf(sq3(dx, dy, dz)), tp.x), tp.y);
+        const int idx = min(max((int)((r - tp.x) * tp.w), 0), tn.x - 2);
+        const float t = (r - (tp.x + (float)idx * tp.z)) * tp.w;
+        const float2 p0 = d.tab_fe[tn.y + idx], p1 = d.tab_fe[tn.y + idx + 1];
+        const float fs = p0.x + t * (p1.x - p0.x);
+        const float e = p0.y + t * (p1.y - p0.y);
+        const float hx = 0.5f * fs * dx, hy = 0.5f * fs * dy, hz = 0.5f * fs * dz;
+        s.fx += fs * dx; s.fy += fs * dy; s.fz += fs * dz;
+        s.pe += 0.5f * e;
+        s.w0 += hx * dx; s.w1 += hx * dy; s.w2 += hx * dz;
+        s.w3 += hy * dy; s.w4 += hy * dz; s.w5 += hz * dz;
+    }
+    return s;
+}
+
 // ---------------------------------------------------------------------------
 // Force kernel: LJ over the Verlet list (src/force/lj.cu:27-63), FENE+WCA
 // (src/force/fene.cu:21-65) and harmonic angle (src/force/angle.cu:22-79) as
@@ -1053,6 +1094,12 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
                                     : lj_variants<false>(d, t, lane, i, pi, L, obs);
             fx = a.fx; fy = a.fy; fz = a.fz; pe = a.pe;
             w0 = a.w0; w1 = a.w1; w2 = a.w2; w3 = a.w3; w4 = a.w4; w5 = a.w5;
+            if (d.ntables) {
+                const LJSum b = d.idx16 ? table_loop<true>(d, t.lt, lane, i, pi, L)
+                                        : table_loop<false>(d, t.lt, lane, i, pi, L);
+                fx += b.fx; fy += b.fy; fz += b.fz; pe += b.pe;
+                w0 += b.w0; w1 += b.w1; w2 += b.w2; w3 += b.w3; w4 += b.w4; w5 += b.w5;
+            }
         }
         const int hdr = (terms & 6) ? d.bhdr[i] : 0;
         if (terms & 2) {

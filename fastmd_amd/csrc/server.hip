@@ -28,8 +28,8 @@ uint64_t system_bytes(const fmd_system_desc* d);
 // Owned deep copy of a system descriptor.
 struct DescCopy {
     fmd_system_desc d{};
-    std::vector<float> pos, vel, masses, lj, bp, ap;
-    std::vector<int32_t> img, mol, bonds, btypes, angles;
+    std::vector<float> pos, vel, masses, lj, bp, ap, trange, tdata;
+    std::vector<int32_t> img, mol, bonds, btypes, angles, tidx, tn;
     explicit DescCopy(const fmd_system_desc& s) : d(s) {
         auto cp = [](auto& v, const auto* p, size_t n) {
             if (p && n) v.assign(p, p + n);
@@ -47,6 +47,13 @@ struct DescCopy {
         d.lj = cp(lj, s.lj, 2 * nt * nt);
         d.bond_params = cp(bp, s.bond_params, 4 * (size_t)s.nbond_types);
         d.angle_params = cp(ap, s.angle_params, 2 * (size_t)s.nangle_types);
+        const size_t ntab = s.ntables > 0 ? (size_t)s.ntables : 0;
+        size_t npts = 0;
+        for (size_t k = 0; ntab && s.table_n && k < ntab; k++) npts += (size_t)std::max(0, s.table_n[k]);
+        d.table_idx = ntab ? cp(tidx, s.table_idx, nt * nt) : nullptr;
+        d.table_n = cp(tn, s.table_n, ntab);
+        d.table_range = cp(trange, s.table_range, 2 * ntab);
+        d.table_data = cp(tdata, s.table_data, 3 * npts);
     }
 };
 

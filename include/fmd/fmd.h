@@ -82,6 +82,13 @@ typedef struct fmd_system_desc {
     int32_t thermo_freq;       /* 0 = no thermo records                            */
     float force_cap;           /* > 0: soft push-off: cap |F_pair| of LJ and limit the */
                                /* displacement per step to 0.02 sigma; 0 = off         */
+    /* Tabulated pair potentials (src/force/table.cu:6-182), added to LJ on the
+     * same neighbor rows; a type pair has either an LJ entry or a table. */
+    int32_t ntables;
+    const int32_t* table_idx;  /* ntypes*ntypes: table of the pair, -1 = none      */
+    const int32_t* table_n;    /* ntables: points per table                        */
+    const float* table_range;  /* ntables*2: rmin, rmax                            */
+    const float* table_data;   /* sum(table_n)*3: r, energy, force/r per point     */
 } fmd_system_desc;
 
 /* One thermo sample (src/analysis/thermo.cu:140-201): per-atom KE and PE,

@@ -364,7 +364,7 @@ inline void angle_literal_f32(const float* pos4, int n, const int* ang4, int nan
 
 // ---------------------------------------------------------------------------
 // FP64 yardstick for the device force kernel.
-// terms: bit 0 = LJ (src/force/lj.cu:27-63), bit 1 = FENE+WCA
+// terms: bit 0 = LJ (src/force/lj.cu:27-63) and tabulated pairs, bit 1 = FENE+WCA
 // (src/force/fene.cu:21-65), bit 2 = angle (src/force/angle.cu:22-79).
 // LJ pairs: i != j, not excluded, and inside by the device's grid-unit test
 // (lj_inside_grid); the force/r clamp of lj.cu:48 is
@@ -382,7 +382,34 @@ struct ForceParams {
     const float* ap;   // per angle type: k, theta0
     int clamp_lj, clamp_fene;
     float fcap;
+    // tabulated pairs: tab_idx[ntypes^2] (-1 = none), per table its point
+    // count and (rmin, rmax), rows (r, energy, force/r) of all tables in order
+    int ntables = 0;
+    const int* tab_idx = nullptr;
+    const int* tab_n = nullptr;
+    const float* tab_range = nullptr;
+    const float* tab_data = nullptr;
 };
+
+// Linear interpolation of one table in FP64 (src/force/table.cu:87-104): the
+// nodes sit at rmin + k dr with dr = (rmax - rmin) / (n - 1); r is clamped to
+// [rmin, rmax], k = floor((r - rmin) / dr) clamped to [0, n - 2], and
+// t = (r - rmin) / dr - k weights nodes k and k + 1.  rows hold (r, e, f).
+inline void table_eval_f64(int n, float rmin, float rmax, const float* rows, double r, double* f,
+                           double* e) {
+    const double lo = rmin, hi = rmax, dr = (hi - lo) / (double)(n - 1);
+    if (r < lo) r = lo;
+    if (r > hi) r = hi;
+    const double x = (r - lo) / dr;
+    int k = (int)std::floor(x);
+    if (k < 0) k = 0;
+    if (k > n - 2) k = n - 2;
+    const double t = x - (double)k;
+    const float* a = rows + 3 * (size_t)k;
+    const float* b = a + 3;
+    *f = (double)a[2] + t * ((double)b[2] - (double)a[2]);
+    *e = (double)a[1] + t * ((double)b[1] - (double)a[1]);
+}
 
 inline void lj_pair_f64(const float* pi, const float* pj, const ForceParams& P, float L,
                         float invL, double* fi, double* pe_i, double* vir) {
@@ -404,6 +431,27 @@ inline void lj_pair_f64(const float* pi, const float* pj, const ForceParams& P, 
     }
     fi[0] += fr * dx; fi[1] += fr * dy; fi[2] += fr * dz;
     *pe_i += 0.5 * 4.0 * eps * (sr12 - sr6);
+    vir[0] += 0.5 * fr * dx * dx; vir[1] += 0.5 * fr * dx * dy; vir[2] += 0.5 * fr * dx * dz;
+    vir[3] += 0.5 * fr * dy * dy; vir[4] += 0.5 * fr * dy * dz; vir[5] += 0.5 * fr * dz * dz;
+}
+
+// Tabulated pair (src/force/table.cu:58-118): selected like an LJ pair
+// (lj_inside_grid); half the energy and half the virial go to atom i.
+inline void table_pair_f64(const float* pi, const float* pj, const ForceParams& P,
+                           const std::vector<size_t>& toff, float L, float invL, double* fi,
+                           double* pe_i, double* vir) {
+    int ti = type_of(pi[3]), tj = type_of(pj[3]);
+    int tb = P.tab_idx[ti * P.ntypes + tj];
+    if (tb < 0) return;
+    if (!lj_inside_grid(pi, pj, P.rc * P.rc, L)) return;
+    double dx = min_image_exact_d(pi[0], pj[0], L, invL);
+    double dy = min_image_exact_d(pi[1], pj[1], L, invL);
+    double dz = min_image_exact_d(pi[2], pj[2], L, invL);
+    double fr, e;
+    table_eval_f64(P.tab_n[tb], P.tab_range[2 * tb], P.tab_range[2 * tb + 1],
+                   P.tab_data + 3 * toff[tb], std::sqrt(dx * dx + dy * dy + dz * dz), &fr, &e);
+    fi[0] += fr * dx; fi[1] += fr * dy; fi[2] += fr * dz;
+    *pe_i += 0.5 * e;
     vir[0] += 0.5 * fr * dx * dx; vir[1] += 0.5 * fr * dx * dy; vir[2] += 0.5 * fr * dx * dz;
     vir[3] += 0.5 * fr * dy * dy; vir[4] += 0.5 * fr * dy * dz; vir[5] += 0.5 * fr * dz * dz;
 }
@@ -487,10 +535,15 @@ inline void forces_f64(const float* pos4, int n, float L, const ForceParams& P, 
     if (terms & 1) {
         std::vector<int> pairs;
         pair_set_cells(pos4, n, L, invL, P.rc * 1.0001f + 1e-4f, eoff, elst, pairs);
+        std::vector<size_t> toff(P.ntables + 1, 0);
+        for (int k = 0; k < P.ntables; k++) toff[k + 1] = toff[k] + (size_t)P.tab_n[k];
         for (size_t p = 0; p < pairs.size(); p += 2) {
             int i = pairs[p], j = pairs[p + 1];
             if (type_of(pos4[4 * i + 3]) < 0 || type_of(pos4[4 * j + 3]) < 0) continue;
             lj_pair_f64(pos4 + 4 * i, pos4 + 4 * j, P, L, invL, f8 + 4 * i, f8 + 4 * i + 3, vir);
+            if (P.ntables)
+                table_pair_f64(pos4 + 4 * i, pos4 + 4 * j, P, toff, L, invL, f8 + 4 * i,
+                               f8 + 4 * i + 3, vir);
         }
     }
     if (terms & 2)

@@ -65,6 +65,10 @@ def lib():
         L.orc_forces_f64.argtypes = [_fp, C.c_int, C.c_float, C.c_int, _fp, C.c_float, _fp, _fp,
                                      C.c_int, C.c_int, C.c_float, C.c_int, _ip, _ip, _ip, _ip,
                                      C.c_int, _ip, C.c_int, _dp, _dp]
+        L.orc_forces_tab_f64.argtypes = [_fp, C.c_int, C.c_float, C.c_int, _fp, C.c_float, _fp, _fp,
+                                         C.c_int, C.c_int, C.c_float, C.c_int, _ip, _ip, _ip, _ip,
+                                         C.c_int, _ip, C.c_int, C.c_int, _ip, _ip, _fp, _fp, _dp, _dp]
+        L.orc_table_eval.argtypes = [C.c_int, C.c_float, C.c_float, _fp, C.c_int, _dp, _dp, _dp]
         L.orc_philox.argtypes = [C.POINTER(C.c_uint32)] * 3
         L.orc_normal3.argtypes = [C.c_uint64, C.c_uint32, C.c_int64, _fp]
         L.orc_nh_chain.restype = C.c_double
@@ -187,8 +191,29 @@ def angle_literal(pos4, ang4, ap, L):
     return f, v
 
 
+def pack_tables(tables):
+    """(table_idx, [(rmin, rmax, rows (n, 3) of r, energy, force/r), ...]) as
+    flat arrays: idx, points per table, ranges, rows."""
+    tidx, tabs = tables
+    tn = i32([np.asarray(t[2]).shape[0] for t in tabs])
+    tr = f32([[t[0], t[1]] for t in tabs]).reshape(-1)
+    td = f32(np.concatenate([np.asarray(t[2], np.float32).reshape(-1, 3) for t in tabs]))
+    return i32(tidx).reshape(-1), tn, tr, td
+
+
+def table_eval(rmin, rmax, rows, r):
+    """FP64 linear interpolation of one table at distances r: (force/r, energy)."""
+    rows = f32(rows).reshape(-1, 3)
+    r = np.ascontiguousarray(r, np.float64).reshape(-1)
+    f = np.zeros_like(r)
+    e = np.zeros_like(r)
+    lib().orc_table_eval(rows.shape[0], rmin, rmax, _p(rows, C.c_float), len(r), _p(r, C.c_double),
+                         _p(f, C.c_double), _p(e, C.c_double))
+    return f, e
+
+
 def forces_f64(pos4, L, lj, ntypes=1, rc=2.5, bonds=None, btypes=None, bp=None, ang4=None, ap=None,
-               terms=7, clamp_lj=1, clamp_fene=1, fcap=0.0):
+               terms=7, clamp_lj=1, clamp_fene=1, fcap=0.0, tables=None):
     pos4 = f32(pos4)
     n = pos4.shape[0]
     bonds = i32(np.zeros((0, 2)) if bonds is None else bonds).reshape(-1, 2)
@@ -199,6 +224,15 @@ def forces_f64(pos4, L, lj, ntypes=1, rc=2.5, bonds=None, btypes=None, bp=None, 
     eo, el = exclusions(n, bonds)
     f8 = np.zeros((n, 4), np.float64)
     vir = np.zeros(6, np.float64)
+    if tables is not None:
+        tidx, tn, tr, td = pack_tables(tables)
+        lib().orc_forces_tab_f64(_p(pos4, C.c_float), n, L, ntypes, _p(f32(lj), C.c_float), rc,
+                                 _p(bp, C.c_float), _p(ap, C.c_float), clamp_lj, clamp_fene, fcap, terms,
+                                 _p(eo, C.c_int32), _p(el, C.c_int32), _p(bonds, C.c_int32),
+                                 _p(btypes, C.c_int32), len(bonds), _p(ang4, C.c_int32), len(ang4),
+                                 len(tn), _p(tidx, C.c_int32), _p(tn, C.c_int32), _p(tr, C.c_float),
+                                 _p(td, C.c_float), _p(f8, C.c_double), _p(vir, C.c_double))
+        return f8, vir
     lib().orc_forces_f64(_p(pos4, C.c_float), n, L, ntypes, _p(f32(lj), C.c_float), rc,
                          _p(bp, C.c_float), _p(ap, C.c_float), clamp_lj, clamp_fene, fcap, terms,
                          _p(eo, C.c_int32), _p(el, C.c_int32), _p(bonds, C.c_int32),

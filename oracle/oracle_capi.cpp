@@ -79,6 +79,24 @@ void orc_forces_f64(const float* pos4, int n, float L, int ntypes, const float* 
     forces_f64(pos4, n, L, P, terms, eoff, elst, bonds, btype, nbonds, ang4, nang, f8, vir);
 }
 
+// forces_f64 with tabulated pairs
+void orc_forces_tab_f64(const float* pos4, int n, float L, int ntypes, const float* lj2, float rc,
+                        const float* bp, const float* ap, int clamp_lj, int clamp_fene, float fcap,
+                        int terms, const int* eoff, const int* elst, const int* bonds,
+                        const int* btype, int nbonds, const int* ang4, int nang, int ntables,
+                        const int* tab_idx, const int* tab_n, const float* tab_range,
+                        const float* tab_data, double* f8, double* vir) {
+    ForceParams P{ntypes, lj2, rc, bp, ap, clamp_lj, clamp_fene, fcap};
+    P.ntables = ntables; P.tab_idx = tab_idx; P.tab_n = tab_n;
+    P.tab_range = tab_range; P.tab_data = tab_data;
+    forces_f64(pos4, n, L, P, terms, eoff, elst, bonds, btype, nbonds, ang4, nang, f8, vir);
+}
+
+void orc_table_eval(int n, float rmin, float rmax, const float* rows, int nr, const double* r,
+                    double* f, double* e) {
+    for (int k = 0; k < nr; k++) table_eval_f64(n, rmin, rmax, rows, r[k], f + k, e + k);
+}
+
 void orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
     philox4x32_10(ctr, key, out);
 }

@@ -32,7 +32,7 @@ skin 0.3
 dt 0.001
 nvt_langevin 1.0 1.0 1.0 42
 nsteps 400
-dump_freq 0
+dump_freq 100
 thermo 1 50 thermo.dat
 rg 1 100 rg.dat
 lj 0 0 1.0 1.0
@@ -60,6 +60,19 @@ restart 200 restart.data
     assert rg[0] == "# step  Rg" and len(rg) == 5
     assert [int(l.split()[0]) for l in rg[1:]] == [100, 200, 300, 400]
     assert all(2.0 < float(l.split()[1]) < 6.0 for l in rg[1:])
+    # trajectory frames every 100 steps (src/io/dump.cu layout)
+    raw = (tmp_path / "traj.bin").read_bytes()
+    hdr = np.frombuffer(raw[:12], np.int32)
+    assert list(hdr) == [0x4D444247, 2000, 1]
+    fsz = 8 + 4 + 4 + 12 * 2000
+    assert len(raw) == 12 + 4 * fsz
+    for k in range(4):
+        off = 12 + k * fsz
+        assert np.frombuffer(raw[off:off + 8], np.int64)[0] == 100 * (k + 1)
+        assert np.frombuffer(raw[off + 8:off + 12], np.int32)[0] == 2000
+        Lk = np.frombuffer(raw[off + 12:off + 16], np.float32)[0]
+        xyz = np.frombuffer(raw[off + 16:off + fsz], np.float32).reshape(2000, 3)
+        assert abs(Lk - L) < 1e-4 and xyz.min() >= 0.0 and xyz.max() <= Lk
     # restart files read back with the same topology
     for name in ("restart_200.data", "restart_final.data"):
         t2 = fmd.Topo.from_data(str(tmp_path / name))

@@ -223,3 +223,72 @@ def test_admission_replay_fifo_and_budget(fmd):
 def test_admission_replay_rejects_oversize(fmd):
     ta, tf = fmd.admission_replay([0.0, 1.0, 2.0], np.array([10, 500, 10], np.uint64), [1.0, 1.0, 1.0], 100)
     assert ta[1] == -1.0 and ta[0] == 0.0 and ta[2] == 2.0
+
+
+# tabulated potential files (tests/test_table_parser.cu:5-63, tests/test_config.cu:5-40)
+_TABLE_VALID = """# comment
+LJ_TABLE
+N 5 R 0.5 1.5
+
+1 0.5000 100.0000 -50.0000
+2 0.7500  50.0000 -25.0000
+3 1.0000   0.0000   0.0000
+4 1.2500 -10.0000   5.0000
+5 1.5000  -5.0000   2.5000
+"""
+
+
+def _table_conf(tmp_path, lines, lj=""):
+    dp = _write(tmp_path, "m.data", open(os.path.join(GOLDEN, "mini.data")).read())
+    return _write(tmp_path, "t.conf", f"natoms 4\nntypes 1\nrc 2.5\nskin 0.3\ndt 0.001\nnsteps 1\n"
+                                      f"nvt_langevin 1.0 1.0 1.0\n{lj}bond_type 0 30 1.5 1 1\n"
+                                      f"angle_type 0 5 180\nlammps_data_file {dp}\n{lines}")
+
+
+def test_table_file_with_range(fmd, tmp_path):
+    tf = _write(tmp_path, "tab.txt", _TABLE_VALID)
+    a = fmd.Topo.from_conf(_table_conf(tmp_path, f"table 0 0 {tf} LJ_TABLE\n")).arrays()
+    assert a["table_idx"].tolist() == [0] and a["table_n"].tolist() == [5]
+    assert a["table_range"].tolist() == [[0.5, 1.5]]
+    # rows are (r, energy, force/r) as in the file
+    assert a["table_data"][0].tolist() == [0.5, 100.0, -50.0]
+    assert a["table_data"][4].tolist() == [1.5, -5.0, 2.5]
+
+
+def test_table_file_n_only_header_takes_range_from_rows(fmd, tmp_path):
+    tf = _write(tmp_path, "tab.txt", "PAIR_0\nN 3\n\n1 0.5 10.0 -5.0\n2 1.0  5.0 -2.5\n3 1.5  0.0  0.0\n")
+    a = fmd.Topo.from_conf(_table_conf(tmp_path, f"table 0 0 {tf} PAIR_0\n")).arrays()
+    assert a["table_range"].tolist() == [[0.5, 1.5]] and a["table_n"].tolist() == [3]
+
+
+def test_table_two_pairs_share_one_file_keyword(fmd, tmp_path):
+    tf = _write(tmp_path, "tab.txt", _TABLE_VALID + "\nOTHER\nN 2 R 0.5 1.0\n\n1 0.5 0 0\n2 1.0 0 0\n")
+    dp = _write(tmp_path, "m.data", open(os.path.join(GOLDEN, "mini.data")).read())
+    conf = (f"natoms 4\nntypes 3\nrc 2.5\nskin 0.3\ndt 0.001\nnsteps 1\nnvt_langevin 1 1 1\n"
+            f"lj 0 0 1 1\nbond_type 0 30 1.5 1 1\nangle_type 0 5 180\nlammps_data_file {dp}\n"
+            f"table 0 1 {tf} LJ_TABLE\ntable 1 2 {tf} LJ_TABLE\ntable 2 2 {tf} OTHER\n")
+    a = fmd.Topo.from_conf(_write(tmp_path, "c.conf", conf)).arrays()
+    idx = a["table_idx"].reshape(3, 3)
+    assert idx.tolist() == [[-1, 0, -1], [0, -1, 0], [-1, 0, 1]]
+    assert a["table_n"].tolist() == [5, 2] and a["table_data"].shape == (7, 3)
+
+
+@pytest.mark.parametrize("text,keyword,code", [
+    ("WRONG\nN 2 R 0 1\n\n1 0 0 0\n2 1 0 0\n", "RIGHT", "PARSE"),           # keyword missing
+    ("K\nN 3 R 0 1\n\n1 0 0 0\n2 1 0 0\n", "K", "PARSE"),                    # fewer rows than N
+    ("K\nN 3\n\n1 0.5 0 0\n2 0.5 0 0\n3 1.0 0 0\n", "K", "PARSE"),          # r not increasing
+    (None, "K", "IO"),                                                      # no such file
+])
+def test_table_file_errors(fmd, tmp_path, text, keyword, code):
+    tf = _write(tmp_path, "tab.txt", text) if text is not None else str(tmp_path / "none.txt")
+    with pytest.raises(fmd.FmdError) as ei:
+        fmd.Topo.from_conf(_table_conf(tmp_path, f"table 0 0 {tf} {keyword}\n"))
+    assert ei.value.code == getattr(fmd.api, "ERR_" + code)
+
+
+def test_table_and_lj_on_one_pair_is_rejected(fmd, tmp_path):
+    # tests/test_config.cu:25-40
+    tf = _write(tmp_path, "tab.txt", "LJ_TABLE\nN 2 R 0.5 1.0\n\n1 0.5 0 0\n2 1.0 0 0\n")
+    with pytest.raises(fmd.FmdError) as ei:
+        fmd.Topo.from_conf(_table_conf(tmp_path, f"table 0 0 {tf} LJ_TABLE\n", lj="lj 0 0 1.0 1.0\n"))
+    assert ei.value.code == fmd.api.ERR_PARSE and "lj and table" in str(ei.value)

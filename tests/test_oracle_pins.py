@@ -380,3 +380,104 @@ def test_rg_equals_pairwise_form(orc):
     p2 = p.copy()
     p2[5, 1] = F(p[5, 1] - L)
     assert abs(orc.rg(p2, img2, mol, L) - orc.rg(p, img, mol, L)) < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# Tabulated pairs (src/force/table.cu:6-182)
+# ---------------------------------------------------------------------------
+def _nodes():
+    r = 0.5 + 0.125 * np.arange(17)          # exact binary nodes on [0.5, 2.5]
+    return r, np.stack([r, 3.0 - r, r * r], axis=1).astype(F)  # rows (r, e, f)
+
+
+def test_table_interpolant_closed_form(orc):
+    """The linear interpolant of f(r) = r^2 on the segment [r_k, r_k+1] holding
+    r is r^2 + (r - r_k)(r_k+1 - r); e(r) = 3 - r is reproduced exactly; r is
+    clamped to the table's range.  Segments are located by searchsorted."""
+    nodes, rows = _nodes()
+    rng = np.random.default_rng(5)
+    r = np.concatenate([rng.uniform(0.2, 2.8, 4000), nodes, [0.0, 0.49, 2.51, 9.0]])
+    f, e = orc.table_eval(0.5, 2.5, rows, r)
+    rc = np.clip(r, 0.5, 2.5)
+    k = np.clip(np.searchsorted(nodes, rc, side="right") - 1, 0, 15)
+    want_f = rc * rc + (rc - nodes[k]) * (nodes[k + 1] - rc)
+    assert np.max(np.abs(f - want_f)) < 1e-13
+    assert np.max(np.abs(e - (3.0 - rc))) < 1e-13
+    # at the nodes the interpolant is the tabulated value
+    fn, en = orc.table_eval(0.5, 2.5, rows, nodes)
+    assert np.array_equal(fn, rows[:, 2].astype(np.float64)) and np.array_equal(en, rows[:, 1].astype(np.float64))
+
+
+def _spaced_points(n, L, dmin, seed):
+    rng = np.random.default_rng(seed)
+    pts = []
+    while len(pts) < n:
+        c = rng.uniform(0, L, 3)
+        if pts:
+            d = np.array(pts) - c
+            d -= L * np.rint(d / L)
+            if np.min(np.linalg.norm(d, axis=1)) < dmin:
+                continue
+        pts.append(c)
+    return np.array(pts, F)
+
+
+def test_table_pair_sums_brute_force(orc):
+    """Pair sums with a table on the (0, 1) pair only, against an O(N^2) loop
+    over minimum images in FP64 using the interpolant: force on i is
+    (f/r)(r) d_ij, half the pair energy and half the pair virial per atom."""
+    L, rc = 6.0, 2.5
+    xyz = _spaced_points(48, L, 0.8, 3)
+    types = np.arange(48) % 2
+    p4 = pos4_of(xyz, types)
+    rs = np.linspace(0.7, rc, 300)
+    rows = np.stack([rs, 2.0 * np.exp(-rs), np.sin(3 * rs) / rs], axis=1).astype(F)
+    tables = ([-1, 0, 0, -1], [(0.7, rc, rows)])
+    lj0 = np.zeros(8, F)
+    f8, vir = orc.forces_f64(p4, F(L), lj0, ntypes=2, rc=rc, terms=1, tables=tables)
+    x = xyz.astype(np.float64)
+    fb = np.zeros((48, 4))
+    vb = np.zeros(6)
+    for i in range(48):
+        for j in range(48):
+            if i == j or types[i] == types[j]:
+                continue
+            dv = x[i] - x[j]
+            dv -= L * np.rint(dv / L)
+            r = np.linalg.norm(dv)
+            if r >= rc:
+                continue
+            fr, e = orc.table_eval(0.7, rc, rows, [r])
+            fb[i, :3] += fr[0] * dv
+            fb[i, 3] += 0.5 * e[0]
+            vb += 0.5 * fr[0] * np.array([dv[0] * dv[0], dv[0] * dv[1], dv[0] * dv[2],
+                                          dv[1] * dv[1], dv[1] * dv[2], dv[2] * dv[2]])
+    assert np.abs(fb).max() > 0.1
+    assert np.max(np.abs(f8 - fb)) < 1e-10 * np.abs(fb).max()
+    assert np.max(np.abs(vir - vb)) < 1e-10 * np.abs(vb).max()
+    # momentum conservation of a pair force
+    assert np.max(np.abs(f8[:, :3].sum(axis=0))) < 1e-9
+
+
+def lj_table_rows(n=10000, rlo=0.01, rhi=2.5, eps=1.0, sig=1.0):
+    """An LJ table with the force/r clamp of src/force/lj.cu:48 (the construction
+    of tests/test_table.cu:29-45)."""
+    r = rlo + (rhi - rlo) / (n - 1) * np.arange(n)
+    sr6 = (sig / r) ** 6
+    fr = np.clip(24.0 * eps / (r * r) * (2.0 * sr6 * sr6 - sr6), -1000.0, 1000.0)
+    return np.stack([r, 4.0 * eps * (sr6 * sr6 - sr6), fr], axis=1).astype(F)
+
+
+def test_lj_table_matches_lj(orc):
+    """tests/test_table.cu:8-115: an LJ table of 10,000 points reproduces the
+    LJ forces and energies within the interpolation tolerance (1e-3)."""
+    L = 7.0
+    xyz, _ = _saw_chain(60, L, seed=4)
+    p4 = pos4_of(xyz)
+    fl, vl = orc.forces_f64(p4, F(L), KG_LJ, terms=1)
+    ft, vt = orc.forces_f64(p4, F(L), np.zeros(2, F), terms=1, tables=([0], [(0.01, 2.5, lj_table_rows())]))
+    floor = np.sqrt(np.mean(np.sum(fl[:, :3] ** 2, axis=1)))
+    err = np.max(np.linalg.norm(ft[:, :3] - fl[:, :3], axis=1) / np.maximum(np.linalg.norm(fl[:, :3], axis=1), floor))
+    assert 0 < err < 1e-3
+    assert abs(ft[:, 3].sum() - fl[:, 3].sum()) < 1e-3 * abs(fl[:, 3].sum())
+    assert np.max(np.abs(vt - vl)) < 1e-2 * np.abs(vl).max()
