@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_PKG, "libfastmd.so")
 
 OK = 0
 ERR_ARG, ERR_IO, ERR_PARSE, ERR_HIP, ERR_NOMEM, ERR_BUDGET = 1, 2, 3, 4, 5, 6
+CORR_LEVELS, CORR_POINTS = 16, 16
 ERR_NBR_OVERFLOW, ERR_STRETCH, ERR_NAN, ERR_STATE, ERR_CELL = 7, 8, 9, 10, 11
 LANGEVIN, NVT_NH, NPT_NH = 0, 1, 2
 TERM_LJ, TERM_BOND, TERM_ANGLE, TERM_ALL = 1, 2, 4, 7
@@ -40,7 +41,7 @@ class SystemDesc(C.Structure):
         ("chain_length", C.c_int32), ("seed", C.c_uint64), ("nsteps", C.c_int64),
         ("thermo_freq", C.c_int32), ("force_cap", C.c_float),
         ("ntables", C.c_int32), ("table_idx", _ip), ("table_n", _ip), ("table_range", _fp),
-        ("table_data", _fp),
+        ("table_data", _fp), ("stress_freq", C.c_int32),
     ]
 
 
@@ -57,7 +58,8 @@ class ConfExtra(C.Structure):
     _fields_ = [("thermo_on", C.c_int32), ("thermo_freq", C.c_int32), ("thermo_file", C.c_char * 256),
                 ("restart_freq", C.c_int32), ("restart_file", C.c_char * 256),
                 ("forces_dump_file", C.c_char * 256), ("dump_freq", C.c_int32),
-                ("rg_on", C.c_int32), ("rg_freq", C.c_int32), ("rg_file", C.c_char * 256)]
+                ("rg_on", C.c_int32), ("rg_freq", C.c_int32), ("rg_file", C.c_char * 256),
+                ("stress_on", C.c_int32), ("stress_freq", C.c_int32), ("stress_file", C.c_char * 256)]
 
 
 class ServerOpts(C.Structure):
@@ -105,6 +107,9 @@ SIGNATURES = [
     ("fmd_get_box", C.c_int32, [C.c_void_p, C.c_int32, _dp]),
     ("fmd_get_state_box", C.c_int32, [C.c_void_p, C.c_int32, _dp]),
     ("fmd_get_rg", C.c_int32, [C.c_void_p, C.c_int32, _dp, _dp, C.c_int32, _ip]),
+    ("fmd_corr_push", C.c_int32, [C.c_void_p, C.c_int32, _dp]),
+    ("fmd_corr_reset", C.c_int32, [C.c_void_p, C.c_int32]),
+    ("fmd_get_corr", C.c_int32, [C.c_void_p, C.c_int32, _dp, _ip, _ip]),
     ("fmd_get_thermo", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(Thermo)]),
     ("fmd_drain_thermo", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(Thermo), C.c_int32, _ip,
                                      C.POINTER(C.c_int64)]),
@@ -166,7 +171,7 @@ class Desc:
                  bond_types=None, bond_params=None, angles=None, angle_params=None, masses=None,
                  rc=2.5, skin=0.3, dt=0.001, ensemble=LANGEVIN, T=(1.0, 1.0), Tdamp=1.0,
                  P=(0.0, 0.0), Pdamp=1.0, chain=3, seed=42, nsteps=1000, thermo_freq=0,
-                 force_cap=0.0, tables=None):
+                 force_cap=0.0, tables=None, stress_freq=0):
         """tables = (table_idx, [(rmin, rmax, rows), ...]): table_idx holds
         ntypes*ntypes entries (-1 = none), rows is (n, 3) of r, energy, force/r."""
         self.pos4 = np.ascontiguousarray(pos4, dtype=np.float32).reshape(-1, 4)
@@ -204,6 +209,7 @@ class Desc:
         d.chain_length, d.seed, d.nsteps = chain, seed, nsteps
         d.thermo_freq, d.force_cap = thermo_freq, force_cap
         d.ntables = 0
+        d.stress_freq = stress_freq
         if tables is not None:
             tidx, tabs = tables
             self.table_idx = np.ascontiguousarray(tidx, dtype=np.int32).reshape(-1)
@@ -417,6 +423,22 @@ class Engine:
         out = np.zeros(n.value, np.float64)
         self._chk(lib().fmd_get_rg(self.h, sid, C.byref(rg), _ptr(out, C.c_double), n.value, C.byref(n)))
         return rg.value, out
+
+    def corr_push(self, sid, sample6):
+        """Push one stress sample (xx, xy, xz, yy, yz, zz) into the system's correlator."""
+        x = np.ascontiguousarray(sample6, np.float64).reshape(6)
+        self._chk(lib().fmd_corr_push(self.h, sid, _ptr(x, C.c_double)))
+
+    def corr_reset(self, sid):
+        self._chk(lib().fmd_corr_reset(self.h, sid))
+
+    def corr(self, sid):
+        """Unnormalised correlator sums (levels, points, 6) and counts (levels, points)."""
+        c = np.zeros((CORR_LEVELS, CORR_POINTS, 6), np.float64)
+        n = np.zeros((CORR_LEVELS, CORR_POINTS), np.int32)
+        lv = C.c_int32()
+        self._chk(lib().fmd_get_corr(self.h, sid, _ptr(c, C.c_double), _ptr(n, C.c_int32), C.byref(lv)))
+        return c[:lv.value], n[:lv.value]
 
     def state_box(self, sid):
         """Box side of the positions state() returns (differs from box() under NPT between steps)."""

@@ -145,6 +145,15 @@ int32_t fmd_get_state(fmd_engine* e, int32_t sys, float* pos4, float* vel3, int3
     return e ? e->e->get_state(sys, pos4, vel3, image3) : FMD_ERR_ARG;
 }
 int32_t fmd_get_box(fmd_engine* e, int32_t sys, double* L) { return e ? e->e->get_box(sys, L) : FMD_ERR_ARG; }
+int32_t fmd_corr_push(fmd_engine* e, int32_t sys, const double* sample6) {
+    return e ? e->e->corr_push(sys, sample6) : FMD_ERR_ARG;
+}
+int32_t fmd_corr_reset(fmd_engine* e, int32_t sys) {
+    return e ? e->e->corr_reset(sys) : FMD_ERR_ARG;
+}
+int32_t fmd_get_corr(fmd_engine* e, int32_t sys, double* corr, int32_t* counts, int32_t* levels) {
+    return e ? e->e->get_corr(sys, corr, counts, levels) : FMD_ERR_ARG;
+}
 int32_t fmd_get_rg(fmd_engine* e, int32_t sys, double* rg, double* per_chain, int32_t cap, int32_t* nchains) {
     return e ? e->e->get_rg(sys, rg, per_chain, cap, nchains) : FMD_ERR_ARG;
 }

@@ -89,6 +89,9 @@ class Engine {
     int32_t get_box(int32_t sys, double* L, bool of_positions = false);
     int32_t get_thermo(int32_t sys, fmd_thermo* out);
     int32_t get_rg(int32_t sys, double* rg, double* per_chain, int32_t cap, int32_t* nchains);
+    int32_t corr_push(int32_t sys, const double* x);
+    int32_t corr_reset(int32_t sys);
+    int32_t get_corr(int32_t sys, double* corr, int32_t* counts, int32_t* levels);
     int32_t drain_thermo(int32_t sys, fmd_thermo* out, int32_t cap, int32_t* n, int64_t* lost);
 
     int32_t autotune(const int32_t* ids, int32_t n, int reps);

@@ -155,6 +155,7 @@ static Sizing sizing(const fmd_system_desc* d, int bcap, int acap, int kmult = 1
     b += (uint64_t)(d->natoms + 1) * 8 + (uint64_t)d->nbonds * 8 + (uint64_t)d->nangles * 48;
     b += sizeof(RepState) + (uint64_t)s.thermo_cap * sizeof(ThermoRec) + 4096;
     for (int k = 0; k < d->ntables; k++) b += 24 + (uint64_t)d->table_n[k] * 8 + 1024;
+    if (d->stress_freq > 0) b += sizeof(CorrState) + 256;
     s.bytes = b;
     return s;
 }
@@ -187,6 +188,7 @@ static int32_t validate(const fmd_system_desc* d, std::string& msg) {
         if (t < 0 || t >= d->ntypes) { msg = "atom type out of range"; return FMD_ERR_ARG; }
     }
     if (d->nbonds < 0 || d->nangles < 0) { msg = "negative counts"; return FMD_ERR_ARG; }
+    if (d->stress_freq < 0) { msg = "stress_freq must be >= 0"; return FMD_ERR_ARG; }
     if (d->ntables < 0 || d->ntables > 4096) { msg = "ntables out of range"; return FMD_ERR_ARG; }
     if (d->ntables > 0) {
         if (!d->table_idx || !d->table_n || !d->table_range || !d->table_data) {
@@ -410,6 +412,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
         tno = (int2*)carve((size_t)ntab * 8);
         tfe = (float2*)carve(h_tfe.size() * 8);
     }
+    D.corr = d->stress_freq > 0 ? (CorrState*)carve(sizeof(CorrState)) : nullptr;
     D.ntables = ntab;
     D.tab_idx = tidx; D.tab_parm = tparm; D.tab_no = tno; D.tab_fe = tfe;
     D.t_boff = t_boff; D.t_blst = t_blst; D.t_aoff = t_aoff; D.t_alst = t_alst;
@@ -495,6 +498,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
         S0.V0 = (double)d->L * (double)d->L * (double)d->L;
     }
     S0.thermo_freq = d->thermo_freq;
+    S0.stress_freq = std::max(0, d->stress_freq);
     S0.thermo_cap = S.thermo_cap;
 
     hipStream_t s = stream_;
@@ -517,6 +521,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     zero(D.bhdr, ns * 4);
     zero(D.wpart, nt * kNumPart * 8);
     zero(D.tdisp, nt * 4);
+    if (D.corr) zero(D.corr, sizeof(CorrState));
     up(t_boff, boff.data(), (na + 1) * 4);
     up(t_blst, blst.data(), blst.size() * 4);
     up(t_aoff, aoff.data(), (na + 1) * 4);
@@ -1032,6 +1037,39 @@ int32_t Engine::get_rg(int32_t sys, double* rg, double* per_chain, int32_t cap, 
     *rg = s / nc;
     if (per_chain)
         for (int c = 0; c < nc && c < cap; c++) per_chain[c] = h[c];
+    return FMD_OK;
+}
+
+// Stress autocorrelator (src/analysis/correlator.cu:24-32,94-110).
+int32_t Engine::corr_push(int32_t sys, const double* x) {
+    Replica* r = rep(sys);
+    if (!r || !x) return fail(FMD_ERR_ARG, "bad arguments");
+    if (!r->desc.corr) return fail(FMD_ERR_ARG, "system has no stress correlator (stress_freq = 0)");
+    Sample6 v;
+    for (int c = 0; c < 6; c++) v.v[c] = x[c];
+    launch_corr_push(d_descs_ + sys, v, stream_);
+    return hip_check(hipGetLastError(), "corr push");
+}
+
+int32_t Engine::corr_reset(int32_t sys) {
+    Replica* r = rep(sys);
+    if (!r) return fail(FMD_ERR_ARG, "bad arguments");
+    if (!r->desc.corr) return fail(FMD_ERR_ARG, "system has no stress correlator (stress_freq = 0)");
+    return hip_check(hipMemsetAsync(r->desc.corr, 0, sizeof(CorrState), stream_), "corr reset");
+}
+
+int32_t Engine::get_corr(int32_t sys, double* corr, int32_t* counts, int32_t* levels) {
+    Replica* r = rep(sys);
+    if (!r) return fail(FMD_ERR_ARG, "bad arguments");
+    if (!r->desc.corr) return fail(FMD_ERR_ARG, "system has no stress correlator (stress_freq = 0)");
+    CorrState h;
+    int32_t rc = hip_check(hipMemcpyAsync(&h, r->desc.corr, sizeof(CorrState), hipMemcpyDeviceToHost, stream_),
+                           "copy corr");
+    if (!rc) rc = sync();
+    if (rc) return rc;
+    if (corr) std::memcpy(corr, h.corr, sizeof(h.corr));
+    if (counts) std::memcpy(counts, h.counts, sizeof(h.counts));
+    if (levels) *levels = kCorrLevels;
     return FMD_OK;
 }
 

@@ -47,6 +47,7 @@ int main(int argc, char** argv) {
                     d.T_start, d.T_stop, d.Tdamp, (unsigned long)d.seed);
     }
     if (ex.rg_on) std::printf("  rg_on");
+    if (ex.stress_on) std::printf("  stress_on");
     if (ex.restart_freq >= 0) std::printf("  restart_on");
     std::printf("\n\n");
 
@@ -188,6 +189,32 @@ int main(int argc, char** argv) {
     const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::printf("Wall time: %.3f s  |  Steps/s: %.1f  |  ns/day: %.3f\n", el,
                 el > 0 ? nsteps / el : 0.0, el > 0 ? (nsteps * (double)d.dt) / (el / 86400.0) : 0.0);
+    if (ex.stress_on) {
+        // stress autocorrelation file (src/main.cu:758-783): level by level, lags
+        // with samples, "lag_steps C_xx C_xy C_xz C_yy C_yz C_zz count", where
+        // lag_steps = stress_freq * 16^level * (lag + 1)
+        std::vector<double> corr(FMD_CORR_LEVELS * FMD_CORR_POINTS * 6);
+        std::vector<int32_t> cnt(FMD_CORR_LEVELS * FMD_CORR_POINTS);
+        int32_t levels = 0;
+        if (fmd_get_corr(eng, sys, corr.data(), cnt.data(), &levels)) return die("stress", fmd_last_error(eng));
+        FILE* sf = std::fopen(ex.stress_file, "w");
+        if (!sf) return die("stress", ex.stress_file);
+        std::fprintf(sf, "# lag_steps  C_xx  C_xy  C_xz  C_yy  C_yz  C_zz  count\n");
+        long long lag_step = ex.stress_freq;
+        for (int lv = 0; lv < levels; lv++) {
+            for (int lag = 0; lag < FMD_CORR_POINTS; lag++) {
+                const int32_t c = cnt[lv * FMD_CORR_POINTS + lag];
+                if (c == 0) continue;
+                std::fprintf(sf, "%lld", lag_step * (lag + 1));
+                for (int k = 0; k < 6; k++)
+                    std::fprintf(sf, " %.8e", corr[(lv * FMD_CORR_POINTS + lag) * 6 + k] / c);
+                std::fprintf(sf, " %d\n", c);
+            }
+            lag_step *= FMD_CORR_POINTS;
+        }
+        std::fclose(sf);
+        std::printf("Stress autocorrelation written to %s\n", ex.stress_file);
+    }
     if (tf) std::fclose(tf);
     if (rgf) std::fclose(rgf);
     std::fclose(traj);

@@ -89,7 +89,21 @@ struct RepState {
     // thermo ring
     int thermo_freq;
     int thermo_cap;
+    int stress_freq;       // correlator sample interval (0 = off)
     int64_t thermo_written;  // total records written
+};
+
+// Multiple-tau correlator of one replica (src/analysis/correlator.cu:33-92),
+// FP64 sums: per level a ring of kCorrPoints samples, the lag sums and counts,
+// and the block accumulator feeding the next level every kCorrPoints samples.
+constexpr int kCorrLevels = 16, kCorrPoints = 16;
+struct CorrState {
+    double buf[kCorrLevels * kCorrPoints * 6];
+    double corr[kCorrLevels * kCorrPoints * 6];
+    double acc[kCorrLevels * 6];
+    int counts[kCorrLevels * kCorrPoints];
+    int ins[kCorrLevels];
+    int bcount[kCorrLevels];
 };
 
 struct RepDesc {
@@ -147,6 +161,7 @@ struct RepDesc {
     const float4* tab_parm;  // per table: rmin, rmax, dr, 1/dr
     const int2* tab_no;    // per table: points, offset into tab_fe
     const float2* tab_fe;  // per point: force / r, energy
+    CorrState* corr;       // stress autocorrelator (null when off)
     // reductions and records
     double* wpart;         // [ntiles][kNumPart]
     float* tdisp;          // [ntiles] largest squared displacement since the last rebuild (-1: non-finite)

@@ -137,8 +137,9 @@ int32_t parse_conf(const std::string& path, Topo& t) {
             iss >> t.extra.thermo_on >> t.extra.thermo_freq >> f;
             copy_name(t.extra.thermo_file, f);
         } else if (key == "stress") {
-            // stress autocorrelation output is not part of the accelerated path;
-            // the key is accepted and ignored
+            std::string f;
+            iss >> t.extra.stress_on >> t.extra.stress_freq >> f;
+            copy_name(t.extra.stress_file, f);
         } else if (key == "rg") {
             std::string f;
             iss >> t.extra.rg_on >> t.extra.rg_freq >> f;
@@ -433,6 +434,9 @@ int32_t finalize_topo(Topo& t) {
     if (t.skin < 0.0f) return fail(FMD_ERR_PARSE, "skin must be >= 0");
     if (t.extra.thermo_on && t.extra.thermo_freq <= 0)
         return fail(FMD_ERR_PARSE, "thermo frequency must be > 0");
+    if (t.extra.stress_on && t.extra.stress_freq <= 0)
+        return fail(FMD_ERR_PARSE, "stress frequency must be > 0");
+    if (t.extra.rg_on && t.extra.rg_freq <= 0) return fail(FMD_ERR_PARSE, "rg frequency must be > 0");
     return FMD_OK;
 }
 
@@ -683,6 +687,7 @@ void fill_desc(Topo& t, fmd_system_desc& d) {
     d.seed = t.seed;
     d.nsteps = t.nsteps;
     d.thermo_freq = t.extra.thermo_on ? t.extra.thermo_freq : 0;
+    d.stress_freq = t.extra.stress_on ? t.extra.stress_freq : 0;
     d.force_cap = 0.0f;
     d.ntables = (int32_t)t.table_n.size();
     d.table_idx = d.ntables ? t.table_idx.data() : nullptr;

@@ -35,6 +35,9 @@ void launch_boundary(const Launch& l, int mode);
 void launch_control(const Launch& l, int op, int64_t arg);
 // Radius of gyration of each of a replica's chains.
 void launch_rg(const RepDesc* d, int nchains, double* out, hipStream_t s);
+// One stress sample (xx, xy, xz, yy, yz, zz), passed by value.
+struct Sample6 { double v[6]; };
+void launch_corr_push(const RepDesc* d, const Sample6& x, hipStream_t s);
 // Per replica of the list: (err, active, step lo, step hi).
 void launch_status(const Launch& l, int4* out);
 // Reference cell ids / Morton keys by global id for one replica.

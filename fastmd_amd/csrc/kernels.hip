@@ -183,7 +183,8 @@ __device__ void start_step(const RepDesc& d, RepState* st, int64_t s) {
     const double hdt = 0.5 * (double)d.dt, dt = (double)d.dt;
     // observables on NPT steps (the barostat needs the virial), thermo steps and
     // the last step of a run (its result is read back)
-    st->want_obs = (d.ensemble == 2) || (st->thermo_freq > 0 && (s % st->thermo_freq) == 0) || (s == st->stop);
+    st->want_obs = (d.ensemble == 2) || (st->thermo_freq > 0 && (s % st->thermo_freq) == 0) ||
+                   (st->stress_freq > 0 && (s % st->stress_freq) == 0) || (s == st->stop);
     if (d.ensemble == 0) {
         st->kT = T;
         st->vscale = 1.0f;
@@ -1240,36 +1241,59 @@ __global__ void __launch_bounds__(256) k_chunk(const RepDesc* __restrict__ descs
     }
 }
 
-// ---------------------------------------------------------------------------
-// Per-replica boundary: fixed-order FP64 reduction of the wave partials, then
-// the scalar integrator work (post-force chain / barostat of this step,
-// start-of-step scalars of the next), thermo record, step counter.
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBndThreads) k_boundary(const RepDesc* __restrict__ descs,
-                                                          const int* __restrict__ reps, int nr, int mode) {
-    __shared__ double red[13][kBndThreads / 64];
-    const RepDesc& d = descs[reps[blockIdx.x]];
-    RepState* st = d.st;
-    if (mode == BND_START) {
-        if (threadIdx.x == 0 && st->active && !st->pre_ready) start_step(d, st, st->step + 1);
-        return;
+// Multiple-tau correlator push (src/analysis/correlator.cu:33-92), by the
+// whole block: at each level thread (lag, c) adds sample * the sample lag
+// pushes back (lags up to the ring's insert position, as the reference
+// counts them), then thread 0 advances the ring and the block accumulator,
+// which hands its mean to the next level every kCorrPoints samples.  smp is
+// the sample (shared) and is replaced by the block mean on the way up.
+__device__ void corr_push(CorrState* cs, double* smp) {
+    __shared__ int cont;
+    const int t = threadIdx.x, lag = t / 6, c = t % 6;
+    for (int lv = 0; lv < kCorrLevels; lv++) {
+        const int idx = cs->ins[lv];
+        if (t < kCorrPoints * 6) {
+            const double x = smp[c];
+            const int base = lv * kCorrPoints;
+            if (lag == 0) cs->buf[(base + idx) * 6 + c] = x;
+            if (lag <= idx) {
+                const double o = (lag == 0) ? x : cs->buf[(base + idx - lag) * 6 + c];
+                cs->corr[(base + lag) * 6 + c] += x * o;
+                if (c == 0) cs->counts[base + lag]++;
+            }
+        }
+        __syncthreads();
+        if (t == 0) {
+            cs->ins[lv] = (idx + 1) % kCorrPoints;
+            const int n = cs->bcount[lv] + 1;
+            for (int k = 0; k < 6; k++) cs->acc[lv * 6 + k] += smp[k];
+            cont = n >= kCorrPoints;
+            if (cont) {
+                for (int k = 0; k < 6; k++) {
+                    smp[k] = cs->acc[lv * 6 + k] / (double)kCorrPoints;
+                    cs->acc[lv * 6 + k] = 0.0;
+                }
+            }
+            cs->bcount[lv] = cont ? 0 : n;
+        }
+        __syncthreads();
+        if (!cont) break;
     }
-    if (mode == BND_STEP && !st->active) return;
-    // fixed-order reduction: per-thread strided sums, wave shuffles, then the
-    // waves in order (the same order whatever else is in the batch)
-    // second level: lane l of a wave reads component l % 16 of chunk 4 w + l / 16
-    // (+ 4 x waves per pass), the chunks in a fixed order
-    {
-        const int lane = lane_id(), wv = threadIdx.x >> 6, c = lane & 15;
-        double a = 0.0;
-        for (int ck = 4 * wv + (lane >> 4); ck < d.nchunks; ck += 4 * (kBndThreads / 64))
-            a += d.cpart[(size_t)ck * kNumPart + c];
-        a += __shfl_xor(a, 16, 64);
-        a += __shfl_xor(a, 32, 64);
-        if (lane < 13) red[lane][wv] = a;
-    }
+}
+
+__global__ void __launch_bounds__(128) k_corr_push(const RepDesc* __restrict__ d, Sample6 x) {
+    __shared__ double smp[6];
+    if (threadIdx.x < 6) smp[threadIdx.x] = x.v[threadIdx.x];
     __syncthreads();
-    if (threadIdx.x != 0) return;
+    corr_push(d->corr, smp);
+}
+
+// Scalar tail of the per-replica boundary (thread 0): the post-force chain /
+// barostat of this step, start-of-step scalars of the next, thermo record,
+// step counter.  Returns 1 when this step's pressure tensor is a correlator
+// sample (written to smp).
+__device__ int boundary_tail(const RepDesc& d, RepState* st, int mode, const double (*red)[kBndThreads / 64],
+                             double* smp) {
     double tot[13];
     for (int c = 0; c < 13; c++) {
         tot[c] = 0.0;
@@ -1283,7 +1307,7 @@ __global__ void __launch_bounds__(kBndThreads) k_boundary(const RepDesc* __restr
         st->obs_step = st->step;
         st->pe = pe;
         for (int c = 0; c < 6; c++) { st->vir[c] = W[c]; st->kin[c] = K[c] * (double)st->v_pending * (double)st->v_pending; }
-        return;
+        return 0;
     }
     if (mode == BND_PRIME) {
         st->obs_step = st->step;
@@ -1293,7 +1317,7 @@ __global__ void __launch_bounds__(kBndThreads) k_boundary(const RepDesc* __restr
         st->v_pending = 1.0f;
         st->rec_step = st->step;
         st->pre_ready = 0;
-        return;
+        return 0;
     }
     // BND_STEP: end of step cur
     const int64_t cur = st->step + 1;
@@ -1326,9 +1350,10 @@ __global__ void __launch_bounds__(kBndThreads) k_boundary(const RepDesc* __restr
     }
     st->step = cur;
     st->rec_step = cur;
+    const double Ld = (double)st->L, V = Ld * Ld * Ld;
     if (st->want_obs && st->thermo_freq > 0 && (cur % st->thermo_freq) == 0 && st->thermo_cap > 0) {
         ThermoRec r;
-        const double N = (double)d.natoms, Ld = (double)st->L, V = Ld * Ld * Ld;
+        const double N = (double)d.natoms;
         r.step = cur;
         r.ke = 0.5 * (st->kin[0] + st->kin[3] + st->kin[5]) / N;
         r.pe = pe / N;
@@ -1338,12 +1363,53 @@ __global__ void __launch_bounds__(kBndThreads) k_boundary(const RepDesc* __restr
         d.thermo[st->thermo_written % st->thermo_cap] = r;
         st->thermo_written++;
     }
+    // stress sample: the pressure tensor of the thermo record (src/main.cu:633,658-664)
+    const int sample = d.corr && st->stress_freq > 0 && (cur % st->stress_freq) == 0;
+    if (sample)
+        for (int c = 0; c < 6; c++) smp[c] = (st->kin[c] + W[c]) / V;
     if (cur >= st->stop) {
         st->active = 0;
         st->pre_ready = 0;
     } else {
         start_step(d, st, cur + 1);
     }
+    return sample;
+}
+
+// ---------------------------------------------------------------------------
+// Per-replica boundary: fixed-order FP64 reduction of the wave partials, then
+// the scalar integrator work and records (boundary_tail), then the stress
+// correlator push on sample steps.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBndThreads) k_boundary(const RepDesc* __restrict__ descs,
+                                                          const int* __restrict__ reps, int nr, int mode) {
+    __shared__ double red[13][kBndThreads / 64];
+    __shared__ double smp[6];
+    __shared__ int push;
+    const RepDesc& d = descs[reps[blockIdx.x]];
+    RepState* st = d.st;
+    if (mode == BND_START) {
+        if (threadIdx.x == 0 && st->active && !st->pre_ready) start_step(d, st, st->step + 1);
+        return;
+    }
+    if (mode == BND_STEP && !st->active) return;
+    // fixed-order reduction: per-thread strided sums, wave shuffles, then the
+    // waves in order (the same order whatever else is in the batch)
+    // second level: lane l of a wave reads component l % 16 of chunk 4 w + l / 16
+    // (+ 4 x waves per pass), the chunks in a fixed order
+    {
+        const int lane = lane_id(), wv = threadIdx.x >> 6, c = lane & 15;
+        double a = 0.0;
+        for (int ck = 4 * wv + (lane >> 4); ck < d.nchunks; ck += 4 * (kBndThreads / 64))
+            a += d.cpart[(size_t)ck * kNumPart + c];
+        a += __shfl_xor(a, 16, 64);
+        a += __shfl_xor(a, 32, 64);
+        if (lane < 13) red[lane][wv] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) push = boundary_tail(d, st, mode, red, smp);
+    __syncthreads();
+    if (push) corr_push(d.corr, smp);
 }
 
 // Reference cell id and Morton key by global id (parity entry points).
@@ -1396,6 +1462,9 @@ void launch_decide(const Launch& l) {
 void launch_rg(const RepDesc* d, int nchains, double* out, hipStream_t s) {
     if (nchains <= 0) return;
     k_rg<<<nchains, 64, 0, s>>>(d, out);
+}
+void launch_corr_push(const RepDesc* d, const Sample6& x, hipStream_t s) {
+    k_corr_push<<<1, 128, 0, s>>>(d, x);
 }
 void launch_status(const Launch& l, int4* out) {
     if (l.nr <= 0) return;

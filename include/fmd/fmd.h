@@ -89,7 +89,14 @@ typedef struct fmd_system_desc {
     const int32_t* table_n;    /* ntables: points per table                        */
     const float* table_range;  /* ntables*2: rmin, rmax                            */
     const float* table_data;   /* sum(table_n)*3: r, energy, force/r per point     */
+    /* Multiple-tau stress autocorrelation (src/analysis/correlator.cu:33-92):
+     * the pressure tensor is pushed every stress_freq steps; 0 = off. */
+    int32_t stress_freq;
 } fmd_system_desc;
+
+/* Multiple-tau correlator shape (src/analysis/correlator.cuh:4-6). */
+#define FMD_CORR_LEVELS 16
+#define FMD_CORR_POINTS 16
 
 /* One thermo sample (src/analysis/thermo.cu:140-201): per-atom KE and PE,
  * T = 2 KE/(3N), pressure tensor P_ab = (sum m v_a v_b + W_ab)/V. */
@@ -129,6 +136,8 @@ typedef struct fmd_conf_extra {
     int32_t dump_freq;
     int32_t rg_on, rg_freq;
     char rg_file[256];
+    int32_t stress_on, stress_freq;
+    char stress_file[256];
 } fmd_conf_extra;
 int32_t fmd_topo_conf_extra(const fmd_topo* t, fmd_conf_extra* out);
 /* 1-2 exclusions, CSR by atom, sorted unique (build_exclusions,
@@ -209,6 +218,16 @@ int32_t fmd_get_state_box(fmd_engine* e, int32_t sys, double* L);
  * chain values; nchains (if non-NULL) the chain count. */
 int32_t fmd_get_rg(fmd_engine* e, int32_t sys, double* rg, double* per_chain, int32_t cap,
                    int32_t* nchains);
+/* The system's stress autocorrelator (a system created with stress_freq > 0).
+ * fmd_corr_push adds one sample (xx, xy, xz, yy, yz, zz) as
+ * MultipleTauCorrelator::push_sample (src/analysis/correlator.cu:94-99);
+ * fmd_corr_reset clears it (correlator.cu:24-32); fmd_get_corr returns the
+ * unnormalised sums corr[level][lag][6] and counts[level][lag], with
+ * FMD_CORR_LEVELS x FMD_CORR_POINTS entries (get_results, correlator.cu:101-110).
+ * Steps push the pressure tensor of every stress_freq-th step by themselves. */
+int32_t fmd_corr_push(fmd_engine* e, int32_t sys, const double* sample6);
+int32_t fmd_corr_reset(fmd_engine* e, int32_t sys);
+int32_t fmd_get_corr(fmd_engine* e, int32_t sys, double* corr, int32_t* counts, int32_t* levels);
 /* Latest reductions (thermo of the last completed step). */
 int32_t fmd_get_thermo(fmd_engine* e, int32_t sys, fmd_thermo* out);
 /* Drains the thermo records written every thermo_freq steps since the last

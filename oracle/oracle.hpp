@@ -730,4 +730,41 @@ inline double rg_ensemble(int n, const float* pos4, const int* image, const int*
     }
     return nch ? sum / nch : 0.0;
 }
+
+// ---------------------------------------------------------------------------
+// Multiple-tau correlator (src/analysis/correlator.cu:33-92), FP64 sums.
+// Level l keeps a ring of P samples; a push writes the sample at the ring's
+// insert position idx and adds sample * (the sample lag pushes back) to
+// corr[l][lag] for lag = 0..idx (the reference's min(idx + 1, P) stored count,
+// which restarts when the ring wraps), bumping counts[l][lag].  Every P pushes
+// into a level, the mean of those P samples is pushed into the next level.
+// ---------------------------------------------------------------------------
+struct MultiTau {
+    static constexpr int LV = 16, P = 16;
+    std::vector<double> buf, corr, acc;
+    std::vector<int> counts, ins, bcount;
+    MultiTau() : buf(LV * P * 6, 0.0), corr(LV * P * 6, 0.0), acc(LV * 6, 0.0), counts(LV * P, 0),
+                 ins(LV, 0), bcount(LV, 0) {}
+    void push(const double* x_in) {
+        double x[6];
+        for (int c = 0; c < 6; c++) x[c] = x_in[c];
+        for (int l = 0; l < LV; l++) {
+            const int idx = ins[l];
+            for (int c = 0; c < 6; c++) buf[(l * P + idx) * 6 + c] = x[c];
+            for (int lag = 0; lag <= idx; lag++) {
+                const int older = (idx - lag + P) % P;
+                for (int c = 0; c < 6; c++) corr[(l * P + lag) * 6 + c] += x[c] * buf[(l * P + older) * 6 + c];
+                counts[l * P + lag]++;
+            }
+            ins[l] = (idx + 1) % P;
+            for (int c = 0; c < 6; c++) acc[l * 6 + c] += x[c];
+            if (++bcount[l] < P) break;
+            for (int c = 0; c < 6; c++) {
+                x[c] = acc[l * 6 + c] / (double)P;
+                acc[l * 6 + c] = 0.0;
+            }
+            bcount[l] = 0;
+        }
+    }
+};
 }  // namespace orc

@@ -274,6 +274,18 @@ def rg(pos4, image, mol, L):
     return f(len(p), p.ctypes.data, im.ctypes.data, m.ctypes.data, float(L))
 
 
+def corr_run(samples):
+    """A fresh multiple-tau correlator fed the samples (n, 6): sums (16, 16, 6), counts (16, 16)."""
+    x = np.ascontiguousarray(samples, np.float64).reshape(-1, 6)
+    c = np.zeros((16, 16, 6), np.float64)
+    n = np.zeros((16, 16), np.int32)
+    f = lib().orc_corr_run
+    f.restype = None
+    f.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    f(len(x), x.ctypes.data, c.ctypes.data, n.ctypes.data)
+    return c, n
+
+
 def md_run(sysd, nsteps):
     """Runs the oracle MD driver on a dict describing the system (see tests)."""
     n = sysd["pos4"].shape[0]

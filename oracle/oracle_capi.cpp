@@ -328,4 +328,13 @@ double orc_rg(int n, const float* pos4, const int32_t* image, const int32_t* mol
     return rg_ensemble(n, pos4, image, mol, L);
 }
 
+// Pushes nsamples samples (6 each) through a fresh correlator; corr[16*16*6]
+// and counts[16*16] receive its sums.
+void orc_corr_run(int nsamples, const double* samples, double* corr, int32_t* counts) {
+    MultiTau m;
+    for (int k = 0; k < nsamples; k++) m.push(samples + 6 * (size_t)k);
+    for (size_t i = 0; i < m.corr.size(); i++) corr[i] = m.corr[i];
+    for (size_t i = 0; i < m.counts.size(); i++) counts[i] = m.counts[i];
+}
+
 }  // extern "C"

@@ -1,6 +1,6 @@
 """End-to-end drop-in: the fastmd binary run on a run.conf in the reference's
 key format (tests/integration/bonded_ring/fastmd_template.conf), writing the
-reference's thermo, Rg and restart files and stdout lines
+reference's thermo, Rg, trajectory, stress-correlation and restart files and stdout lines
 (src/main.cu:403-811; benchmarks/run.py:44-57 parses the stdout)."""
 import os
 import re
@@ -35,6 +35,7 @@ nsteps 400
 dump_freq 100
 thermo 1 50 thermo.dat
 rg 1 100 rg.dat
+stress 1 10 stress.dat
 lj 0 0 1.0 1.0
 bond_type 0 30.0 1.5 1.0 1.0
 angle_type 0 5.0 180.0
@@ -73,6 +74,17 @@ restart 200 restart.data
         Lk = np.frombuffer(raw[off + 12:off + 16], np.float32)[0]
         xyz = np.frombuffer(raw[off + 16:off + fsz], np.float32).reshape(2000, 3)
         assert abs(Lk - L) < 1e-4 and xyz.min() >= 0.0 and xyz.max() <= Lk
+    # stress autocorrelation (src/main.cu:758-783): 40 samples every 10 steps;
+    # level 0 has 16 lags, level 1 (means of 16) two samples -> two lags
+    assert "Stress autocorrelation written to stress.dat" in out
+    st = (tmp_path / "stress.dat").read_text().splitlines()
+    assert st[0] == "# lag_steps  C_xx  C_xy  C_xz  C_yy  C_yz  C_zz  count"
+    cols = [l.split() for l in st[1:]]
+    assert all(len(c) == 8 for c in cols)
+    assert [int(c[0]) for c in cols] == [10 * (k + 1) for k in range(16)] + [160, 320]
+    lag = np.arange(16)
+    assert [int(c[7]) for c in cols] == list(2 * (16 - lag) + np.maximum(0, 8 - lag)) + [2, 1]
+    assert all(float(c[1]) > 0 for c in cols[:1])  # <P_xx^2> at lag 0
     # restart files read back with the same topology
     for name in ("restart_200.data", "restart_final.data"):
         t2 = fmd.Topo.from_data(str(tmp_path / name))

@@ -481,3 +481,55 @@ def test_lj_table_matches_lj(orc):
     assert 0 < err < 1e-3
     assert abs(ft[:, 3].sum() - fl[:, 3].sum()) < 1e-3 * abs(fl[:, 3].sum())
     assert np.max(np.abs(vt - vl)) < 1e-2 * np.abs(vl).max()
+
+
+# ---------------------------------------------------------------------------
+# Multiple-tau stress correlator (src/analysis/correlator.cu:33-92)
+# ---------------------------------------------------------------------------
+def test_corr_constant_signal(orc):
+    # tests/test_correlator.cu:6-38: 64 pushes of (1, 0, ...) -> C_xx = 1 at every lag
+    c, n = orc.corr_run(np.tile([1.0, 0, 0, 0, 0, 0], (64, 1)))
+    seen = n[0] > 0
+    assert seen.all()
+    assert np.allclose(c[0, seen, 0] / n[0, seen], 1.0, atol=1e-12)
+    assert not c[:, :, 1:].any()
+
+
+def test_corr_decaying_signal(orc):
+    # tests/test_correlator.cu:40-72: exp(-i / 10) decays with lag
+    x = np.zeros((256, 6))
+    x[:, 0] = np.exp(-np.arange(256, dtype=np.float32) / np.float32(10.0))
+    c, n = orc.corr_run(x)
+    assert c[0, 0, 0] / n[0, 0] > c[0, 1, 0] / n[0, 1]
+
+
+def corr_by_levels(x):
+    """The correlator restated over whole sequences: level l + 1's samples are
+    means of consecutive blocks of 16 of level l's; at level l, sample t pairs
+    with sample t - lag for every lag <= t mod 16."""
+    c = np.zeros((16, 16, 6))
+    n = np.zeros((16, 16), np.int64)
+    seq = np.asarray(x, np.float64)
+    for lv in range(16):
+        t = np.arange(len(seq))
+        for lag in range(16):
+            sel = t[(t % 16) >= lag]
+            c[lv, lag] = (seq[sel] * seq[sel - lag]).sum(axis=0)
+            n[lv, lag] = len(sel)
+        m = len(seq) // 16
+        if m == 0:
+            break
+        seq = seq[:16 * m].reshape(m, 16, 6).mean(axis=1)
+    return c, n
+
+
+@pytest.mark.parametrize("nsamples", [1, 15, 16, 17, 300, 5000])
+def test_corr_equals_level_sequence_form(orc, nsamples):
+    x = np.random.default_rng(nsamples).normal(size=(nsamples, 6))
+    c, n = orc.corr_run(x)
+    cw, nw = corr_by_levels(x)
+    assert np.array_equal(n, nw)
+    assert np.allclose(c, cw, rtol=1e-12, atol=1e-12 * np.abs(cw).max())
+    # level-0 counts in closed form: pushes t < N with t mod 16 >= lag
+    lag = np.arange(16)
+    assert np.array_equal(n[0], (nsamples // 16) * (16 - lag) + np.maximum(0, nsamples % 16 - lag))
