@@ -103,10 +103,14 @@ SIGNATURES = [
     ("fmd_step", C.c_int32, [C.c_void_p, _ip, C.c_int32, C.c_int64]),
     ("fmd_step_plain", C.c_int32, [C.c_void_p, C.c_int32, C.c_int64]),
     ("fmd_system_step", C.c_int64, [C.c_void_p, C.c_int32]),
+    ("fmd_system_natoms", C.c_int32, [C.c_void_p, C.c_int32]),
     ("fmd_get_state", C.c_int32, [C.c_void_p, C.c_int32, _fp, _fp, _ip]),
     ("fmd_get_box", C.c_int32, [C.c_void_p, C.c_int32, _dp]),
     ("fmd_get_state_box", C.c_int32, [C.c_void_p, C.c_int32, _dp]),
     ("fmd_get_rg", C.c_int32, [C.c_void_p, C.c_int32, _dp, _dp, C.c_int32, _ip]),
+    ("fmd_snapshot_size", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64)]),
+    ("fmd_snapshot", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_uint64]),
+    ("fmd_system_resume", C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64, _ip]),
     ("fmd_corr_push", C.c_int32, [C.c_void_p, C.c_int32, _dp]),
     ("fmd_corr_reset", C.c_int32, [C.c_void_p, C.c_int32]),
     ("fmd_get_corr", C.c_int32, [C.c_void_p, C.c_int32, _dp, _ip, _ip]),
@@ -423,6 +427,22 @@ class Engine:
         out = np.zeros(n.value, np.float64)
         self._chk(lib().fmd_get_rg(self.h, sid, C.byref(rg), _ptr(out, C.c_double), n.value, C.byref(n)))
         return rg.value, out
+
+    def snapshot(self, sid):
+        """The system's whole device state as bytes (exact checkpoint)."""
+        n = C.c_uint64()
+        self._chk(lib().fmd_snapshot_size(self.h, sid, C.byref(n)))
+        buf = np.zeros(n.value, np.uint8)
+        self._chk(lib().fmd_snapshot(self.h, sid, buf.ctypes.data, n.value))
+        return buf.tobytes()
+
+    def resume(self, snap):
+        """A new system continuing exactly from a snapshot."""
+        buf = np.frombuffer(snap, np.uint8)
+        sid = C.c_int32()
+        self._chk(lib().fmd_system_resume(self.h, buf.ctypes.data, len(buf), C.byref(sid)))
+        self.natoms[sid.value] = lib().fmd_system_natoms(self.h, sid.value)
+        return sid.value
 
     def corr_push(self, sid, sample6):
         """Push one stress sample (xx, xy, xz, yy, yz, zz) into the system's correlator."""

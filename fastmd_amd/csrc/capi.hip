@@ -141,10 +141,23 @@ int32_t fmd_step_plain(fmd_engine* e, int32_t sys, int64_t nsteps) {
     return e ? e->e->step(&sys, 1, nsteps, false) : FMD_ERR_ARG;
 }
 int64_t fmd_system_step(fmd_engine* e, int32_t sys) { return e ? e->e->step_count(sys) : -1; }
+int32_t fmd_system_natoms(fmd_engine* e, int32_t sys) {
+    fmd::Replica* r = e ? e->e->rep(sys) : nullptr;
+    return r ? r->natoms : -1;
+}
 int32_t fmd_get_state(fmd_engine* e, int32_t sys, float* pos4, float* vel3, int32_t* image3) {
     return e ? e->e->get_state(sys, pos4, vel3, image3) : FMD_ERR_ARG;
 }
 int32_t fmd_get_box(fmd_engine* e, int32_t sys, double* L) { return e ? e->e->get_box(sys, L) : FMD_ERR_ARG; }
+int32_t fmd_snapshot_size(fmd_engine* e, int32_t sys, uint64_t* bytes) {
+    return e ? e->e->snapshot_size(sys, bytes) : FMD_ERR_ARG;
+}
+int32_t fmd_snapshot(fmd_engine* e, int32_t sys, void* buf, uint64_t cap) {
+    return e ? e->e->snapshot(sys, buf, cap) : FMD_ERR_ARG;
+}
+int32_t fmd_system_resume(fmd_engine* e, const void* buf, uint64_t size, int32_t* sys) {
+    return e ? e->e->resume(buf, size, sys) : FMD_ERR_ARG;
+}
 int32_t fmd_corr_push(fmd_engine* e, int32_t sys, const double* sample6) {
     return e ? e->e->corr_push(sys, sample6) : FMD_ERR_ARG;
 }

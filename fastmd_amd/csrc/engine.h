@@ -26,6 +26,7 @@ struct Replica {
     RepState* st = nullptr; // device
     void* block = nullptr;  // the replica's device arrays, carved from one block
     size_t block_bytes = 0;
+    size_t block_used = 0;  // bytes of the block the arrays occupy
     uint64_t bytes = 0;
     int64_t steps_done = 0; // host mirror of st->step
     int64_t thermo_read = 0;
@@ -89,6 +90,9 @@ class Engine {
     int32_t get_box(int32_t sys, double* L, bool of_positions = false);
     int32_t get_thermo(int32_t sys, fmd_thermo* out);
     int32_t get_rg(int32_t sys, double* rg, double* per_chain, int32_t cap, int32_t* nchains);
+    int32_t snapshot_size(int32_t sys, uint64_t* bytes);
+    int32_t snapshot(int32_t sys, void* buf, uint64_t cap);
+    int32_t resume(const void* buf, uint64_t size, int32_t* sys);
     int32_t corr_push(int32_t sys, const double* x);
     int32_t corr_reset(int32_t sys);
     int32_t get_corr(int32_t sys, double* corr, int32_t* counts, int32_t* levels);

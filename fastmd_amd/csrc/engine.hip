@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "kernels.h"
 
@@ -430,6 +431,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     }
     r.block = base;
     r.block_bytes = got;
+    r.block_used = block_bytes;
     r.bytes = got;
     used = 0;
     layout();
@@ -1037,6 +1039,131 @@ int32_t Engine::get_rg(int32_t sys, double* rg, double* per_chain, int32_t cap, 
     *rg = s / nc;
     if (per_chain)
         for (int c = 0; c < nc && c < cap; c++) per_chain[c] = h[c];
+    return FMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Snapshot and resume.  A replica's whole state lives in its device block
+// (per-slot arrays in the current sorted order, the Verlet list and its
+// reference positions, the thermostat / barostat and step counters, the
+// correlator and the thermo ring, the topology and parameters), and its
+// descriptor points into that block only.  A snapshot is a header, the
+// descriptor with every pointer stored as an offset into the block, and the
+// block's bytes; resuming uploads the block into a new one and rebases the
+// pointers, so the resumed replica continues bit for bit.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr uint32_t kSnapMagic = 0x534D4446u;  // "FDMS"
+constexpr uint32_t kSnapVersion = 1;
+struct SnapHeader {
+    uint32_t magic, version, desc_bytes, state_bytes;
+    uint64_t used;
+    int32_t natoms, nslots, ntiles, cls, ensemble, thermo_cap;
+    int64_t steps_done, thermo_read;
+    RepDesc desc;  // pointers as (offset into the block + 1), 0 = null
+};
+
+// Applies f to every pointer member of a descriptor.
+template <class F>
+void for_each_ptr(RepDesc& d, F&& f) {
+    f(d.pos); f(d.vel); f(d.force); f(d.pos_ref); f(d.ipos); f(d.img);
+    f(d.tmp_pos); f(d.tmp_vel); f(d.tmp_img); f(d.cell_of); f(d.rank); f(d.order); f(d.order2);
+    f(d.slot_of); f(d.cell_cnt); f(d.cell_start); f(d.nnbr); f(d.nbr); f(d.bhdr); f(d.bnd); f(d.ang);
+    f(d.t_boff); f(d.t_blst); f(d.t_aoff); f(d.t_alst); f(d.t_choff);
+    f(d.lj); f(d.bparm); f(d.aparm); f(d.mass);
+    f(d.tab_idx); f(d.tab_parm); f(d.tab_no); f(d.tab_fe); f(d.corr);
+    f(d.wpart); f(d.tdisp); f(d.cpart); f(d.st); f(d.thermo);
+}
+}  // namespace
+
+int32_t Engine::snapshot_size(int32_t sys, uint64_t* bytes) {
+    Replica* r = rep(sys);
+    if (!r || !bytes) return fail(FMD_ERR_ARG, "bad arguments");
+    *bytes = sizeof(SnapHeader) + r->block_used;
+    return FMD_OK;
+}
+
+int32_t Engine::snapshot(int32_t sys, void* buf, uint64_t cap) {
+    Replica* r = rep(sys);
+    if (!r || !buf) return fail(FMD_ERR_ARG, "bad arguments");
+    if (cap < sizeof(SnapHeader) + r->block_used) return fail(FMD_ERR_ARG, "snapshot buffer too small");
+    SnapHeader h;
+    std::memset(&h, 0, sizeof(h));
+    h.magic = kSnapMagic; h.version = kSnapVersion;
+    h.desc_bytes = sizeof(RepDesc); h.state_bytes = sizeof(RepState);
+    h.used = r->block_used;
+    h.natoms = r->natoms; h.nslots = r->nslots; h.ntiles = r->ntiles; h.cls = r->cls;
+    h.ensemble = r->ensemble; h.thermo_cap = r->thermo_cap;
+    h.steps_done = r->steps_done; h.thermo_read = r->thermo_read;
+    h.desc = r->desc;
+    const char* base = (const char*)r->block;
+    bool outside = false;
+    for_each_ptr(h.desc, [&](auto*& p) {
+        const char* q = (const char*)p;
+        if (!q) return;
+        if (q < base || q >= base + r->block_used) { outside = true; return; }
+        p = reinterpret_cast<std::remove_reference_t<decltype(p)>>((uintptr_t)(q - base) + 1);
+    });
+    if (outside) return fail(FMD_ERR_ARG, "internal: descriptor points outside the replica block");
+    int32_t rc = sync();
+    if (!rc) rc = hip_check(hipMemcpy((char*)buf + sizeof(SnapHeader), r->block, r->block_used,
+                                      hipMemcpyDeviceToHost), "copy snapshot");
+    if (rc) return rc;
+    std::memcpy(buf, &h, sizeof(h));
+    return FMD_OK;
+}
+
+int32_t Engine::resume(const void* buf, uint64_t size, int32_t* sys) {
+    if (!buf || !sys || size < sizeof(SnapHeader)) return fail(FMD_ERR_ARG, "bad arguments");
+    SnapHeader h;
+    std::memcpy(&h, buf, sizeof(h));
+    if (h.magic != kSnapMagic || h.version != kSnapVersion || h.desc_bytes != sizeof(RepDesc) ||
+        h.state_bytes != sizeof(RepState))
+        return fail(FMD_ERR_ARG, "not a snapshot of this library version");
+    if (size != sizeof(SnapHeader) + h.used) return fail(FMD_ERR_ARG, "snapshot size does not match its header");
+    bool bad = false;
+    for_each_ptr(h.desc, [&](auto*& p) {
+        const uintptr_t o = (uintptr_t)p;
+        if (o > h.used) bad = true;
+    });
+    if (bad || h.natoms <= 0 || h.nslots < h.natoms || h.cls < 0 || h.cls >= kNumClasses || !h.desc.st)
+        return fail(FMD_ERR_ARG, "corrupt snapshot header");
+    if (in_use_ + h.used > budget_)
+        return fail(FMD_ERR_BUDGET, "snapshot needs " + std::to_string(h.used) + " bytes; budget left " +
+                                        std::to_string(budget_ - in_use_));
+    int id = -1;
+    for (int k = 0; k < kMaxReplicas; k++)
+        if (!reps_[k]) { id = k; break; }
+    if (id < 0) return fail(FMD_ERR_BUDGET, "replica table full");
+    size_t got = 0;
+    char* base = (char*)acquire_block(h.used, &got);
+    if (!base) return fail(FMD_ERR_NOMEM, "device allocation failed");
+    Replica* R = new Replica();
+    Replica& r = *R;
+    r.id = id;
+    r.natoms = h.natoms; r.nslots = h.nslots; r.ntiles = h.ntiles; r.cls = h.cls;
+    r.ensemble = h.ensemble; r.thermo_cap = h.thermo_cap;
+    r.steps_done = h.steps_done; r.thermo_read = h.thermo_read;
+    r.block = base; r.block_bytes = got; r.block_used = h.used; r.bytes = got;
+    r.desc = h.desc;
+    for_each_ptr(r.desc, [&](auto*& p) {
+        const uintptr_t o = (uintptr_t)p;
+        p = o ? reinterpret_cast<std::remove_reference_t<decltype(p)>>(base + (o - 1)) : nullptr;
+    });
+    r.st = r.desc.st;
+    int32_t rc = hip_check(hipMemcpyAsync(base, (const char*)buf + sizeof(SnapHeader), h.used,
+                                          hipMemcpyHostToDevice, stream_), "upload snapshot");
+    if (!rc) rc = hip_check(hipMemcpyAsync(d_descs_ + id, &r.desc, sizeof(RepDesc), hipMemcpyHostToDevice, stream_),
+                            "upload descriptor");
+    if (!rc) rc = sync();
+    if (rc) {
+        release_block(r.block, r.block_bytes);
+        delete R;
+        return rc;
+    }
+    reps_[id] = R;
+    in_use_ += r.bytes;
+    *sys = id;
     return FMD_OK;
 }
 

@@ -202,6 +202,8 @@ int32_t fmd_step(fmd_engine* e, const int32_t* sys, int32_t nsys, int64_t nsteps
 int32_t fmd_step_plain(fmd_engine* e, int32_t sys, int64_t nsteps);
 /* Steps completed so far. */
 int64_t fmd_system_step(fmd_engine* e, int32_t sys);
+/* Bead count of a system (-1 if there is no such system). */
+int32_t fmd_system_natoms(fmd_engine* e, int32_t sys);
 /* State by global id: pos4 (x, y, z, type bits), vel3, image3; any may be NULL. */
 int32_t fmd_get_state(fmd_engine* e, int32_t sys, float* pos4, float* vel3, int32_t* image3);
 /* Current box side (changes under NPT): as in the reference, the barostat
@@ -218,6 +220,17 @@ int32_t fmd_get_state_box(fmd_engine* e, int32_t sys, double* L);
  * chain values; nchains (if non-NULL) the chain count. */
 int32_t fmd_get_rg(fmd_engine* e, int32_t sys, double* rg, double* per_chain, int32_t cap,
                    int32_t* nchains);
+/* Exact checkpoint of a system's whole device state: positions, velocities
+ * and images in the current sorted order, the Verlet list and its reference
+ * positions, thermostat / barostat state, step and rebuild counters, the
+ * stress correlator and unread thermo records.  A resumed system continues bit
+ * for bit as if it had not stopped, in this or another engine of the same
+ * library version.  The reference checkpoints through LAMMPS data restart
+ * files only (src/main.cu:736-752), which keep positions, velocities and
+ * images but not the thermostat or the list.  Call between steps. */
+int32_t fmd_snapshot_size(fmd_engine* e, int32_t sys, uint64_t* bytes);
+int32_t fmd_snapshot(fmd_engine* e, int32_t sys, void* buf, uint64_t cap);
+int32_t fmd_system_resume(fmd_engine* e, const void* buf, uint64_t size, int32_t* sys);
 /* The system's stress autocorrelator (a system created with stress_freq > 0).
  * fmd_corr_push adds one sample (xx, xy, xz, yy, yz, zz) as
  * MultipleTauCorrelator::push_sample (src/analysis/correlator.cu:94-99);
