@@ -20,7 +20,8 @@ namespace fmd {
 
 int size_class(int natoms) { return natoms < 8192 ? 0 : (natoms < 32768 ? 1 : 2); }
 
-static inline int ceil64(int n) { return (n + 63) / 64 * 64; }
+// slot count: whole groups of kGroupTiles tiles
+static inline int ceil_slots(int n) { return (n + kGroupSlots - 1) / kGroupSlots * kGroupSlots; }
 
 
 Engine::Engine(int device, uint64_t budget) : device_(device) {
@@ -127,7 +128,7 @@ static int cbits_for(int nx) {
 static Sizing sizing(const fmd_system_desc* d, int bcap, int acap, int kmult = 1) {
     Sizing s;
     s.natoms = d->natoms;
-    s.nslots = ceil64(d->natoms);
+    s.nslots = ceil_slots(d->natoms);
     s.ntiles = s.nslots / 64;
     const double rcs = (double)d->rc + (double)d->skin;
     const double rho = (double)d->natoms / ((double)d->L * d->L * d->L);
@@ -152,6 +153,7 @@ static Sizing sizing(const fmd_system_desc* d, int bcap, int acap, int kmult = 1
     b += (uint64_t)s.natoms * 4;
     b += (uint64_t)s.ncell_cap * 8 + 4;
     b += (uint64_t)s.ntiles * 64 * ((uint64_t)s.kcap * 4 + (uint64_t)s.bcap * 4 + (uint64_t)s.acap * 8);
+    b += (uint64_t)(s.ntiles / kGroupTiles) * (4 + (uint64_t)kGroupStage * 4) + 512;
     b += (uint64_t)s.ntiles * (kNumPart * 8 + 4) + (uint64_t)((s.ntiles + kChunkTiles - 1) / kChunkTiles) * kNumPart * 8;
     b += (uint64_t)(d->natoms + 1) * 8 + (uint64_t)d->nbonds * 8 + (uint64_t)d->nangles * 48;
     b += sizeof(RepState) + (uint64_t)s.thermo_cap * sizeof(ThermoRec) + 4096;
@@ -387,6 +389,8 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     D.cell_start = (int*)carve(((size_t)S.ncell_cap + 1) * 4);
     D.nnbr = (int*)carve(ns * 4);
     D.nbr = (int*)carve(nt * S.kcap * 64 * 4);
+    D.gcount = (int*)carve(nt / kGroupTiles * 4);
+    D.gstage = (int*)carve(nt / kGroupTiles * kGroupStage * 4);
     D.bhdr = (int*)carve(ns * 4);
     D.bnd = (int*)carve(nt * S.bcap * 64 * 4);
     D.ang = (int2*)carve(nt * S.acap * 64 * 8);
@@ -520,6 +524,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     zero(D.ipos, ns * 16);
     zero(D.cell_cnt, (size_t)S.ncell_cap * 4);
     zero(D.nnbr, ns * 4);
+    zero(D.gcount, nt / kGroupTiles * 4);
     zero(D.bhdr, ns * 4);
     zero(D.wpart, nt * kNumPart * 8);
     zero(D.tdisp, nt * 4);
@@ -557,7 +562,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     }
     Launch l = launch_of(*b);
     launch_rebuild(l, 1, 1);
-    launch_force(l, 0, b->ntl, 256, FMD_TERM_ALL, 0, 1);
+    launch_force(l, 0, b->ntl, 256, FMD_TERM_ALL, 0, FM_OBS);
     launch_boundary(l, BND_PRIME);
     rc = hip_check(hipStreamSynchronize(s), "sync(prime)");
     if (!rc) {
@@ -693,7 +698,7 @@ void Engine::launch_step_seq(Batch& b) {
     launch_decide(l);
     launch_rebuild(l, 0, 1);
     for (int c = 0; c < kNumClasses; c++)
-        launch_force(l, b.cls_t0[c], b.cls_nt[c], tuned_[0][c], FMD_TERM_ALL, 1, 0);
+        launch_force(l, b.cls_t0[c], b.cls_nt[c], tuned_[0][c], FMD_TERM_ALL, 1, FM_STEP);
     launch_boundary(l, BND_STEP);
 }
 
@@ -874,7 +879,7 @@ int32_t Engine::build_neighbors(int32_t sys) {
     launch_control(l, CTL_FORCE_REBUILD, 0);
     launch_rebuild(l, 1, 1);
     // slots were permuted: recompute the forces in the new order
-    launch_force(l, 0, b->ntl, 256, FMD_TERM_ALL, 0, 1);
+    launch_force(l, 0, b->ntl, 256, FMD_TERM_ALL, 0, FM_OBS);
     launch_boundary(l, BND_REDUCE);
     if (int32_t rc = sync()) return rc;
     RepState s;
@@ -893,12 +898,24 @@ int32_t Engine::get_pairs(int32_t sys, int64_t* npairs, int32_t* ij, int64_t cap
     if (!rc) rc = hip_check(hipMemcpy(nn.data(), D.nnbr, nn.size() * 4, hipMemcpyDeviceToHost), "copy nnbr");
     if (!rc) rc = hip_check(hipMemcpy(img.data(), D.img, img.size() * 16, hipMemcpyDeviceToHost), "copy img");
     if (rc) return rc;
+    // entries are LDS byte offsets into the group's neighbor set (gcount >= 0) or slots
+    const int ng = D.ntiles / kGroupTiles;
+    std::vector<int> gcount(ng), gstage((size_t)ng * kGroupStage);
+    rc = hip_check(hipMemcpy(gcount.data(), D.gcount, gcount.size() * 4, hipMemcpyDeviceToHost), "copy gcount");
+    if (!rc) rc = hip_check(hipMemcpy(gstage.data(), D.gstage, gstage.size() * 4, hipMemcpyDeviceToHost), "copy gstage");
+    if (rc) return rc;
     std::vector<std::pair<int, int>> pairs;
     for (int i = 0; i < D.natoms; i++) {
-        const int lt = i / 64, lane = i % 64;
+        const int lt = i / 64, lane = i % 64, g = lt / kGroupTiles;
         const uint16_t* row16 = (const uint16_t*)(nbr.data() + (size_t)lt * D.kcap * 64);
+        const uint16_t* gs16 = (const uint16_t*)gstage.data() + (size_t)g * kGroupStage;  // as on the device
         for (int k = 0; k < nn[i]; k++) {
-            const int j = D.idx16 ? (int)row16[(size_t)k * 64 + lane] : nbr[((size_t)lt * D.kcap + k) * 64 + lane];
+            int j = D.idx16 ? (int)row16[(size_t)k * 64 + lane] : nbr[((size_t)lt * D.kcap + k) * 64 + lane];
+            if (gcount[g] >= 0) {
+                const int q = j >> 4;
+                if ((j & 15) || q >= gcount[g]) return fail(FMD_ERR_STATE, "corrupt neighbor entry");
+                j = D.idx16 ? (int)gs16[q] : gstage[(size_t)g * kGroupStage + q];
+            }
             if (j < 0 || j >= D.natoms) return fail(FMD_ERR_STATE, "corrupt neighbor entry");
             pairs.emplace_back(img[i].w, img[j].w);
         }
@@ -951,7 +968,7 @@ int32_t Engine::compute_forces(int32_t sys, uint32_t terms, float* f4, double* v
     Batch* b = batch_for({sys});
     if (!b) return FMD_ERR_ARG;
     Launch l = launch_of(*b);
-    launch_force(l, 0, b->ntl, 256, (int)(terms & 7u), 0, 1);
+    launch_force(l, 0, b->ntl, 256, (int)(terms & 7u), 0, FM_OBS);
     launch_boundary(l, BND_REDUCE);
     int32_t rc = sync();
     if (rc) return rc;
@@ -973,7 +990,7 @@ int32_t Engine::compute_forces(int32_t sys, uint32_t terms, float* f4, double* v
         for (int c = 0; c < 6; c++) vir[c] = s.vir[c];
     if (pe) *pe = s.pe;
     if ((terms & 7u) != 7u) {  // restore the full forces the next step's kick uses
-        launch_force(l, 0, b->ntl, 256, FMD_TERM_ALL, 0, 1);
+        launch_force(l, 0, b->ntl, 256, FMD_TERM_ALL, 0, FM_OBS);
         launch_boundary(l, BND_REDUCE);
         rc = sync();
     }
@@ -1068,7 +1085,7 @@ template <class F>
 void for_each_ptr(RepDesc& d, F&& f) {
     f(d.pos); f(d.vel); f(d.force); f(d.pos_ref); f(d.ipos); f(d.img);
     f(d.tmp_pos); f(d.tmp_vel); f(d.tmp_img); f(d.cell_of); f(d.rank); f(d.order); f(d.order2);
-    f(d.slot_of); f(d.cell_cnt); f(d.cell_start); f(d.nnbr); f(d.nbr); f(d.bhdr); f(d.bnd); f(d.ang);
+    f(d.slot_of); f(d.cell_cnt); f(d.cell_start); f(d.nnbr); f(d.nbr); f(d.gcount); f(d.gstage); f(d.bhdr); f(d.bnd); f(d.ang);
     f(d.t_boff); f(d.t_blst); f(d.t_aoff); f(d.t_alst); f(d.t_choff);
     f(d.lj); f(d.bparm); f(d.aparm); f(d.mass);
     f(d.tab_idx); f(d.tab_parm); f(d.tab_no); f(d.tab_fe); f(d.corr);
@@ -1220,7 +1237,7 @@ int32_t Engine::get_thermo(int32_t sys, fmd_thermo* out) {
         Batch* b = batch_for({sys});
         if (!b) return FMD_ERR_ARG;
         Launch l = launch_of(*b);
-        launch_force(l, 0, b->ntl, 256, FMD_TERM_ALL, 0, 1);
+        launch_force(l, 0, b->ntl, 256, FMD_TERM_ALL, 0, FM_OBS);
         launch_boundary(l, BND_REDUCE);
         rc = sync();
         if (!rc) rc = read_state(*r, s);
@@ -1280,8 +1297,9 @@ int32_t Engine::time_force(const int32_t* ids, int32_t n, uint32_t terms, int re
         for (int i = 0; i < r->natoms; i++) { p += nn[i]; nb += hdr[i] & 0xFF; na += (hdr[i] >> 8) & 0xFF; }
         P += p;
         // per bead: 16 B position, 4 B row count, 16 B force; per listed pair:
-        // its 16-bit or 32-bit index (the position gathers are cache traffic)
-        if (terms & 1) B += 36.0 * r->natoms + (r->desc.idx16 ? 2.0 : 4.0) * (double)p;
+        // a 4-byte index, whatever width the rows are stored in (the position
+        // gathers are cache traffic)
+        if (terms & 1) B += 36.0 * r->natoms + 4.0 * (double)p;
         if (terms & 6) B += 4.0 * r->natoms + ((terms & 1) ? 0.0 : 32.0 * r->natoms);
         if (terms & 2) B += 4.0 * (double)nb;
         if (terms & 4) B += 8.0 * (double)na;
@@ -1295,7 +1313,7 @@ int32_t Engine::time_force(const int32_t* ids, int32_t n, uint32_t terms, int re
     for (int c = 1; c < kNumClasses; c++)
         if (b->cls_nt[c] > b->cls_nt[big]) big = c;
     const int blk = tuned_[0][big];
-    auto run = [&]() { launch_force(l, 0, b->ntl, blk, (int)(terms & 7u), 0, 1); };
+    auto run = [&]() { launch_force(l, 0, b->ntl, blk, (int)(terms & 7u), 0, FM_FAST); };
     run();
     (void)hipEventRecord(e0, stream_);
     for (int k = 0; k < reps; k++) run();
@@ -1307,7 +1325,7 @@ int32_t Engine::time_force(const int32_t* ids, int32_t n, uint32_t terms, int re
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     // restore the full forces the next step's kick reads
-    launch_force(l, 0, b->ntl, 256, FMD_TERM_ALL, 0, 1);
+    launch_force(l, 0, b->ntl, 256, FMD_TERM_ALL, 0, FM_OBS);
     if ((rc = sync())) return rc;
     if (ms) *ms = (double)t / reps;
     if (bytes) *bytes = B;
@@ -1339,7 +1357,7 @@ int32_t Engine::autotune(const int32_t* ids, int32_t n, int reps) {
             float best = 1e30f;
             int bestb = 256;
             for (int bi = 0; bi < 3 && !rc; bi++) {
-                auto run = [&]() { launch_force(l, b->cls_t0[c], b->cls_nt[c], cand[bi], FMD_TERM_ALL, 0, 1); };
+                auto run = [&]() { launch_force(l, b->cls_t0[c], b->cls_nt[c], cand[bi], FMD_TERM_ALL, 0, FM_OBS); };
                 run();  // warm
                 (void)hipEventRecord(e0, stream_);
                 for (int k = 0; k < reps; k++) run();

@@ -15,11 +15,18 @@ namespace fmd {
 
 constexpr int kTile = 64;           // slots per tile = one wave64
 constexpr int kMaxReplicas = 4096;  // entries of an engine's replica table
-// The list build stages a tile's candidate atoms -- the union of the
-// 27-neighborhoods of the cells holding its atoms -- in LDS when they fit: at
-// most kMaxUnion cells and kStageAtoms atoms.
-constexpr int kMaxUnion = 128;
-constexpr int kStageAtoms = 1792;
+// Groups of kGroupTiles consecutive tiles of a replica (a replica's slot count
+// is a multiple of the group size) share one neighborhood.  The list build
+// stages a group's candidate atoms -- the union of the 27-neighborhoods of the
+// cells holding its atoms, at most kMaxUnion cells and kBuildStage atoms -- in
+// LDS.  The atoms the group's rows actually reference (its neighbor set, at
+// most kGroupStage) are listed per group; the force kernel stages them in LDS
+// and the rows hold their LDS byte offsets.
+constexpr int kGroupTiles = 4;
+constexpr int kGroupSlots = kGroupTiles * kTile;
+constexpr int kMaxUnion = 256;
+constexpr int kBuildStage = 3072;
+constexpr int kGroupStage = 2048;
 constexpr int kMaxTypes = 16;
 constexpr int kMaxChain = 10;
 constexpr int kNumPart = 16;   // FP64 partials per tile: pe, W[6], K[6], pad
@@ -138,7 +145,11 @@ struct RepDesc {
     int* cell_cnt;         // [ncell_cap]
     int* cell_start;       // [ncell_cap + 1]
     int* nnbr;             // [nslots]
-    int* nbr;              // [ntiles][kcap][64] slots: 16-bit if idx16, else 32-bit
+    int* nbr;              // [ntiles][kcap][64] entries: 16-bit if idx16, else 32-bit; the LDS byte
+                           // offset of the neighbor in its group's staged set, or its slot
+                           // when the group's gcount is -1
+    int* gcount;           // [ntiles / kGroupTiles] atoms of the group's neighbor set, -1: rows hold slots
+    int* gstage;           // [ntiles / kGroupTiles][kGroupStage] slots of the neighbor set (16-bit if idx16)
     int* bhdr;             // [nslots] nbonds | nangles << 8
     int* bnd;              // [ntiles][bcap][64] partner slot | type << 26
     int2* ang;             // [ntiles][acap][64] (slot | role << 26, slot | type << 26)

@@ -30,7 +30,11 @@ void launch_decide(const Launch& l);
 // `build`, the Verlet lists); `always` ignores the active/rebuild flags.
 void launch_rebuild(const Launch& l, int always, int build);
 // Force kernel over tiles [t0, t0+nt) of the list with the given block size.
-void launch_force(const Launch& l, int t0, int nt, int block, int terms, int kick, int always);
+// Force kernel modes: FM_STEP skips inactive replicas and records PE and
+// virial on the steps that want them; FM_OBS always runs and records them;
+// FM_FAST always runs and records forces only (a plain step's work).
+enum ForceMode { FM_STEP = 0, FM_OBS = 1, FM_FAST = 2 };
+void launch_force(const Launch& l, int t0, int nt, int block, int terms, int kick, int mode);
 void launch_boundary(const Launch& l, int mode);
 void launch_control(const Launch& l, int op, int64_t arg);
 // Radius of gyration of each of a replica's chains.

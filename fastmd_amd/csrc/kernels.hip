@@ -15,6 +15,8 @@
 //   boundary  per replica: FP64 reductions, NH chain / barostat, thermo record
 #include "kernels.h"
 
+#include <climits>
+
 namespace fmd {
 
 // ---------------------------------------------------------------------------
@@ -86,6 +88,13 @@ __device__ inline float sq3(float x, float y, float z) { return x * x + y * y + 
 
 // Word 3 of a raw buffer resource on gfx9 parts: 32-bit data format, no swizzle.
 constexpr int kBufferWord3 = 0x00020000;
+
+// Indexed ("structured") buffer load of one 16-byte record: the resource
+// carries the stride, the hardware forms index * stride (buffer_load_dwordx4
+// ... idxen), and an index at or past the record count reads zeros.
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+__device__ u32x4 struct_load_b128(__amdgpu_buffer_rsrc_t rsrc, unsigned vindex, unsigned voffset,
+                                  unsigned soffset, int aux) __asm("llvm.amdgcn.struct.ptr.buffer.load.v4i32");
 
 // ---------------------------------------------------------------------------
 // FP64 Nose-Hoover chain (src/integrate/nose_hoover.cu:86-178 / :356-447) and
@@ -598,17 +607,21 @@ __global__ void __launch_bounds__(256) k_gather(REBUILD_ARGS) {
 // Verlet list + bonded slot tables (src/neighbor/verlet_list.cu:108-335).
 // Full list, r^2 < (rc+skin)^2 on the reference FP32 distance, i != j, 1-2
 // bonded partners excluded; stored tile-major nbr[tile][k][lane] so the force
-// kernel streams one contiguous 256-byte row per neighbor index.  A count above
-// kcap is reported as ERR_NBR_OVERFLOW, never silently truncated.
+// kernel streams one contiguous row per neighbor index.  A count above kcap is
+// reported as ERR_NBR_OVERFLOW, never silently truncated.
 //
-// One wave per tile.  The tile's atoms lie in a few Morton-consecutive cells;
-// the union of those cells' 27-neighborhoods (typically 45-80 cells, about
-// 1000-1500 atoms) is staged in LDS with coalesced loads, and each lane scans
-// its own 27 cells from LDS in (dz, dy, dx) order, candidates in ascending slot
-// order, exactly as the global-memory scan that serves tiles whose union does
-// not fit.  Both produce the same rows.
+// One workgroup of kGroupTiles waves per group of tiles.  The group's atoms lie
+// in a few Morton-consecutive cells; the union of those cells' 27-neighborhoods
+// (typically 90-140 cells, about 2000-2900 atoms) is staged in LDS with
+// coalesced loads, and each lane scans its own 27 cells from LDS in
+// (dz, dy, dx) order, candidates in ascending slot order, exactly as the
+// global-memory scan that serves groups whose union does not fit.  Both give
+// the same rows.  The atoms the group's rows reference (its neighbor set, about
+// 1400) are then numbered in slot order and listed in gstage; the rows are
+// rewritten to those atoms' byte offsets in the force kernel's LDS copy of the
+// set (or to slots when the set exceeds kGroupStage).
 // ---------------------------------------------------------------------------
-constexpr int kMaxWin = 1024;      // cells of a tile's neighbor window (bitmap words <= 32)
+constexpr int kMaxWin = 1024;      // cells of a group's neighbor window (bitmap words <= 32)
 
 // set bits of m below this lane
 __device__ inline int lane_prefix(unsigned long long m) {
@@ -625,13 +638,14 @@ __device__ inline int wrap_cell(int c, int o, int nx) {
 
 // Scan of one chunk of kBuildChunk candidates (slots q0 + k, k < n): the
 // distance tests give a bit mask, then the set bits are appended in order,
-// skipping the atom itself and its bonded partners.  W16 rows hold 16-bit
-// slots (replicas of at most 65536 slots), other rows 32-bit slots.
+// skipping the atom itself and its bonded partners.  Entry k is stored as
+// ebase + k (a staged index or a slot) and, with `used`, marks used[ebase + k].
+// W16 rows hold 16-bit entries (replicas of at most 65536 slots), others 32-bit.
 template <bool W16>
 __device__ __forceinline__ void build_chunk(const float4 (&pj)[kBuildChunk], int q0, int n, int i, float4 pi,
                                             float L, float invL, float rcs2, const int (&ex)[4], int nb,
                                             int bcap, const int* bt, int* out, unsigned short* out16, int kcap,
-                                            int& cnt) {
+                                            int& cnt, int ebase, unsigned char* used) {
     unsigned m = 0;
 #pragma unroll
     for (int k = 0; k < kBuildChunk; k++) {
@@ -649,22 +663,55 @@ __device__ __forceinline__ void build_chunk(const float4 (&pj)[kBuildChunk], int
 #pragma unroll
     for (int k = 0; k < kBuildChunk; k++) {
         if (((m >> k) & 1u) && cnt < kcap) {
-            if (W16) out16[cnt * kTile] = (unsigned short)(q0 + k);
-            else out[cnt * kTile] = q0 + k;
+            if (W16) out16[cnt * kTile] = (unsigned short)(ebase + k);
+            else out[cnt * kTile] = ebase + k;
+            if (used) used[ebase + k] = 1;
         }
         cnt += (m >> k) & 1u;
     }
 }
 
-__global__ void __launch_bounds__(64) k_build(REBUILD_ARGS) {
-    __shared__ float s_px[kStageAtoms], s_py[kStageAtoms], s_pz[kStageAtoms];  // staged positions (SoA)
+// Block-wide exclusive prefix of one int per thread (kGroupSlots threads).
+__device__ inline int block_excl_scan(int x, int* s_wsum, int* total) {
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+    int v = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(v, o, 64);
+        if (lane >= o) v += y;
+    }
+    if (lane == 63) s_wsum[wv] = v;
+    __syncthreads();
+    int before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kGroupTiles; w++) {
+        const int ws = s_wsum[w];
+        before += (w < wv) ? ws : 0;
+        all += ws;
+    }
+    __syncthreads();
+    *total = all;
+    return before + v - x;
+}
+
+__global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
+    __shared__ float s_px[kBuildStage], s_py[kBuildStage], s_pz[kBuildStage];  // staged union (SoA)
+    __shared__ unsigned short s_seg[kBuildStage];  // staged atom -> union cell
+    __shared__ unsigned char s_used[kBuildStage];  // staged atom is in a row of the group
     __shared__ int4 s_cell[kMaxUnion];             // (first slot, count, LDS offset, key)
     __shared__ unsigned int s_bits[kMaxWin / 32];   // union cells of the window
     __shared__ unsigned char s_wmap[kMaxWin];       // window cell -> union index
-    __shared__ unsigned char s_seg[kStageAtoms];   // LDS atom -> union index
-    REBUILD_LOOP {
-        REBUILD_PROLOGUE
-        const int lane = lane_id();
+    __shared__ int s_red[6][kGroupTiles];           // per-wave window bounds
+    __shared__ int s_wsum[kGroupTiles];
+    __shared__ int s_c0[3], s_nu, s_nstage;
+    unsigned short* s_cidx = (unsigned short*)s_px;  // after the scan: compact index of a used atom
+    const int wv = threadIdx.x >> 6, lane = lane_id(), tid = threadIdx.x;
+    for (int gb = blockIdx.x, ng = *nrt / kGroupTiles; gb < ng; gb += gridDim.x) {
+        const Tile t = tiles[rlist[gb * kGroupTiles + wv]];
+        const RepDesc& d = descs[t.rep];
+        RepState* st = d.st;
+        const int i = t.lt * kTile + lane;
+        const int grp = t.lt / kGroupTiles;
         const bool valid = i < d.natoms;
         int err = 0;
         // bonded partners (slots), the exclusion set and the bonded tables
@@ -713,11 +760,14 @@ __global__ void __launch_bounds__(64) k_build(REBUILD_ARGS) {
         if (valid) {
             cx = cell_coord(pi.x, ics, nx); cy = cell_coord(pi.y, ics, nx); cz = cell_coord(pi.z, ics, nx);
         }
-        // The tile's cells lie in a small box of cell coordinates (taken relative
-        // to lane 0's cell, modulo the box).  Its 27-neighborhoods fall in that
-        // box grown by one cell per side (the whole axis if that wraps): a window
-        // of at most kMaxWin cells.  A bitmap over the window marks the union,
-        // whose cells are numbered in window order.
+        // The group's cells lie in a small box of cell coordinates (taken
+        // relative to the cell of the group's first atom, which is always a real
+        // atom, modulo the box).  Their 27-neighborhoods fall in that box grown
+        // by one cell per side (the whole axis if that wraps): a window of at
+        // most kMaxWin cells.  A bitmap over the window marks the union, whose
+        // cells are numbered in window order.
+        if (tid == 0) { s_c0[0] = cx; s_c0[1] = cy; s_c0[2] = cz; }
+        __syncthreads();
         const int h = nx >> 1;
         auto rel = [&](int c, int c0) {
             int r = c - c0;
@@ -725,17 +775,28 @@ __global__ void __launch_bounds__(64) k_build(REBUILD_ARGS) {
             r -= (r >= nx - h) ? nx : 0;
             return r;
         };
-        const int c0x = __shfl(cx, 0, 64), c0y = __shfl(cy, 0, 64), c0z = __shfl(cz, 0, 64);
+        const int c0x = s_c0[0], c0y = s_c0[1], c0z = s_c0[2];
         const int rx = valid ? rel(cx, c0x) : 0, ry = valid ? rel(cy, c0y) : 0, rz = valid ? rel(cz, c0z) : 0;
+        {
+            const int lo_x = wave_min_i(valid ? rx : INT_MAX), hi_x = wave_max_i(valid ? rx : INT_MIN);
+            const int lo_y = wave_min_i(valid ? ry : INT_MAX), hi_y = wave_max_i(valid ? ry : INT_MIN);
+            const int lo_z = wave_min_i(valid ? rz : INT_MAX), hi_z = wave_max_i(valid ? rz : INT_MIN);
+            if (lane == 0) {
+                s_red[0][wv] = lo_x; s_red[1][wv] = hi_x; s_red[2][wv] = lo_y;
+                s_red[3][wv] = hi_y; s_red[4][wv] = lo_z; s_red[5][wv] = hi_z;
+            }
+        }
+        __syncthreads();
         int sx, sy, sz, ex_, ey_, ez_;
-        auto window = [&](int r, int& s0, int& e0) {
-            const int lo = wave_min_i(r), hi = wave_max_i(r);
+        auto window = [&](int a, int& s0, int& e0) {
+            int lo = INT_MAX, hi = INT_MIN;
+            for (int w = 0; w < kGroupTiles; w++) { lo = min(lo, s_red[2 * a][w]); hi = max(hi, s_red[2 * a + 1][w]); }
             s0 = lo - 1;
             e0 = min(hi - lo + 3, nx);  // the whole axis once the grown box wraps
         };
-        window(rx, sx, ex_);
-        window(ry, sy, ey_);
-        window(rz, sz, ez_);
+        window(0, sx, ex_);
+        window(1, sy, ey_);
+        window(2, sz, ez_);
         // window cell of this atom's neighbor cell c in (dz, dy, dx) order; the
         // offset from the window start is >= 0 and, on an axis the window
         // wraps (extent nx), taken modulo nx
@@ -750,12 +811,12 @@ __global__ void __launch_bounds__(64) k_build(REBUILD_ARGS) {
             return (wz * ey_ + wy) * ex_ + wx;
         };
         const int W = ex_ * ey_ * ez_;
-        bool staged = W <= kMaxWin;
-        int nu = 0;
+        bool staged = W <= kMaxWin;  // uniform over the group
+        int nu = 0, nstage = 0;
         if (staged) {
-            for (int j = lane; j < kMaxWin / 32; j += kTile) s_bits[j] = 0u;
+            for (int j = tid; j < kMaxWin / 32; j += kGroupSlots) s_bits[j] = 0u;
             __syncthreads();
-            // each cell of the tile marks its 27 neighbors (its first lane does)
+            // each cell of the group marks its 27 neighbors (its first lane in a wave does)
             const int prev = __shfl_up(cx + nx * (cy + nx * cz), 1, 64);
             if (valid && (lane == 0 || prev != cx + nx * (cy + nx * cz)))
                 for (int c = 0; c < 27; c++) {
@@ -763,82 +824,88 @@ __global__ void __launch_bounds__(64) k_build(REBUILD_ARGS) {
                     atomicOr(&s_bits[w >> 5], 1u << (w & 31));
                 }
             __syncthreads();
-            // number the marked cells in window order
-            const int nwords = (W + 31) >> 5;  // <= kMaxWin / 32 <= 64 lanes
-            const unsigned bits = (lane < nwords) ? s_bits[lane] : 0u;
-            int x = __popc(bits);
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int y = __shfl_up(x, o, 64);
-                if (lane >= o) x += y;
-            }
-            nu = __shfl(x, 63, 64);
-            staged = nu <= kMaxUnion;
-            if (staged) {
-                // lane l numbers bit (l & 31) of word j + (l >> 5): two words per pass
-                const int base = x - __popc(bits);  // marked cells in words below this lane's word
-                auto wrapn = [&](int p) {           // p in [-nx, 2 nx)
-                    p += (p < 0) ? nx : 0;
-                    p -= (p >= nx) ? nx : 0;
-                    return p;
-                };
-                for (int j = 0; j < nwords; j += 2) {
-                    const int word = j + (lane >> 5), bit = lane & 31;
-                    const unsigned wb = __shfl(bits, word & 63, 64);
-                    const int wbase = __shfl(base, word & 63, 64);
-                    if (word < nwords && ((wb >> bit) & 1u)) {
-                        const int w = word * 32 + bit;
-                        const int u = wbase + __popc(wb & ((1u << bit) - 1u));
-                        const int wx = w % ex_, wy = (w / ex_) % ey_, wz = w / (ex_ * ey_);
-                        s_wmap[w] = (unsigned char)u;
-                        s_cell[u] = make_int4(0, 0, 0, (int)cell_key(wrapn(c0x + sx + wx), wrapn(c0y + sy + wy),
-                                                                     wrapn(c0z + sz + wz)));
-                    }
-                }
-            }
-            __syncthreads();
-        }
-        // counts, LDS offsets and the staged positions
-        int nstage = 0;
-        if (staged) {
-            for (int u0 = 0; u0 < nu; u0 += 64) {
-                const int u = u0 + lane;
-                int s0 = 0, c = 0;
-                if (u < nu) {
-                    const int k = s_cell[u].w;
-                    s0 = cstart[k];
-                    c = cstart[k + 1] - s0;
-                }
-                int x = c;
+            if (wv == 0) {
+                // number the marked cells in window order
+                const int nwords = (W + 31) >> 5;  // <= kMaxWin / 32 <= 64 lanes
+                const unsigned bits = (lane < nwords) ? s_bits[lane] : 0u;
+                int x = __popc(bits);
 #pragma unroll
                 for (int o = 1; o < 64; o <<= 1) {
                     const int y = __shfl_up(x, o, 64);
                     if (lane >= o) x += y;
                 }
-                const int off = nstage + x - c;
-                if (u < nu) s_cell[u] = make_int4(s0, c, off, s_cell[u].w);
-                nstage += __shfl(x, 63, 64);
-            }
-            staged = nstage <= kStageAtoms;
-        }
-        __syncthreads();
-        if (staged) {
-            for (int u = lane; u < nu; u += 64) {
-                const int4 ce = s_cell[u];
-                for (int j = 0; j < ce.y; j++) s_seg[ce.z + j] = (unsigned char)u;
+                const int nuw = __shfl(x, 63, 64);
+                if (lane == 0) s_nu = nuw;
+                if (nuw <= kMaxUnion) {
+                    // lane l numbers bit (l & 31) of word j + (l >> 5): two words per pass
+                    const int base = x - __popc(bits);  // marked cells in words below this lane's word
+                    auto wrapn = [&](int p) {           // p in [-nx, 2 nx)
+                        p += (p < 0) ? nx : 0;
+                        p -= (p >= nx) ? nx : 0;
+                        return p;
+                    };
+                    for (int j = 0; j < nwords; j += 2) {
+                        const int word = j + (lane >> 5), bit = lane & 31;
+                        const unsigned wb = __shfl(bits, word & 63, 64);
+                        const int wbase = __shfl(base, word & 63, 64);
+                        if (word < nwords && ((wb >> bit) & 1u)) {
+                            const int w = word * 32 + bit;
+                            const int u = wbase + __popc(wb & ((1u << bit) - 1u));
+                            const int wx = w % ex_, wy = (w / ex_) % ey_, wz = w / (ex_ * ey_);
+                            s_wmap[w] = (unsigned char)u;
+                            s_cell[u] = make_int4(0, 0, 0, (int)cell_key(wrapn(c0x + sx + wx),
+                                                                         wrapn(c0y + sy + wy),
+                                                                         wrapn(c0z + sz + wz)));
+                        }
+                    }
+                    // counts and LDS offsets of the union cells
+                    int ns = 0;
+                    for (int u0 = 0; u0 < nuw; u0 += 64) {
+                        const int u = u0 + lane;
+                        int s0 = 0, c = 0;
+                        if (u < nuw) {
+                            const int k = s_cell[u].w;
+                            s0 = cstart[k];
+                            c = cstart[k + 1] - s0;
+                        }
+                        int y = c;
+#pragma unroll
+                        for (int o = 1; o < 64; o <<= 1) {
+                            const int z = __shfl_up(y, o, 64);
+                            if (lane >= o) y += z;
+                        }
+                        if (u < nuw) s_cell[u] = make_int4(s0, c, ns + y - c, s_cell[u].w);
+                        ns += __shfl(y, 63, 64);
+                    }
+                    if (lane == 0) s_nstage = ns;
+                }
             }
             __syncthreads();
-            for (int q0 = 0; q0 < nstage; q0 += 8 * kTile) {
-                float4 v4[8];
+            nu = s_nu;
+            staged = nu <= kMaxUnion;
+            if (staged) {
+                nstage = s_nstage;
+                staged = nstage <= kBuildStage;
+            }
+        }
+        if (staged) {
+            for (int u = tid; u < nu; u += kGroupSlots) {
+                const int4 ce = s_cell[u];
+                for (int j = 0; j < ce.y; j++) s_seg[ce.z + j] = (unsigned short)u;
+            }
+            for (int q = tid; q < nstage; q += kGroupSlots) s_used[q] = 0;
+            __syncthreads();
+            for (int q0 = 0; q0 < nstage; q0 += 4 * kGroupSlots) {
+                float4 v4[4];
 #pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const int q = min(q0 + k * kTile + lane, nstage - 1);
+                for (int k = 0; k < 4; k++) {
+                    const int q = min(q0 + k * kGroupSlots + tid, nstage - 1);
                     const int4 ce = s_cell[s_seg[q]];
                     v4[k] = gpos[ce.x + q - ce.z];
                 }
 #pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const int q = q0 + k * kTile + lane;
+                for (int k = 0; k < 4; k++) {
+                    const int q = q0 + k * kGroupSlots + tid;
                     if (q < nstage) {
                         s_px[q] = v4[k].x;
                         s_py[q] = v4[k].y;
@@ -848,8 +915,12 @@ __global__ void __launch_bounds__(64) k_build(REBUILD_ARGS) {
             }
             __syncthreads();
         }
-        int cnt = 0;
+        int cnt = 0, qself = 0;
         if (valid && staged) {
+            {  // the atom's own staged index (its own cell is window cell win_of(13))
+                const int us = s_wmap[win_of(13)];
+                if (us < nu) qself = s_cell[us].z + (i - s_cell[us].x);
+            }
             for (int c = 0; c < 27; c++) {
                 const int w = win_of(c);
                 bool seen = false;
@@ -871,11 +942,15 @@ __global__ void __launch_bounds__(64) k_build(REBUILD_ARGS) {
                     }
                     if (d.idx16)
                         build_chunk<true>(pj, ce.x + q0, ce.y - q0, i, pi, L, invL, rcs2, ex, nb, d.bcap, bt, out,
-                                          out16, kcap, cnt);
+                                          out16, kcap, cnt, ce.z + q0, s_used);
                     else
                         build_chunk<false>(pj, ce.x + q0, ce.y - q0, i, pi, L, invL, rcs2, ex, nb, d.bcap, bt, out,
-                                           out16, kcap, cnt);
+                                           out16, kcap, cnt, ce.z + q0, s_used);
                 }
+            }
+            if (qself < 0 || qself >= nstage) {
+                err |= ERR_INTERNAL;
+                qself = 0;
             }
         } else if (valid) {
             // global-memory scan with the same visiting order
@@ -896,28 +971,70 @@ __global__ void __launch_bounds__(64) k_build(REBUILD_ARGS) {
                     for (int k2 = 0; k2 < kBuildChunk; k2++) pj[k2] = gpos[min(q0 + k2, e0 - 1)];
                     if (d.idx16)
                         build_chunk<true>(pj, q0, e0 - q0, i, pi, L, invL, rcs2, ex, nb, d.bcap, bt, out, out16,
-                                          kcap, cnt);
+                                          kcap, cnt, q0, nullptr);
                     else
                         build_chunk<false>(pj, q0, e0 - q0, i, pi, L, invL, rcs2, ex, nb, d.bcap, bt, out, out16,
-                                           kcap, cnt);
+                                           kcap, cnt, q0, nullptr);
                 }
             }
         }
+        const int stored = min(cnt, kcap);
+        int selfent = i;  // the padding entry: the atom itself (rejected by r^2 = 0)
+        if (staged) {
+            // the group's neighbor set: used staged atoms (and every atom itself),
+            // numbered in staged (slot) order
+            if (valid) s_used[qself] = 1;
+            __syncthreads();  // the scan's last reads of s_px precede the s_cidx writes below
+            const int per = (nstage + kGroupSlots - 1) / kGroupSlots;
+            const int q0 = min(tid * per, nstage), q1 = min(q0 + per, nstage);
+            int c = 0;
+            for (int q = q0; q < q1; q++) c += s_used[q];
+            int ncomp = 0;
+            int idx = block_excl_scan(c, s_wsum, &ncomp);
+            const bool compact = ncomp <= kGroupStage;
+            for (int q = q0; q < q1; q++)
+                if (s_used[q]) {
+                    s_cidx[q] = (unsigned short)idx;
+                    if (compact) {
+                        const int4 ce = s_cell[s_seg[q]];
+                        const int slot = ce.x + q - ce.z;
+                        if (d.idx16) ((unsigned short*)d.gstage)[(size_t)grp * kGroupStage + idx] = (unsigned short)slot;
+                        else d.gstage[(size_t)grp * kGroupStage + idx] = slot;
+                    }
+                    idx++;
+                }
+            if (tid == 0) d.gcount[grp] = compact ? ncomp : -1;
+            __syncthreads();
+            // rows: staged index -> LDS byte offset in the neighbor set, or slot
+            auto remap = [&](int e) {
+                if (compact) return (int)s_cidx[e] << 4;
+                const int4 ce = s_cell[s_seg[e]];
+                return ce.x + e - ce.z;
+            };
+            if (valid) {
+                for (int k = 0; k < stored; k++) {
+                    if (d.idx16) out16[k * kTile] = (unsigned short)remap(out16[k * kTile]);
+                    else out[k * kTile] = remap(out[k * kTile]);
+                }
+                selfent = remap(qself);
+            }
+        } else if (tid == 0) {
+            d.gcount[grp] = -1;
+        }
         if (valid) {
             if (cnt > kcap) err |= ERR_NBR_OVERFLOW;
-            const int stored = min(cnt, kcap);
-            // pad to a whole chunk with the atom itself (rejected by r^2 = 0), so
-            // the force kernel loads whole chunks without per-row guards
+            // pad to a whole chunk with the atom itself, so the force kernel
+            // loads whole chunks without per-row guards
             const int padded = ((stored + kLJChunk - 1) / kLJChunk) * kLJChunk;
             for (int k = stored; k < padded; k++) {
-                if (d.idx16) out16[k * kTile] = (unsigned short)i;
-                else out[k * kTile] = i;
+                if (d.idx16) out16[k * kTile] = (unsigned short)selfent;
+                else out[k * kTile] = selfent;
             }
             d.nnbr[i] = stored;
         }
         err = wave_or(err);
         if (err && lane == 0) atomicOr(&st->err, err);
-        __syncthreads();  // the next tile reuses the LDS
+        __syncthreads();  // the next group reuses the LDS
     }
 }
 
@@ -928,23 +1045,27 @@ __global__ void __launch_bounds__(64) k_build(REBUILD_ARGS) {
 // sigma^2 / u^2 and 24 eps / u^2 folded in, and the force and virial sums are
 // kept in grid units and scaled by u and u^2 once per atom.  The rows are
 // padded to whole chunks of kLJChunk with the atom itself (q = 0 is rejected),
-// so whole chunks are loaded unconditionally; the next chunk's indices are in
-// flight while the current chunk's positions are gathered (buffer loads: one
-// shift per address).  The cutoff masks 1/q, so there is no per-pair branch.
-// OBS adds the per-atom PE and virial (thermo steps and NPT), CAP the
-// push-off pair-force cap, ONE_TYPE a single (eps, sigma) pair.
+// so whole chunks are loaded unconditionally; the next chunk's entries are in
+// flight while the current chunk's positions are read.  The cutoff masks 1/q,
+// so there is no per-pair branch.  OBS adds the per-atom PE and virial (thermo
+// steps and NPT), CAP the push-off pair-force cap, ONE_TYPE a single
+// (eps, sigma) pair with eps and sigma nonzero.  With LDS the entries are byte
+// offsets into the group's neighbor set staged in LDS (sp); otherwise they are
+// slots, read with indexed buffer loads (the hardware scales the slot by the
+// 16-byte record).
 struct LJSum {
     float fx, fy, fz, pe, w0, w1, w2, w3, w4, w5;  // force, PE and virial of one atom's rows
 };
 
-template <bool OBS, bool CAP, bool ONE_TYPE, bool W16>
-__device__ __forceinline__ LJSum lj_loop(const RepDesc& d, int lt, int lane, int i, int4 pi, float L) {
+template <bool OBS, bool CAP, bool ONE_TYPE, bool W16, bool LDS>
+__device__ __forceinline__ LJSum lj_loop(const RepDesc& d, int lt, int lane, int i, int4 pi, float L,
+                                         const int4* sp) {
     const int nn = d.nnbr[i];
     const FMD_GLOBAL int* nl = gview(d.nbr) + (size_t)lt * d.kcap * kTile + lane;
     const FMD_GLOBAL unsigned short* nl16 =
         (const FMD_GLOBAL unsigned short*)(gview(d.nbr) + (size_t)lt * d.kcap * kTile) + lane;
     const __amdgpu_buffer_rsrc_t prs =
-        __builtin_amdgcn_make_buffer_rsrc((void*)d.ipos, 0, d.nslots * (int)sizeof(int4), kBufferWord3);
+        __builtin_amdgcn_make_buffer_rsrc((void*)d.ipos, (short)sizeof(int4), d.nslots, kBufferWord3);
     const float u = L * 2.3283064365386963e-10f;
     const GridCut gc = grid_cut(d.rc2, u);
     const float iu2 = (float)(1.0 / ((double)u * (double)u));
@@ -952,6 +1073,11 @@ __device__ __forceinline__ LJSum lj_loop(const RepDesc& d, int lt, int lane, int
     const float2 p0 = ljrow[0];
     const float s2g0 = p0.y * p0.y * iu2, c24g0 = 24.0f * p0.x * iu2, c2e0 = 2.0f * p0.x;
     const float capg = CAP ? d.fcap / u : 0.0f;  // fcap / r = (fcap / u) / sqrt(q)
+    // one (eps, sigma): force / r = K sr2 sr6 (2 sr6 - 1) with K = 24 eps / sigma^2,
+    // summed without K (applied once per atom); the +-1000 clamp becomes +-1000 / K
+    constexpr bool fold = ONE_TYPE && !CAP;  // ONE_TYPE only with eps, sigma != 0
+    const float kf = fold ? 24.0f * p0.x / (p0.y * p0.y) : 1.0f;
+    const float fclamp = fold ? 1000.0f / kf : 1000.0f;
     float gx = 0.f, gy = 0.f, gz = 0.f, pe = 0.f;
     float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
     unsigned nxt[kLJChunk];
@@ -963,8 +1089,13 @@ __device__ __forceinline__ LJSum lj_loop(const RepDesc& d, int lt, int lane, int
         uint4 pjs[kLJChunk];
 #pragma unroll
         for (int k = 0; k < kLJChunk; k++) {
-            const auto v = __builtin_amdgcn_raw_buffer_load_b128(prs, nxt[k] << 4, 0, 0);
-            pjs[k] = make_uint4(v[0], v[1], v[2], v[3]);
+            if constexpr (LDS) {  // the row holds the LDS byte offset of the staged neighbor
+                const int4 v = *(const int4*)((const char*)sp + nxt[k]);
+                pjs[k] = make_uint4((unsigned)v.x, (unsigned)v.y, (unsigned)v.z, (unsigned)v.w);
+            } else {
+                const u32x4 v = struct_load_b128(prs, nxt[k], 0, 0, 0);
+                pjs[k] = make_uint4(v[0], v[1], v[2], v[3]);
+            }
         }
         if (k0 + kLJChunk < nn) {
 #pragma unroll
@@ -987,8 +1118,10 @@ __device__ __forceinline__ LJSum lj_loop(const RepDesc& d, int lt, int lane, int
             const float sr2 = s2g * qinv;
             const float sr6 = sr2 * sr2 * sr2;
             const float sr12 = sr6 * sr6;
-            float fr = c24g * qinv * (2.0f * sr12 - sr6);  // force / r in real units
-            fr = fminf(fmaxf(fr, -1000.0f), 1000.0f);       // src/force/lj.cu:48
+            float fr;
+            if constexpr (fold) fr = (sr2 * sr6) * fmaf(sr6, 2.0f, -1.0f);  // force / r over K
+            else fr = c24g * qinv * (2.0f * sr12 - sr6);        // force / r in real units
+            fr = __builtin_amdgcn_fmed3f(fr, -fclamp, fclamp);  // src/force/lj.cu:48
             if (CAP) {
                 const float cap = capg * __builtin_amdgcn_rsqf(q);
                 fr = fminf(fmaxf(fr, -cap), cap);
@@ -1002,20 +1135,21 @@ __device__ __forceinline__ LJSum lj_loop(const RepDesc& d, int lt, int lane, int
             }
         }
     }
-    const float hu2 = 0.5f * u * u;
-    return LJSum{gx * u, gy * u, gz * u, pe, v0 * hu2, v1 * hu2, v2 * hu2, v3 * hu2, v4 * hu2, v5 * hu2};
+    const float fu = kf * u, hu2 = 0.5f * kf * u * u;
+    return LJSum{gx * fu, gy * fu, gz * fu, pe, v0 * hu2, v1 * hu2, v2 * hu2, v3 * hu2, v4 * hu2, v5 * hu2};
 }
 
-#define LJ_ARGS d, t.lt, lane, i, pi, L
-template <bool W16>
+#define LJ_ARGS d, t.lt, lane, i, pi, L, sp
+template <bool W16, bool LDS>
 __device__ __forceinline__ LJSum lj_variants(const RepDesc& d, const Tile& t, int lane, int i, int4 pi,
-                                             float L, bool obs) {
-    const bool cap = d.fcap > 0.0f, one = d.ntypes == 1;
+                                             float L, bool obs, const int4* sp) {
+    const bool cap = d.fcap > 0.0f, one = d.ntypes == 1 && d.lj[0].x != 0.0f && d.lj[0].y != 0.0f;
     if (!cap && one)
-        return obs ? lj_loop<true, false, true, W16>(LJ_ARGS) : lj_loop<false, false, true, W16>(LJ_ARGS);
+        return obs ? lj_loop<true, false, true, W16, LDS>(LJ_ARGS) : lj_loop<false, false, true, W16, LDS>(LJ_ARGS);
     if (!cap)
-        return obs ? lj_loop<true, false, false, W16>(LJ_ARGS) : lj_loop<false, false, false, W16>(LJ_ARGS);
-    return obs ? lj_loop<true, true, false, W16>(LJ_ARGS) : lj_loop<false, true, false, W16>(LJ_ARGS);
+        return obs ? lj_loop<true, false, false, W16, LDS>(LJ_ARGS)
+                   : lj_loop<false, false, false, W16, LDS>(LJ_ARGS);
+    return obs ? lj_loop<true, true, false, W16, LDS>(LJ_ARGS) : lj_loop<false, true, false, W16, LDS>(LJ_ARGS);
 }
 #undef LJ_ARGS
 
@@ -1026,7 +1160,7 @@ __device__ __forceinline__ LJSum lj_variants(const RepDesc& d, const Tile& t, in
 // to [0, n - 2], weight t = (r - (rmin + k dr)) / dr.  Half the pair energy and
 // half the pair virial go to this atom, as in the LJ loop.
 template <bool W16>
-__device__ LJSum table_loop(const RepDesc& d, int lt, int lane, int i, int4 pi, float L) {
+__device__ __forceinline__ LJSum table_loop(const RepDesc& d, int lt, int lane, int i, int4 pi, float L, const int4* sp) {
     const int nn = d.nnbr[i];
     const int* nl = d.nbr + (size_t)lt * d.kcap * kTile + lane;
     const unsigned short* nl16 = (const unsigned short*)(d.nbr + (size_t)lt * d.kcap * kTile) + lane;
@@ -1036,7 +1170,7 @@ __device__ LJSum table_loop(const RepDesc& d, int lt, int lane, int i, int4 pi, 
     LJSum s{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int k = 0; k < nn; k++) {
         const int j = W16 ? (int)nl16[k * kTile] : nl[k * kTile];
-        const int4 pj = d.ipos[j];
+        const int4 pj = sp ? sp[j >> 4] : d.ipos[j];  // LDS byte offset or slot
         const int tb = trow[pj.w];
         if (tb < 0) continue;
         const float ax = grid_d(pi.x, pj.x), ay = grid_d(pi.y, pj.y), az = grid_d(pi.z, pj.z);
@@ -1070,16 +1204,41 @@ __device__ LJSum table_loop(const RepDesc& d, int lt, int lane, int i, int4 pi, 
 template <int TERMS>
 __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict__ descs,
                                                const Tile* __restrict__ tiles, int t0, int ntl,
-                                               int kick, int always) {
+                                               int kick, int mode) {
     constexpr int terms = TERMS;
+    __shared__ int4 s_pos[(TERMS & 1) ? kGroupStage : 1];  // the group's neighbor set
+    // t0 and ntl are whole groups and a workgroup (1, 2 or 4 waves) lies in one
+    // group, so the returns below are uniform over the workgroup
     const int w = t0 + xcd_wave_in_grid();
     if (w >= t0 + ntl) return;
     const Tile t = tiles[w];
     const RepDesc& d = descs[t.rep];
     RepState* st = d.st;
-    if (!always && !st->active) return;
+    if (mode == FM_STEP && !st->active) return;
     const int lane = lane_id();
     const int i = t.lt * kTile + lane;
+    // stage the group's neighbor set (coalesced slot list, one 16-byte position each)
+    const int grp = t.lt / kGroupTiles;
+    const int nst = (terms & 1) ? d.gcount[grp] : -1;
+    if (nst >= 0) {
+        const int nth = blockDim.x, last = nst - 1;
+        const FMD_GLOBAL int4* gp = gview(d.ipos);
+        auto stage = [&](auto gs) {  // four slots, then four positions, in flight at once
+            for (int q = threadIdx.x; q < nst; q += 4 * nth) {
+                const int qb = q + nth, qc = q + 2 * nth, qd = q + 3 * nth;
+                const int sa = gs[q], sb = gs[min(qb, last)], sc = gs[min(qc, last)], sd = gs[min(qd, last)];
+                const int4 va = gp[sa], vb = gp[sb], vc = gp[sc], vd = gp[sd];
+                s_pos[q] = va;
+                if (qb < nst) s_pos[qb] = vb;
+                if (qc < nst) s_pos[qc] = vc;
+                if (qd < nst) s_pos[qd] = vd;
+            }
+        };
+        if (d.idx16) stage((const FMD_GLOBAL unsigned short*)gview(d.gstage) + (size_t)grp * kGroupStage);
+        else stage(gview(d.gstage) + (size_t)grp * kGroupStage);
+        __syncthreads();
+    }
+    const int4* sp = nst >= 0 ? s_pos : nullptr;
     float fx = 0.f, fy = 0.f, fz = 0.f, pe = 0.f;
     float w0 = 0.f, w1 = 0.f, w2 = 0.f, w3 = 0.f, w4 = 0.f, w5 = 0.f;
     double k0 = 0, k1 = 0, k2 = 0, k3 = 0, k4 = 0, k5 = 0;
@@ -1089,15 +1248,21 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
         const int ti = pi.w;
         const float L = st->Lp;
         const float ulen = L * 2.3283064365386963e-10f;  // fixed-point step, L * 2^-32
-        const bool obs = always || st->want_obs;
+        const bool obs = (mode == FM_OBS) || (mode == FM_STEP && st->want_obs);
         if (terms & 1) {
-            const LJSum a = d.idx16 ? lj_variants<true>(d, t, lane, i, pi, L, obs)
-                                    : lj_variants<false>(d, t, lane, i, pi, L, obs);
+            LJSum a;
+            if (sp) {
+                if (d.idx16) a = lj_variants<true, true>(d, t, lane, i, pi, L, obs, sp);
+                else a = lj_variants<false, true>(d, t, lane, i, pi, L, obs, sp);
+            } else {
+                if (d.idx16) a = lj_variants<true, false>(d, t, lane, i, pi, L, obs, sp);
+                else a = lj_variants<false, false>(d, t, lane, i, pi, L, obs, sp);
+            }
             fx = a.fx; fy = a.fy; fz = a.fz; pe = a.pe;
             w0 = a.w0; w1 = a.w1; w2 = a.w2; w3 = a.w3; w4 = a.w4; w5 = a.w5;
             if (d.ntables) {
-                const LJSum b = d.idx16 ? table_loop<true>(d, t.lt, lane, i, pi, L)
-                                        : table_loop<false>(d, t.lt, lane, i, pi, L);
+                const LJSum b = d.idx16 ? table_loop<true>(d, t.lt, lane, i, pi, L, sp)
+                                        : table_loop<false>(d, t.lt, lane, i, pi, L, sp);
                 fx += b.fx; fy += b.fy; fz += b.fz; pe += b.pe;
                 w0 += b.w0; w1 += b.w1; w2 += b.w2; w3 += b.w3; w4 += b.w4; w5 += b.w5;
             }
@@ -1449,7 +1614,7 @@ __global__ void k_morton_keys(const RepDesc* __restrict__ dp, int natoms, uint32
 // ---------------------------------------------------------------------------
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 static constexpr int kStrideBlocks = 4096;  // grid cap of the grid-stride rebuild kernels
-static constexpr int kBuildBlocks = 8192;   // grid cap of the one-wave list-build workgroups
+static constexpr int kBuildBlocks = 8192;   // grid cap of the list-build workgroups (one group each)
 
 void launch_pre(const Launch& l) {
     if (l.ntl <= 0) return;
@@ -1490,25 +1655,25 @@ void launch_rebuild(const Launch& l, int always, int build) {
     k_rank<<<g, 256, 0, l.stream>>>(l.descs, l.tiles, l.rlist, l.nrt);
     k_gather<<<g, 256, 0, l.stream>>>(l.descs, l.tiles, l.rlist, l.nrt);
     if (build) {
-        int gb = l.ntl;
+        int gb = cdiv(l.ntl, kGroupTiles);
         if (gb > kBuildBlocks) gb = kBuildBlocks;
-        k_build<<<gb, 64, 0, l.stream>>>(l.descs, l.tiles, l.rlist, l.nrt);
+        k_build<<<gb, kGroupSlots, 0, l.stream>>>(l.descs, l.tiles, l.rlist, l.nrt);
     }
 }
-void launch_force(const Launch& l, int t0, int nt, int block, int terms, int kick, int always) {
+void launch_force(const Launch& l, int t0, int nt, int block, int terms, int kick, int mode) {
     if (nt <= 0) return;
     block = valid_block(block);
     const int g = cdiv(nt, block / 64);
     const size_t lds = 0;
     switch (terms & 7) {
-        case 1: k_force<1><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
-        case 2: k_force<2><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
-        case 3: k_force<3><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
-        case 4: k_force<4><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
-        case 5: k_force<5><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
-        case 6: k_force<6><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
-        case 7: k_force<7><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
-        default: k_force<0><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, always); break;
+        case 1: k_force<1><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, mode); break;
+        case 2: k_force<2><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, mode); break;
+        case 3: k_force<3><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, mode); break;
+        case 4: k_force<4><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, mode); break;
+        case 5: k_force<5><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, mode); break;
+        case 6: k_force<6><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, mode); break;
+        case 7: k_force<7><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, mode); break;
+        default: k_force<0><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, mode); break;
     }
 }
 void launch_boundary(const Launch& l, int mode) {

@@ -251,12 +251,13 @@ int32_t fmd_drain_thermo(fmd_engine* e, int32_t sys, fmd_thermo* out, int32_t ca
 int64_t fmd_system_rebuilds(fmd_engine* e, int32_t sys);
 
 /* Times `reps` launches of the force kernel (terms as in fmd_compute_forces,
- * no kick) over the listed systems with HIP events on the engine's stream;
- * returns the average launch time and the kernel's algorithmic HBM bytes per
- * launch: 36 B per atom (position, count, force) + 4 B per listed neighbor
- * for LJ, + 4 B per atom header + 4 B per bond entry + 8 B per angle entry for
- * the bonded terms.  *pairs = listed neighbors.  Measurement only; no
- * counterpart in the reference. */
+ * no kick, forces only: the work of a step that records no thermo) over the
+ * listed systems with HIP events on the engine's stream; returns the average
+ * launch time and the kernel's algorithmic HBM bytes per launch: 36 B per atom
+ * (position, count, force) + 4 B per listed neighbor for LJ (whether the rows
+ * are stored in 16 or 32 bits), + 4 B per atom header + 4 B per bond entry +
+ * 8 B per angle entry for the bonded terms.  *pairs = listed neighbors.
+ * Measurement only; no counterpart in the reference. */
 int32_t fmd_time_force(fmd_engine* e, const int32_t* sys, int32_t nsys, uint32_t terms, int32_t reps,
                        double* avg_ms, double* alg_bytes, int64_t* pairs);
 
