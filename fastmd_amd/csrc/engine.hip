@@ -153,7 +153,7 @@ static Sizing sizing(const fmd_system_desc* d, int bcap, int acap, int kmult = 1
     b += (uint64_t)s.natoms * 4;
     b += (uint64_t)s.ncell_cap * 8 + 4;
     b += (uint64_t)s.ntiles * 64 * ((uint64_t)s.kcap * 4 + (uint64_t)s.bcap * 4 + (uint64_t)s.acap * 8);
-    b += (uint64_t)(s.ntiles / kGroupTiles) * (4 + (uint64_t)kGroupStage * 4) + 512;
+    b += (uint64_t)(s.ntiles / kGroupTiles) * (4 + (uint64_t)kGroupStage * 4) + (uint64_t)s.nslots + 768;
     b += (uint64_t)s.ntiles * (kNumPart * 8 + 4) + (uint64_t)((s.ntiles + kChunkTiles - 1) / kChunkTiles) * kNumPart * 8;
     b += (uint64_t)(d->natoms + 1) * 8 + (uint64_t)d->nbonds * 8 + (uint64_t)d->nangles * 48;
     b += sizeof(RepState) + (uint64_t)s.thermo_cap * sizeof(ThermoRec) + 4096;
@@ -391,6 +391,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     D.nbr = (int*)carve(nt * S.kcap * 64 * 4);
     D.gcount = (int*)carve(nt / kGroupTiles * 4);
     D.gstage = (int*)carve(nt / kGroupTiles * kGroupStage * 4);
+    D.gperm = (unsigned char*)carve(ns);
     D.bhdr = (int*)carve(ns * 4);
     D.bnd = (int*)carve(nt * S.bcap * 64 * 4);
     D.ang = (int2*)carve(nt * S.acap * 64 * 8);
@@ -697,8 +698,11 @@ void Engine::launch_step_seq(Batch& b) {
     launch_pre(l);
     launch_decide(l);
     launch_rebuild(l, 0, 1);
-    for (int c = 0; c < kNumClasses; c++)
-        launch_force(l, b.cls_t0[c], b.cls_nt[c], tuned_[0][c], FMD_TERM_ALL, 1, FM_STEP);
+    // one launch over every tile (the workgroup shape of the class holding most tiles)
+    int big = 0;
+    for (int c = 1; c < kNumClasses; c++)
+        if (b.cls_nt[c] > b.cls_nt[big]) big = c;
+    launch_force(l, 0, b.ntl, tuned_[0][big], FMD_TERM_ALL, 1, FM_STEP);
     launch_boundary(l, BND_STEP);
 }
 
@@ -904,9 +908,15 @@ int32_t Engine::get_pairs(int32_t sys, int64_t* npairs, int32_t* ij, int64_t cap
     rc = hip_check(hipMemcpy(gcount.data(), D.gcount, gcount.size() * 4, hipMemcpyDeviceToHost), "copy gcount");
     if (!rc) rc = hip_check(hipMemcpy(gstage.data(), D.gstage, gstage.size() * 4, hipMemcpyDeviceToHost), "copy gstage");
     if (rc) return rc;
+    std::vector<unsigned char> perm(D.nslots);
+    rc = hip_check(hipMemcpy(perm.data(), D.gperm, perm.size(), hipMemcpyDeviceToHost), "copy gperm");
+    if (rc) return rc;
     std::vector<std::pair<int, int>> pairs;
-    for (int i = 0; i < D.natoms; i++) {
-        const int lt = i / 64, lane = i % 64, g = lt / kGroupTiles;
+    for (int col = 0; col < D.nslots; col++) {
+        // the row in column col belongs to atom i of the column's group
+        const int lt = col / 64, lane = col % 64, g = lt / kGroupTiles;
+        const int i = g * kGroupSlots + perm[col];
+        if (i >= D.natoms) continue;
         const uint16_t* row16 = (const uint16_t*)(nbr.data() + (size_t)lt * D.kcap * 64);
         const uint16_t* gs16 = (const uint16_t*)gstage.data() + (size_t)g * kGroupStage;  // as on the device
         for (int k = 0; k < nn[i]; k++) {
@@ -1085,7 +1095,7 @@ template <class F>
 void for_each_ptr(RepDesc& d, F&& f) {
     f(d.pos); f(d.vel); f(d.force); f(d.pos_ref); f(d.ipos); f(d.img);
     f(d.tmp_pos); f(d.tmp_vel); f(d.tmp_img); f(d.cell_of); f(d.rank); f(d.order); f(d.order2);
-    f(d.slot_of); f(d.cell_cnt); f(d.cell_start); f(d.nnbr); f(d.nbr); f(d.gcount); f(d.gstage); f(d.bhdr); f(d.bnd); f(d.ang);
+    f(d.slot_of); f(d.cell_cnt); f(d.cell_start); f(d.nnbr); f(d.nbr); f(d.gcount); f(d.gstage); f(d.gperm); f(d.bhdr); f(d.bnd); f(d.ang);
     f(d.t_boff); f(d.t_blst); f(d.t_aoff); f(d.t_alst); f(d.t_choff);
     f(d.lj); f(d.bparm); f(d.aparm); f(d.mass);
     f(d.tab_idx); f(d.tab_parm); f(d.tab_no); f(d.tab_fe); f(d.corr);

@@ -150,6 +150,8 @@ struct RepDesc {
                            // when the group's gcount is -1
     int* gcount;           // [ntiles / kGroupTiles] atoms of the group's neighbor set, -1: rows hold slots
     int* gstage;           // [ntiles / kGroupTiles][kGroupStage] slots of the neighbor set (16-bit if idx16)
+    unsigned char* gperm;  // [nslots] atom (index in its group) whose row is in column (tile, lane);
+                           // a group's rows are placed by descending length
     int* bhdr;             // [nslots] nbonds | nangles << 8
     int* bnd;              // [ntiles][bcap][64] partner slot | type << 26
     int2* ang;             // [ntiles][acap][64] (slot | role << 26, slot | type << 26)

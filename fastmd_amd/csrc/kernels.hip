@@ -622,6 +622,7 @@ __global__ void __launch_bounds__(256) k_gather(REBUILD_ARGS) {
 // set (or to slots when the set exceeds kGroupStage).
 // ---------------------------------------------------------------------------
 constexpr int kMaxWin = 1024;      // cells of a group's neighbor window (bitmap words <= 32)
+constexpr int kPlaceBand = 16;     // row entries moved per pass of the placement
 
 // set bits of m below this lane
 __device__ inline int lane_prefix(unsigned long long m) {
@@ -636,11 +637,33 @@ __device__ inline int wrap_cell(int c, int o, int nx) {
     return c;
 }
 
-// Scan of one chunk of kBuildChunk candidates (slots q0 + k, k < n): the
-// distance tests give a bit mask, then the set bits are appended in order,
+// Appends the candidates of mask m (bit k: slot q0 + k) in slot order,
 // skipping the atom itself and its bonded partners.  Entry k is stored as
 // ebase + k (a staged index or a slot) and, with `used`, marks used[ebase + k].
 // W16 rows hold 16-bit entries (replicas of at most 65536 slots), others 32-bit.
+template <bool W16>
+__device__ __forceinline__ void append_chunk(unsigned m, int q0, int i, const int (&ex)[4], int nb, int bcap,
+                                             const int* bt, int* out, unsigned short* out16, int kcap, int& cnt,
+                                             int ebase, unsigned char* used) {
+    auto drop = [&](int e) {
+        const unsigned o = (unsigned)(e - q0);
+        m &= (o < (unsigned)kBuildChunk) ? ~(1u << o) : ~0u;
+    };
+    drop(i); drop(ex[0]); drop(ex[1]); drop(ex[2]); drop(ex[3]);
+    for (int b = 4; b < nb && b < bcap; b++) drop(bt[b * kTile] & kSlotMask);
+#pragma unroll
+    for (int k = 0; k < kBuildChunk; k++) {
+        if (((m >> k) & 1u) && cnt < kcap) {
+            if (W16) out16[cnt * kTile] = (unsigned short)(ebase + k);
+            else out[cnt * kTile] = ebase + k;
+            if (used) used[ebase + k] = 1;
+        }
+        cnt += (m >> k) & 1u;
+    }
+}
+
+// One chunk of kBuildChunk candidates (slots q0 + k, k < n) by the reference
+// FP32 distance: the tests give a bit mask, then the set bits are appended.
 template <bool W16>
 __device__ __forceinline__ void build_chunk(const float4 (&pj)[kBuildChunk], int q0, int n, int i, float4 pi,
                                             float L, float invL, float rcs2, const int (&ex)[4], int nb,
@@ -652,23 +675,35 @@ __device__ __forceinline__ void build_chunk(const float4 (&pj)[kBuildChunk], int
         const float r2 = dist2_ref(pi, pj[k], L, invL);  // every k: no per-candidate branch
         m |= ((k < n) & (r2 < rcs2)) ? (1u << k) : 0u;
     }
-    // the atom itself and its bonded partners, where they fall in this chunk
-    auto drop = [&](int e) {
-        const unsigned o = (unsigned)(e - q0);
-        m &= (o < (unsigned)kBuildChunk) ? ~(1u << o) : ~0u;
-    };
-    drop(i); drop(ex[0]); drop(ex[1]); drop(ex[2]); drop(ex[3]);
-    for (int b = 4; b < nb && b < bcap; b++) drop(bt[b * kTile] & kSlotMask);
-    // append in slot order
+    append_chunk<W16>(m, q0, i, ex, nb, bcap, bt, out, out16, kcap, cnt, ebase, used);
+}
+
+// The same decision with a fixed-point prefilter: q = |Xi - Xj|^2 in grid
+// units (exact integer differences, one rounding each) settles candidates
+// clearly inside (q < lo) or outside (q > hi) the list radius; the few in the
+// band between -- far wider than the difference between the two distance
+// forms -- are decided by the reference FP32 formula on their positions.
+template <bool W16>
+__device__ __forceinline__ void build_chunk_fix(const unsigned (&jx)[kBuildChunk], const unsigned (&jy)[kBuildChunk],
+                                                const unsigned (&jz)[kBuildChunk], int4 pf, float lo, float hi,
+                                                const FMD_GLOBAL float4* gpos, int q0, int n, int i, float4 pi,
+                                                float L, float invL, float rcs2, const int (&ex)[4], int nb,
+                                                int bcap, const int* bt, int* out, unsigned short* out16, int kcap,
+                                                int& cnt, int ebase, unsigned char* used) {
+    unsigned m = 0, band = 0;
 #pragma unroll
     for (int k = 0; k < kBuildChunk; k++) {
-        if (((m >> k) & 1u) && cnt < kcap) {
-            if (W16) out16[cnt * kTile] = (unsigned short)(ebase + k);
-            else out[cnt * kTile] = ebase + k;
-            if (used) used[ebase + k] = 1;
-        }
-        cnt += (m >> k) & 1u;
+        const float q = grid_q(grid_d(pf.x, jx[k]), grid_d(pf.y, jy[k]), grid_d(pf.z, jz[k]));
+        const bool ok = k < n;
+        m |= (ok & (q < lo)) ? (1u << k) : 0u;
+        band |= (ok & (q >= lo) & (q <= hi)) ? (1u << k) : 0u;
     }
+    if (band) {
+        for (int k = 0; k < kBuildChunk; k++)
+            if ((band >> k) & 1u)
+                m |= (dist2_ref(pi, gpos[q0 + k], L, invL) < rcs2) ? (1u << k) : 0u;
+    }
+    append_chunk<W16>(m, q0, i, ex, nb, bcap, bt, out, out16, kcap, cnt, ebase, used);
 }
 
 // Block-wide exclusive prefix of one int per thread (kGroupSlots threads).
@@ -695,7 +730,7 @@ __device__ inline int block_excl_scan(int x, int* s_wsum, int* total) {
 }
 
 __global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
-    __shared__ float s_px[kBuildStage], s_py[kBuildStage], s_pz[kBuildStage];  // staged union (SoA)
+    __shared__ unsigned s_px[kBuildStage], s_py[kBuildStage], s_pz[kBuildStage];  // staged union, fixed point
     __shared__ unsigned short s_seg[kBuildStage];  // staged atom -> union cell
     __shared__ unsigned char s_used[kBuildStage];  // staged atom is in a row of the group
     __shared__ int4 s_cell[kMaxUnion];             // (first slot, count, LDS offset, key)
@@ -704,6 +739,7 @@ __global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
     __shared__ int s_red[6][kGroupTiles];           // per-wave window bounds
     __shared__ int s_wsum[kGroupTiles];
     __shared__ int s_c0[3], s_nu, s_nstage;
+    __shared__ int s_len[kGroupSlots];              // padded row length of each atom of the group
     unsigned short* s_cidx = (unsigned short*)s_px;  // after the scan: compact index of a used atom
     const int wv = threadIdx.x >> 6, lane = lane_id(), tid = threadIdx.x;
     for (int gb = blockIdx.x, ng = *nrt / kGroupTiles; gb < ng; gb += gridDim.x) {
@@ -811,7 +847,18 @@ __global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
             return (wz * ey_ + wy) * ex_ + wx;
         };
         const int W = ex_ * ey_ * ez_;
-        bool staged = W <= kMaxWin;  // uniform over the group
+        // staged scan: prefilter in fixed point; boxes of fewer than 3 cells per
+        // axis (where a pair can sit near half the box) use the global scan
+        bool staged = W <= kMaxWin && nx >= 3;  // uniform over the group
+        const int4 pf = valid ? d.ipos[i] : make_int4(0, 0, 0, 0);
+        float blo, bhi;
+        {
+            const double u = (double)(L * 2.3283064365386963e-10f);
+            const double g = (double)rcs2 / (u * u);
+            const double band = 3e-5 + 1e-6 * (double)L / (double)d.rcs;  // >> the two forms' difference
+            blo = (float)(g * (1.0 - band));
+            bhi = (float)(g * (1.0 + band));
+        }
         int nu = 0, nstage = 0;
         if (staged) {
             for (int j = tid; j < kMaxWin / 32; j += kGroupSlots) s_bits[j] = 0u;
@@ -895,21 +942,22 @@ __global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
             }
             for (int q = tid; q < nstage; q += kGroupSlots) s_used[q] = 0;
             __syncthreads();
+            const FMD_GLOBAL int4* gip = gview(d.ipos);
             for (int q0 = 0; q0 < nstage; q0 += 4 * kGroupSlots) {
-                float4 v4[4];
+                int4 v4[4];
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     const int q = min(q0 + k * kGroupSlots + tid, nstage - 1);
                     const int4 ce = s_cell[s_seg[q]];
-                    v4[k] = gpos[ce.x + q - ce.z];
+                    v4[k] = gip[ce.x + q - ce.z];
                 }
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     const int q = q0 + k * kGroupSlots + tid;
                     if (q < nstage) {
-                        s_px[q] = v4[k].x;
-                        s_py[q] = v4[k].y;
-                        s_pz[q] = v4[k].z;
+                        s_px[q] = (unsigned)v4[k].x;
+                        s_py[q] = (unsigned)v4[k].y;
+                        s_pz[q] = (unsigned)v4[k].z;
                     }
                 }
             }
@@ -934,18 +982,18 @@ __global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
                 }
                 const int4 ce = s_cell[u];
                 for (int q0 = 0; q0 < ce.y; q0 += kBuildChunk) {
-                    float4 pj[kBuildChunk];
+                    unsigned jx[kBuildChunk], jy[kBuildChunk], jz[kBuildChunk];
 #pragma unroll
                     for (int k = 0; k < kBuildChunk; k++) {
                         const int q = ce.z + min(q0 + k, ce.y - 1);
-                        pj[k] = make_float4(s_px[q], s_py[q], s_pz[q], 0.0f);
+                        jx[k] = s_px[q]; jy[k] = s_py[q]; jz[k] = s_pz[q];
                     }
                     if (d.idx16)
-                        build_chunk<true>(pj, ce.x + q0, ce.y - q0, i, pi, L, invL, rcs2, ex, nb, d.bcap, bt, out,
-                                          out16, kcap, cnt, ce.z + q0, s_used);
+                        build_chunk_fix<true>(jx, jy, jz, pf, blo, bhi, gpos, ce.x + q0, ce.y - q0, i, pi, L, invL,
+                                              rcs2, ex, nb, d.bcap, bt, out, out16, kcap, cnt, ce.z + q0, s_used);
                     else
-                        build_chunk<false>(pj, ce.x + q0, ce.y - q0, i, pi, L, invL, rcs2, ex, nb, d.bcap, bt, out,
-                                           out16, kcap, cnt, ce.z + q0, s_used);
+                        build_chunk_fix<false>(jx, jy, jz, pf, blo, bhi, gpos, ce.x + q0, ce.y - q0, i, pi, L, invL,
+                                               rcs2, ex, nb, d.bcap, bt, out, out16, kcap, cnt, ce.z + q0, s_used);
                 }
             }
             if (qself < 0 || qself >= nstage) {
@@ -979,7 +1027,9 @@ __global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
             }
         }
         const int stored = min(cnt, kcap);
+        const int padded = valid ? ((stored + kLJChunk - 1) / kLJChunk) * kLJChunk : 0;
         int selfent = i;  // the padding entry: the atom itself (rejected by r^2 = 0)
+        bool compact = false;
         if (staged) {
             // the group's neighbor set: used staged atoms (and every atom itself),
             // numbered in staged (slot) order
@@ -991,7 +1041,7 @@ __global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
             for (int q = q0; q < q1; q++) c += s_used[q];
             int ncomp = 0;
             int idx = block_excl_scan(c, s_wsum, &ncomp);
-            const bool compact = ncomp <= kGroupStage;
+            compact = ncomp <= kGroupStage;
             for (int q = q0; q < q1; q++)
                 if (s_used[q]) {
                     s_cidx[q] = (unsigned short)idx;
@@ -1004,32 +1054,53 @@ __global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
                     idx++;
                 }
             if (tid == 0) d.gcount[grp] = compact ? ncomp : -1;
-            __syncthreads();
-            // rows: staged index -> LDS byte offset in the neighbor set, or slot
-            auto remap = [&](int e) {
-                if (compact) return (int)s_cidx[e] << 4;
-                const int4 ce = s_cell[s_seg[e]];
-                return ce.x + e - ce.z;
-            };
-            if (valid) {
-                for (int k = 0; k < stored; k++) {
-                    if (d.idx16) out16[k * kTile] = (unsigned short)remap(out16[k * kTile]);
-                    else out[k * kTile] = remap(out[k * kTile]);
-                }
-                selfent = remap(qself);
-            }
         } else if (tid == 0) {
             d.gcount[grp] = -1;
         }
+        // Placement: the group's rows go to columns in descending order of
+        // their padded length (ties by slot), so the 64 rows a wave streams
+        // are of nearly equal length; gperm[column] names the atom.  Rows are
+        // moved a band of kPlaceBand entries at a time (every read of a band
+        // precedes every write), mapping staged indices to LDS byte offsets
+        // (or to slots when the neighbor set did not fit).
+        s_len[tid] = padded;
+        __syncthreads();
+        if (valid && compact) selfent = (int)s_cidx[qself] << 4;
+        int rank = 0, maxpad = 0;
+        for (int b = 0; b < kGroupSlots; b++) {
+            const int lb = s_len[b];
+            rank += (lb > padded || (lb == padded && b < tid)) ? 1 : 0;
+            maxpad = max(maxpad, lb);
+        }
+        d.gperm[(size_t)grp * kGroupSlots + rank] = (unsigned char)tid;
+        auto place = [&](auto* src, auto* dst) {
+            for (int k0 = 0; k0 < maxpad; k0 += kPlaceBand) {
+                int v[kPlaceBand];
+#pragma unroll
+                for (int e = 0; e < kPlaceBand; e++) {
+                    const int k = k0 + e;
+                    int x = selfent;
+                    if (k < stored) {
+                        const int q = (int)src[k * kTile];
+                        if (!staged) x = q;
+                        else if (compact) x = (int)s_cidx[q] << 4;
+                        else x = s_cell[s_seg[q]].x + q - s_cell[s_seg[q]].z;
+                    }
+                    v[e] = x;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int e = 0; e < kPlaceBand; e++)
+                    if (k0 + e < padded) dst[(k0 + e) * kTile] = v[e];
+                __syncthreads();
+            }
+        };
+        // destination column: the same addressing as out / out16 (tile base in int units)
+        int* dtile = d.nbr + (size_t)(grp * kGroupTiles + rank / kTile) * d.kcap * kTile;
+        if (d.idx16) place(out16, (unsigned short*)dtile + (rank % kTile));
+        else place(out, dtile + (rank % kTile));
         if (valid) {
             if (cnt > kcap) err |= ERR_NBR_OVERFLOW;
-            // pad to a whole chunk with the atom itself, so the force kernel
-            // loads whole chunks without per-row guards
-            const int padded = ((stored + kLJChunk - 1) / kLJChunk) * kLJChunk;
-            for (int k = stored; k < padded; k++) {
-                if (d.idx16) out16[k * kTile] = (unsigned short)selfent;
-                else out[k * kTile] = selfent;
-            }
             d.nnbr[i] = stored;
         }
         err = wave_or(err);
@@ -1216,9 +1287,10 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
     RepState* st = d.st;
     if (mode == FM_STEP && !st->active) return;
     const int lane = lane_id();
-    const int i = t.lt * kTile + lane;
-    // stage the group's neighbor set (coalesced slot list, one 16-byte position each)
+    // this lane computes the atom whose row the list build placed in its column
     const int grp = t.lt / kGroupTiles;
+    const int i = grp * kGroupSlots + d.gperm[t.lt * kTile + lane];
+    // stage the group's neighbor set (coalesced slot list, one 16-byte position each)
     const int nst = (terms & 1) ? d.gcount[grp] : -1;
     if (nst >= 0) {
         const int nth = blockDim.x, last = nst - 1;
@@ -1270,7 +1342,7 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
         const int hdr = (terms & 6) ? d.bhdr[i] : 0;
         if (terms & 2) {
             const int nb = hdr & 0xFF;
-            const int* bt = d.bnd + (size_t)t.lt * d.bcap * kTile + lane;
+            const int* bt = d.bnd + (size_t)(i / kTile) * d.bcap * kTile + (i % kTile);
             for (int b = 0; b < nb; b++) {
                 const int e = bt[b * kTile];
                 const int4 pj = d.ipos[e & kSlotMask];
@@ -1304,7 +1376,7 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
         }
         if (terms & 4) {
             const int na = (hdr >> 8) & 0xFF;
-            const int2* at = d.ang + (size_t)t.lt * d.acap * kTile + lane;
+            const int2* at = d.ang + (size_t)(i / kTile) * d.acap * kTile + (i % kTile);
             for (int a = 0; a < na; a++) {
                 const int2 e = at[a * kTile];
                 const int role = (unsigned)e.x >> kSlotBits;
