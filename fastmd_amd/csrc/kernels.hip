@@ -651,14 +651,17 @@ __device__ __forceinline__ void append_chunk(unsigned m, int q0, int i, const in
     };
     drop(i); drop(ex[0]); drop(ex[1]); drop(ex[2]); drop(ex[3]);
     for (int b = 4; b < nb && b < bcap; b++) drop(bt[b * kTile] & kSlotMask);
-#pragma unroll
-    for (int k = 0; k < kBuildChunk; k++) {
-        if (((m >> k) & 1u) && cnt < kcap) {
+    // one pass per set bit, lowest first: a wave runs as many passes as its
+    // fullest lane has pairs in the chunk (about 4 of 8)
+    while (m) {
+        const int k = __builtin_ctz(m);
+        m &= m - 1u;
+        if (cnt < kcap) {
             if (W16) out16[cnt * kTile] = (unsigned short)(ebase + k);
             else out[cnt * kTile] = ebase + k;
             if (used) used[ebase + k] = 1;
         }
-        cnt += (m >> k) & 1u;
+        cnt++;
     }
 }
 
@@ -694,10 +697,12 @@ __device__ __forceinline__ void build_chunk_fix(const unsigned (&jx)[kBuildChunk
 #pragma unroll
     for (int k = 0; k < kBuildChunk; k++) {
         const float q = grid_q(grid_d(pf.x, jx[k]), grid_d(pf.y, jy[k]), grid_d(pf.z, jz[k]));
-        const bool ok = k < n;
-        m |= (ok & (q < lo)) ? (1u << k) : 0u;
-        band |= (ok & (q >= lo) & (q <= hi)) ? (1u << k) : 0u;
+        m |= (q < lo) ? (1u << k) : 0u;
+        band |= (q <= hi) ? (1u << k) : 0u;
     }
+    const unsigned valid = n >= kBuildChunk ? (1u << kBuildChunk) - 1u : (1u << n) - 1u;  // k < n
+    m &= valid;
+    band &= valid & ~m;
     if (band) {
         for (int k = 0; k < kBuildChunk; k++)
             if ((band >> k) & 1u)
@@ -730,7 +735,9 @@ __device__ inline int block_excl_scan(int x, int* s_wsum, int* total) {
 }
 
 __global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
-    __shared__ unsigned s_px[kBuildStage], s_py[kBuildStage], s_pz[kBuildStage];  // staged union, fixed point
+    // staged union, fixed point (one chunk of padding: a cell's last chunk reads whole)
+    __shared__ unsigned s_px[kBuildStage + kBuildChunk], s_py[kBuildStage + kBuildChunk],
+        s_pz[kBuildStage + kBuildChunk];
     __shared__ unsigned short s_seg[kBuildStage];  // staged atom -> union cell
     __shared__ unsigned char s_used[kBuildStage];  // staged atom is in a row of the group
     __shared__ int4 s_cell[kMaxUnion];             // (first slot, count, LDS offset, key)
@@ -982,11 +989,15 @@ __global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
                 }
                 const int4 ce = s_cell[u];
                 for (int q0 = 0; q0 < ce.y; q0 += kBuildChunk) {
+                    // a whole chunk from LDS (immediate offsets; entries past the
+                    // cell are masked out, and the arrays are padded by a chunk)
                     unsigned jx[kBuildChunk], jy[kBuildChunk], jz[kBuildChunk];
+                    const unsigned* px = s_px + ce.z + q0;
+                    const unsigned* py = s_py + ce.z + q0;
+                    const unsigned* pz = s_pz + ce.z + q0;
 #pragma unroll
                     for (int k = 0; k < kBuildChunk; k++) {
-                        const int q = ce.z + min(q0 + k, ce.y - 1);
-                        jx[k] = s_px[q]; jy[k] = s_py[q]; jz[k] = s_pz[q];
+                        jx[k] = px[k]; jy[k] = py[k]; jz[k] = pz[k];
                     }
                     if (d.idx16)
                         build_chunk_fix<true>(jx, jy, jz, pf, blo, bhi, gpos, ce.x + q0, ce.y - q0, i, pi, L, invL,
