@@ -27,7 +27,7 @@ static inline int ceil_slots(int n) { return (n + kGroupSlots - 1) / kGroupSlots
 Engine::Engine(int device, uint64_t budget) : device_(device) {
     for (int c = 0; c < kNumClasses; c++) {
         tuned_[0][c] = 256;
-        tuned_[1][c] = 64;  // the list build runs one-wave workgroups (LDS staging per tile)
+        tuned_[1][c] = kGroupSlots;  // the list build runs one workgroup per group of tiles
     }
     if (hip_check(hipSetDevice(device), "hipSetDevice")) return;
     if (hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate")) return;
@@ -1345,8 +1345,9 @@ int32_t Engine::time_force(const int32_t* ids, int32_t n, uint32_t terms, int re
 
 // ---------------------------------------------------------------------------
 // autotuner: per size class, time the force kernel over the class's tiles for
-// each candidate block shape and keep the fastest.  The list build has one
-// shape (one wave per tile, its LDS staging sized for that).
+// each candidate block shape and keep the fastest (a workgroup of 1, 2 or 4
+// waves lies inside one group; smaller ones stage the group's set more often).
+// The list build has one shape: one workgroup per group of tiles.
 // ---------------------------------------------------------------------------
 int32_t Engine::autotune(const int32_t* ids, int32_t n, int reps) {
     if (!ids || n <= 0) return fail(FMD_ERR_ARG, "empty system list");
@@ -1392,7 +1393,9 @@ int32_t Engine::set_tuned(int kernel, int cls, int block) {
         return fail(FMD_ERR_ARG, "kernel must be 0 or 1 and the size class in [0, 3)");
     if (block != 64 && block != 128 && block != 256)
         return fail(FMD_ERR_ARG, "block shape must be 64, 128 or 256");
-    if (kernel == 1 && block != 64) return fail(FMD_ERR_ARG, "the list build runs 64-thread workgroups");
+    if (kernel == 1 && block != kGroupSlots)
+        return fail(FMD_ERR_ARG, "the list build runs one workgroup of " + std::to_string(kGroupSlots) +
+                                     " threads per group of tiles");
     tuned_[kernel][cls] = block;
     tune_version_++;
     return FMD_OK;

@@ -262,8 +262,9 @@ int32_t fmd_time_force(fmd_engine* e, const int32_t* sys, int32_t nsys, uint32_t
                        double* avg_ms, double* alg_bytes, int64_t* pairs);
 
 /* ----- autotuner -------------------------------------------------------- */
-/* Times the candidate block shapes of the force and list-build kernels for
- * each replica size class on the given systems and keeps the fastest. */
+/* Times the candidate block shapes (64, 128, 256 threads) of the force kernel
+ * for each replica size class on the given systems and keeps the fastest; the
+ * list build has one shape (256 threads, one group of four tiles). */
 int32_t fmd_autotune(fmd_engine* e, const int32_t* sys, int32_t nsys, int32_t reps);
 /* Chosen block size for (kernel 0 = force, 1 = list build, size class). */
 int32_t fmd_tuned_block(const fmd_engine* e, int32_t kernel, int32_t size_class);
