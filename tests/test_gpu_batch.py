@@ -66,6 +66,11 @@ def test_server_results_equal_alone(fmd, trio):
     lens = [120, 45, 200]
     alone = [_run_alone(fmd, d, n)[0] for d, n in zip(trio, lens)]
     srv = fmd.Server(0, graph_steps=20)
+    # block shapes as an autotuned engine reports them: any force shape gives
+    # the same results; the list build has one shape
+    srv.set_blocks({(0, 0): 64, (0, 1): 128, (0, 2): 256, (1, 0): 256, (1, 1): 256, (1, 2): 256})
+    with pytest.raises(fmd.FmdError):
+        srv.set_blocks({(1, 0): 64})
     jobs = [srv.submit(d, n) for d, n in zip(trio, lens)]
     srv.drain()
     for j, st, n in zip(jobs, alone, lens):
