@@ -39,6 +39,10 @@ __device__ inline int xcd_wave_in_grid() {
     return __builtin_amdgcn_readfirstlane(xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6));
 }
 __device__ inline int lane_id() { return threadIdx.x & 63; }
+// set bits of m below this lane
+__device__ inline int lane_prefix(unsigned long long m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
 
 __device__ inline float wave_max(float x) {
 #pragma unroll
@@ -484,17 +488,31 @@ __global__ void __launch_bounds__(1024) k_rebuild_list(const RepDesc* __restrict
     }
 }
 
+// Cell key of each atom and a unique position inside its cell.  The lanes of
+// a wave (Morton-sorted atoms) fall in a few cells: one atomic per distinct
+// key and wave reserves the positions, handed out in lane order.
 __global__ void __launch_bounds__(256) k_bin(REBUILD_ARGS) {
     REBUILD_LOOP {
         REBUILD_PROLOGUE
-        if (i < d.natoms) {
+        const bool valid = i < d.natoms;
+        uint32_t key = 0xffffffffu;
+        if (valid) {
             const float4 r = d.pos[i];
             const int nx = st->nx;
             const float ics = st->inv_cs;
-            const uint32_t key = cell_key(cell_coord(r.x, ics, nx), cell_coord(r.y, ics, nx),
-                                          cell_coord(r.z, ics, nx));
+            key = cell_key(cell_coord(r.x, ics, nx), cell_coord(r.y, ics, nx), cell_coord(r.z, ics, nx));
             d.cell_of[i] = (int)key;
-            d.rank[i] = atomicAdd(&d.cell_cnt[key], 1);
+        }
+        unsigned long long todo = __ballot(valid);
+        while (todo) {  // wave-uniform: one pass per distinct key
+            const int lead = __builtin_ctzll(todo);
+            const uint32_t k = __shfl(key, lead, 64);
+            const unsigned long long same = __ballot(valid && key == k);
+            int base = 0;
+            if (lane_id() == lead) base = atomicAdd(&d.cell_cnt[k], __popcll(same));
+            base = __shfl(base, lead, 64);
+            if (valid && key == k) d.rank[i] = base + lane_prefix(same);
+            todo &= ~same;
         }
     }
 }
@@ -623,11 +641,6 @@ __global__ void __launch_bounds__(256) k_gather(REBUILD_ARGS) {
 // ---------------------------------------------------------------------------
 constexpr int kMaxWin = 1024;      // cells of a group's neighbor window (bitmap words <= 32)
 constexpr int kPlaceBand = 16;     // row entries moved per pass of the placement
-
-// set bits of m below this lane
-__device__ inline int lane_prefix(unsigned long long m) {
-    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-}
 
 // cell coordinate offset by o in -1..1 with periodic wrap
 __device__ inline int wrap_cell(int c, int o, int nx) {
