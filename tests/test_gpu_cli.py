@@ -89,3 +89,55 @@ restart 200 restart.data
     for name in ("restart_200.data", "restart_final.data"):
         t2 = fmd.Topo.from_data(str(tmp_path / name))
         assert t2.natoms == 2000 and len(t2.arrays()["bonds"]) == 40 * 49
+
+
+def test_cli_table_potential_matches_engine(fmd, engine, orc, tmp_path):
+    """run.conf with a `table` pair (src/io/config.cpp:136-162) and a
+    forces_dump_file: the CLI's initial forces dump (src/main.cu:344-362)
+    equals the FP64 oracle with the same table."""
+    from test_oracle_pins import lj_table_rows
+    d = relax_kg(fmd, engine, 10, 40, seed=43)
+    sid = engine.add(d)
+    p4, v, im = engine.state(sid)
+    L = engine.box(sid)
+    engine.remove(sid)
+    t = fmd.Topo.kg_melt(10, 40, 0.85, 43)
+    data = tmp_path / "melt.data"
+    t.write_data(str(data), pos4=p4, vel=v, image=im, L=L, step=0)
+    rows = lj_table_rows(n=4000, rlo=0.5)
+    with open(tmp_path / "lj.table", "w") as f:
+        f.write("# LJ 12-6 with the force/r clamp\nLJ_TAB\nN %d R %.9g %.9g\n\n" % (len(rows), rows[0, 0], rows[-1, 0]))
+        for k, (r, e, fr) in enumerate(rows):
+            f.write("%d %.9g %.9g %.9g\n" % (k + 1, r, e, fr))
+    conf = f"""natoms 400
+ntypes 1
+rc 2.5
+skin 0.3
+dt 0.001
+nvt_langevin 1.0 1.0 1.0 42
+nsteps 50
+table 0 0 {tmp_path / "lj.table"} LJ_TAB
+bond_type 0 30.0 1.5 1.0 1.0
+angle_type 0 5.0 180.0
+lammps_data_file {data}
+forces_dump_file forces.txt
+"""
+    (tmp_path / "run.conf").write_text(conf)
+    r = subprocess.run([BIN, "run.conf"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "Initial forces written to forces.txt" in r.stdout
+    dump = np.loadtxt(tmp_path / "forces.txt", comments="#")
+    # the system as the CLI parsed it (positions from the data file, global-id order)
+    tc = fmd.Topo.from_conf(str(tmp_path / "run.conf"))
+    a = tc.arrays()
+    assert a["table_n"].tolist() == [len(rows)] and np.array_equal(a["table_data"], rows)
+    rng = a["table_range"][0]
+    f8, _ = orc.forces_f64(a["pos4"], np.float32(tc.c.L), np.zeros(2, np.float32), bonds=a["bonds"],
+                           btypes=a["bond_types"], bp=a["bond_params"], ang4=a["angles"], ap=a["angle_params"],
+                           terms=7, tables=([0], [(float(rng[0]), float(rng[1]), a["table_data"])]))
+    ids = dump[:, 0].astype(int)
+    floor = np.sqrt(np.mean(np.sum(f8[:, :3] ** 2, axis=1)))
+    got = dump[:, 1:4]
+    ref = f8[ids, :3]
+    err = np.max(np.linalg.norm(got - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), floor))
+    assert err < 1e-5, err
