@@ -80,6 +80,10 @@ __device__ inline int wave_or(int x) {
 template <class T> __device__ inline const FMD_GLOBAL T* gview(const T* p) {
     return (const FMD_GLOBAL T*)p;
 }
+// The same for stores.  A flat store also counts against the LDS wait
+// counter, so a flat store ahead of LDS reads makes their wait include the
+// store's trip to memory; a global store does not.
+template <class T> __device__ inline FMD_GLOBAL T* gmut(T* p) { return (FMD_GLOBAL T*)p; }
 
 constexpr int kMaxBlockK = 256;  // launch bound of k_build and k_force
 constexpr int kLJChunk = 8;      // neighbors per software-pipelined chunk (kcap % 8 == 0)
@@ -656,8 +660,9 @@ __device__ inline int wrap_cell(int c, int o, int nx) {
 // W16 rows hold 16-bit entries (replicas of at most 65536 slots), others 32-bit.
 template <bool W16>
 __device__ __forceinline__ void append_chunk(unsigned m, int q0, int i, const int (&ex)[4], int nb, int bcap,
-                                             const int* bt, int* out, unsigned short* out16, int kcap, int& cnt,
-                                             int ebase, unsigned char* used) {
+                                             const FMD_GLOBAL int* bt, FMD_GLOBAL int* out,
+                                             FMD_GLOBAL unsigned short* out16, int kcap, int& cnt, int ebase,
+                                             unsigned char* used) {
     auto drop = [&](int e) {
         const unsigned o = (unsigned)(e - q0);
         m &= (o < (unsigned)kBuildChunk) ? ~(1u << o) : ~0u;
@@ -683,8 +688,9 @@ __device__ __forceinline__ void append_chunk(unsigned m, int q0, int i, const in
 template <bool W16>
 __device__ __forceinline__ void build_chunk(const float4 (&pj)[kBuildChunk], int q0, int n, int i, float4 pi,
                                             float L, float invL, float rcs2, const int (&ex)[4], int nb,
-                                            int bcap, const int* bt, int* out, unsigned short* out16, int kcap,
-                                            int& cnt, int ebase, unsigned char* used) {
+                                            int bcap, const FMD_GLOBAL int* bt, FMD_GLOBAL int* out,
+                                            FMD_GLOBAL unsigned short* out16, int kcap, int& cnt, int ebase,
+                                            unsigned char* used) {
     unsigned m = 0;
 #pragma unroll
     for (int k = 0; k < kBuildChunk; k++) {
@@ -694,24 +700,45 @@ __device__ __forceinline__ void build_chunk(const float4 (&pj)[kBuildChunk], int
     append_chunk<W16>(m, q0, i, ex, nb, bcap, bt, out, out16, kcap, cnt, ebase, used);
 }
 
-// The same decision with a fixed-point prefilter: q = |Xi - Xj|^2 in grid
-// units (exact integer differences, one rounding each) settles candidates
-// clearly inside (q < lo) or outside (q > hi) the list radius; the few in the
-// band between -- far wider than the difference between the two distance
-// forms -- are decided by the reference FP32 formula on their positions.
+// The same decision with a coarse fixed-point prefilter.  The staged
+// positions keep the top 16 bits of each grid coordinate, x and y packed in
+// one word and z in the low half of another, so one packed 16-bit subtract per
+// word gives the minimum-image differences (modulo 2^16, as the full grid
+// wraps modulo 2^32) and two 16-bit dot products give their squared length
+// qc.  Truncation moves each coordinate by less than one coarse unit, so
+// |sqrt(qc) - r| < sqrt(3) units: qc below clo is certainly inside the list
+// radius and qc above chi certainly outside (both bounds also allow for the
+// band the exact FP32 test may differ by).  The few candidates in between --
+// about 1e-3 sigma either side of the radius -- are decided by the reference
+// FP32 formula on their positions.
+typedef short fmd_s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short fmd_u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ fmd_s16x2 coarse_diff(unsigned a, unsigned b) {  // two 16-bit lanes, modulo 2^16
+    return __builtin_bit_cast(fmd_s16x2, __builtin_bit_cast(fmd_u16x2, a) - __builtin_bit_cast(fmd_u16x2, b));
+}
+__device__ __forceinline__ int coarse_q(unsigned axy, unsigned az, unsigned bxy, unsigned bz) {
+    const fmd_s16x2 dxy = coarse_diff(axy, bxy), dz = coarse_diff(az, bz);
+    // saturating accumulate: a sum past INT_MAX is far outside any radius
+    return __builtin_amdgcn_sdot2(dz, dz, __builtin_amdgcn_sdot2(dxy, dxy, 0, true), true);
+}
+__device__ __forceinline__ unsigned coarse_xy(int4 p) {
+    return ((unsigned)p.x >> 16) | ((unsigned)p.y & 0xFFFF0000u);
+}
+__device__ __forceinline__ unsigned coarse_z(int4 p) { return (unsigned)p.z >> 16; }
 template <bool W16>
-__device__ __forceinline__ void build_chunk_fix(const unsigned (&jx)[kBuildChunk], const unsigned (&jy)[kBuildChunk],
-                                                const unsigned (&jz)[kBuildChunk], int4 pf, float lo, float hi,
+__device__ __forceinline__ void build_chunk_fix(const unsigned (&jxy)[kBuildChunk], const unsigned (&jz)[kBuildChunk],
+                                                unsigned pxy, unsigned pz, int clo, int chi,
                                                 const FMD_GLOBAL float4* gpos, int q0, int n, int i, float4 pi,
                                                 float L, float invL, float rcs2, const int (&ex)[4], int nb,
-                                                int bcap, const int* bt, int* out, unsigned short* out16, int kcap,
-                                                int& cnt, int ebase, unsigned char* used) {
+                                                int bcap, const FMD_GLOBAL int* bt, FMD_GLOBAL int* out,
+                                                FMD_GLOBAL unsigned short* out16, int kcap, int& cnt, int ebase,
+                                                unsigned char* used) {
     unsigned m = 0, band = 0;
 #pragma unroll
     for (int k = 0; k < kBuildChunk; k++) {
-        const float q = grid_q(grid_d(pf.x, jx[k]), grid_d(pf.y, jy[k]), grid_d(pf.z, jz[k]));
-        m |= (q < lo) ? (1u << k) : 0u;
-        band |= (q <= hi) ? (1u << k) : 0u;
+        const int q = coarse_q(pxy, pz, jxy[k], jz[k]);
+        m |= (q < clo) ? (1u << k) : 0u;
+        band |= (q <= chi) ? (1u << k) : 0u;
     }
     const unsigned valid = n >= kBuildChunk ? (1u << kBuildChunk) - 1u : (1u << n) - 1u;  // k < n
     m &= valid;
@@ -747,10 +774,12 @@ __device__ inline int block_excl_scan(int x, int* s_wsum, int* total) {
     return before + v - x;
 }
 
-__global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
-    // staged union, fixed point (one chunk of padding: a cell's last chunk reads whole)
-    __shared__ unsigned s_px[kBuildStage + kBuildChunk], s_py[kBuildStage + kBuildChunk],
-        s_pz[kBuildStage + kBuildChunk];
+// At most 128 VGPRs (four waves per SIMD, four workgroups per CU with the
+// LDS below): the rebuilding replicas' groups fit in fewer rounds.
+__global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_eu(4, 8))) k_build(REBUILD_ARGS) {
+    // staged union, coarse fixed point: x and y in one word, z in another (one
+    // chunk of padding: a cell's last chunk reads whole)
+    __shared__ unsigned s_pxy[kBuildStage + kBuildChunk], s_pz[kBuildStage + kBuildChunk];
     __shared__ unsigned short s_seg[kBuildStage];  // staged atom -> union cell
     __shared__ unsigned char s_used[kBuildStage];  // staged atom is in a row of the group
     __shared__ int4 s_cell[kMaxUnion];             // (first slot, count, LDS offset, key)
@@ -760,7 +789,7 @@ __global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
     __shared__ int s_wsum[kGroupTiles];
     __shared__ int s_c0[3], s_nu, s_nstage;
     __shared__ int s_len[kGroupSlots];              // padded row length of each atom of the group
-    unsigned short* s_cidx = (unsigned short*)s_px;  // after the scan: compact index of a used atom
+    unsigned short* s_cidx = (unsigned short*)s_pxy;  // after the scan: compact index of a used atom
     const int wv = threadIdx.x >> 6, lane = lane_id(), tid = threadIdx.x;
     for (int gb = blockIdx.x, ng = *nrt / kGroupTiles; gb < ng; gb += gridDim.x) {
         const Tile t = tiles[rlist[gb * kGroupTiles + wv]];
@@ -775,36 +804,37 @@ __global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
         int nb = 0;
         float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
         if (valid) {
-            pi = d.pos[i];
-            const int g = d.img[i].w;
-            const int b0 = d.t_boff[g], b1 = d.t_boff[g + 1];
+            pi = gview(d.pos)[i];
+            const int g = gview(d.img)[i].w;
+            const int b0 = gview(d.t_boff)[g], b1 = gview(d.t_boff)[g + 1];
             for (int e = b0; e < b1; e++) {
-                const int ent = d.t_blst[e];
-                const int ps = d.slot_of[ent & kSlotMask];
-                if (nb < d.bcap) d.bnd[((size_t)t.lt * d.bcap + nb) * kTile + lane] = ps | (ent & ~kSlotMask);
+                const int ent = gview(d.t_blst)[e];
+                const int ps = gview(d.slot_of)[ent & kSlotMask];
+                if (nb < d.bcap) gmut(d.bnd)[((size_t)t.lt * d.bcap + nb) * kTile + lane] = ps | (ent & ~kSlotMask);
                 if (nb < 4) ex[nb] = ps;
                 nb++;
             }
             if (nb > d.bcap) err |= ERR_BONDED;
             int na = 0;
-            const int a0 = d.t_aoff[g], a1 = d.t_aoff[g + 1];
+            const int a0 = gview(d.t_aoff)[g], a1 = gview(d.t_aoff)[g + 1];
             for (int e = a0; e < a1; e++) {
-                const int4 A = d.t_alst[e];
+                const int4 A = gview(d.t_alst)[e];
                 const int role = (A.x == g) ? 0 : ((A.y == g) ? 1 : 2);
                 const int o1 = (role == 0) ? A.y : A.x;
                 const int o2 = (role == 2) ? A.y : A.z;
-                const int s1 = d.slot_of[o1], s2 = d.slot_of[o2];
+                const int s1 = gview(d.slot_of)[o1], s2 = gview(d.slot_of)[o2];
                 if (na < d.acap)
-                    d.ang[((size_t)t.lt * d.acap + na) * kTile + lane] =
+                    gmut(d.ang)[((size_t)t.lt * d.acap + na) * kTile + lane] =
                         make_int2(s1 | (role << kSlotBits), s2 | (A.w << kSlotBits));
                 na++;
             }
             if (na > d.acap) err |= ERR_BONDED;
-            d.bhdr[i] = min(nb, d.bcap) | (min(na, d.acap) << 8);
+            gmut(d.bhdr)[i] = min(nb, d.bcap) | (min(na, d.acap) << 8);
         }
-        const int* bt = d.bnd + (size_t)t.lt * d.bcap * kTile + lane;
-        int* out = d.nbr + (size_t)t.lt * d.kcap * kTile + lane;
-        unsigned short* out16 = (unsigned short*)(d.nbr + (size_t)t.lt * d.kcap * kTile) + lane;
+        const FMD_GLOBAL int* bt = gview(d.bnd) + (size_t)t.lt * d.bcap * kTile + lane;
+        FMD_GLOBAL int* out = gmut(d.nbr) + (size_t)t.lt * d.kcap * kTile + lane;
+        FMD_GLOBAL unsigned short* out16 =
+            (FMD_GLOBAL unsigned short*)(gmut(d.nbr) + (size_t)t.lt * d.kcap * kTile) + lane;
         const int kcap = d.kcap;
         const float rcs2 = d.rcs2;
         const int nx = st->nx;
@@ -870,14 +900,17 @@ __global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
         // staged scan: prefilter in fixed point; boxes of fewer than 3 cells per
         // axis (where a pair can sit near half the box) use the global scan
         bool staged = W <= kMaxWin && nx >= 3;  // uniform over the group
-        const int4 pf = valid ? d.ipos[i] : make_int4(0, 0, 0, 0);
-        float blo, bhi;
+        const int4 pf = valid ? (int4)gview(d.ipos)[i] : make_int4(0, 0, 0, 0);
+        const unsigned cxy = coarse_xy(pf), cz16 = coarse_z(pf);
+        int clo, chi;
         {
-            const double u = (double)(L * 2.3283064365386963e-10f);
-            const double g = (double)rcs2 / (u * u);
+            // radius in coarse units (L / 2^16), widened by the band the exact
+            // FP32 test may differ by and by sqrt(3) units of truncation
+            const double rc = (double)d.rcs * 65536.0 / (double)L;
             const double band = 3e-5 + 1e-6 * (double)L / (double)d.rcs;  // >> the two forms' difference
-            blo = (float)(g * (1.0 - band));
-            bhi = (float)(g * (1.0 + band));
+            const double rin = rc * sqrt(1.0 - band) - 1.7321 - 0.01, rout = rc * sqrt(1.0 + band) + 1.7321 + 0.01;
+            clo = rin > 0.0 ? (int)floor(rin * rin) : 0;
+            chi = (int)ceil(rout * rout);
         }
         int nu = 0, nstage = 0;
         if (staged) {
@@ -886,6 +919,7 @@ __global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
             // each cell of the group marks its 27 neighbors (its first lane in a wave does)
             const int prev = __shfl_up(cx + nx * (cy + nx * cz), 1, 64);
             if (valid && (lane == 0 || prev != cx + nx * (cy + nx * cz)))
+                #pragma unroll 1
                 for (int c = 0; c < 27; c++) {
                     const int w = win_of(c);
                     atomicOr(&s_bits[w >> 5], 1u << (w & 31));
@@ -975,9 +1009,8 @@ __global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
                 for (int k = 0; k < 4; k++) {
                     const int q = q0 + k * kGroupSlots + tid;
                     if (q < nstage) {
-                        s_px[q] = (unsigned)v4[k].x;
-                        s_py[q] = (unsigned)v4[k].y;
-                        s_pz[q] = (unsigned)v4[k].z;
+                        s_pxy[q] = coarse_xy(v4[k]);
+                        s_pz[q] = coarse_z(v4[k]);
                     }
                 }
             }
@@ -989,6 +1022,7 @@ __global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
                 const int us = s_wmap[win_of(13)];
                 if (us < nu) qself = s_cell[us].z + (i - s_cell[us].x);
             }
+            #pragma unroll 1
             for (int c = 0; c < 27; c++) {
                 const int w = win_of(c);
                 bool seen = false;
@@ -1004,20 +1038,21 @@ __global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
                 for (int q0 = 0; q0 < ce.y; q0 += kBuildChunk) {
                     // a whole chunk from LDS (immediate offsets; entries past the
                     // cell are masked out, and the arrays are padded by a chunk)
-                    unsigned jx[kBuildChunk], jy[kBuildChunk], jz[kBuildChunk];
-                    const unsigned* px = s_px + ce.z + q0;
-                    const unsigned* py = s_py + ce.z + q0;
+                    unsigned jxy[kBuildChunk], jz[kBuildChunk];
+                    const unsigned* pxy = s_pxy + ce.z + q0;
                     const unsigned* pz = s_pz + ce.z + q0;
 #pragma unroll
                     for (int k = 0; k < kBuildChunk; k++) {
-                        jx[k] = px[k]; jy[k] = py[k]; jz[k] = pz[k];
+                        jxy[k] = pxy[k]; jz[k] = pz[k];
                     }
                     if (d.idx16)
-                        build_chunk_fix<true>(jx, jy, jz, pf, blo, bhi, gpos, ce.x + q0, ce.y - q0, i, pi, L, invL,
-                                              rcs2, ex, nb, d.bcap, bt, out, out16, kcap, cnt, ce.z + q0, s_used);
+                        build_chunk_fix<true>(jxy, jz, cxy, cz16, clo, chi, gpos, ce.x + q0, ce.y - q0, i, pi, L,
+                                              invL, rcs2, ex, nb, d.bcap, bt, out, out16, kcap, cnt, ce.z + q0,
+                                              s_used);
                     else
-                        build_chunk_fix<false>(jx, jy, jz, pf, blo, bhi, gpos, ce.x + q0, ce.y - q0, i, pi, L, invL,
-                                               rcs2, ex, nb, d.bcap, bt, out, out16, kcap, cnt, ce.z + q0, s_used);
+                        build_chunk_fix<false>(jxy, jz, cxy, cz16, clo, chi, gpos, ce.x + q0, ce.y - q0, i, pi, L,
+                                               invL, rcs2, ex, nb, d.bcap, bt, out, out16, kcap, cnt, ce.z + q0,
+                                               s_used);
                 }
             }
             if (qself < 0 || qself >= nstage) {
@@ -1058,7 +1093,7 @@ __global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
             // the group's neighbor set: used staged atoms (and every atom itself),
             // numbered in staged (slot) order
             if (valid) s_used[qself] = 1;
-            __syncthreads();  // the scan's last reads of s_px precede the s_cidx writes below
+            __syncthreads();  // the scan's last reads of s_pxy precede the s_cidx writes below
             const int per = (nstage + kGroupSlots - 1) / kGroupSlots;
             const int q0 = min(tid * per, nstage), q1 = min(q0 + per, nstage);
             int c = 0;
@@ -1072,14 +1107,15 @@ __global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
                     if (compact) {
                         const int4 ce = s_cell[s_seg[q]];
                         const int slot = ce.x + q - ce.z;
-                        if (d.idx16) ((unsigned short*)d.gstage)[(size_t)grp * kGroupStage + idx] = (unsigned short)slot;
-                        else d.gstage[(size_t)grp * kGroupStage + idx] = slot;
+                        const size_t at = (size_t)grp * kGroupStage + idx;
+                        if (d.idx16) ((FMD_GLOBAL unsigned short*)gmut(d.gstage))[at] = (unsigned short)slot;
+                        else gmut(d.gstage)[at] = slot;
                     }
                     idx++;
                 }
-            if (tid == 0) d.gcount[grp] = compact ? ncomp : -1;
+            if (tid == 0) gmut(d.gcount)[grp] = compact ? ncomp : -1;
         } else if (tid == 0) {
-            d.gcount[grp] = -1;
+            gmut(d.gcount)[grp] = -1;
         }
         // Placement: the group's rows go to columns in descending order of
         // their padded length (ties by slot), so the 64 rows a wave streams
@@ -1096,7 +1132,7 @@ __global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
             rank += (lb > padded || (lb == padded && b < tid)) ? 1 : 0;
             maxpad = max(maxpad, lb);
         }
-        d.gperm[(size_t)grp * kGroupSlots + rank] = (unsigned char)tid;
+        gmut(d.gperm)[(size_t)grp * kGroupSlots + rank] = (unsigned char)tid;
         auto place = [&](auto* src, auto* dst) {
             for (int k0 = 0; k0 < maxpad; k0 += kPlaceBand) {
                 int v[kPlaceBand];
@@ -1120,12 +1156,12 @@ __global__ void __launch_bounds__(kGroupSlots) k_build(REBUILD_ARGS) {
             }
         };
         // destination column: the same addressing as out / out16 (tile base in int units)
-        int* dtile = d.nbr + (size_t)(grp * kGroupTiles + rank / kTile) * d.kcap * kTile;
-        if (d.idx16) place(out16, (unsigned short*)dtile + (rank % kTile));
+        FMD_GLOBAL int* dtile = gmut(d.nbr) + (size_t)(grp * kGroupTiles + rank / kTile) * d.kcap * kTile;
+        if (d.idx16) place(out16, (FMD_GLOBAL unsigned short*)dtile + (rank % kTile));
         else place(out, dtile + (rank % kTile));
         if (valid) {
             if (cnt > kcap) err |= ERR_NBR_OVERFLOW;
-            d.nnbr[i] = stored;
+            gmut(d.nnbr)[i] = stored;
         }
         err = wave_or(err);
         if (err && lane == 0) atomicOr(&st->err, err);
@@ -1156,9 +1192,16 @@ template <bool OBS, bool CAP, bool ONE_TYPE, bool W16, bool LDS>
 __device__ __forceinline__ LJSum lj_loop(const RepDesc& d, int lt, int lane, int i, int4 pi, float L,
                                          const int4* sp) {
     const int nn = d.nnbr[i];
-    const FMD_GLOBAL int* nl = gview(d.nbr) + (size_t)lt * d.kcap * kTile + lane;
-    const FMD_GLOBAL unsigned short* nl16 =
-        (const FMD_GLOBAL unsigned short*)(gview(d.nbr) + (size_t)lt * d.kcap * kTile) + lane;
+    // the tile's rows as a raw buffer: entry k of this lane sits at byte
+    // (k * kTile + lane) * entry size; the row offset goes in the scalar
+    // offset, so fetching the next chunk costs no vector address arithmetic
+    const __amdgpu_buffer_rsrc_t nrs = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(d.nbr + (size_t)lt * d.kcap * kTile), (short)0, d.kcap * kTile * (int)sizeof(int), kBufferWord3);
+    const unsigned vlane = (unsigned)lane * (W16 ? 2u : 4u);
+    auto entry = [&](int k) -> unsigned {
+        if constexpr (W16) return (unsigned)__builtin_amdgcn_raw_buffer_load_b16(nrs, vlane, k * kTile * 2, 0);
+        else return __builtin_amdgcn_raw_buffer_load_b32(nrs, vlane, k * kTile * 4, 0);
+    };
     const __amdgpu_buffer_rsrc_t prs =
         __builtin_amdgcn_make_buffer_rsrc((void*)d.ipos, (short)sizeof(int4), d.nslots, kBufferWord3);
     const float u = L * 2.3283064365386963e-10f;
@@ -1178,7 +1221,7 @@ __device__ __forceinline__ LJSum lj_loop(const RepDesc& d, int lt, int lane, int
     unsigned nxt[kLJChunk];
     if (nn > 0) {
 #pragma unroll
-        for (int k = 0; k < kLJChunk; k++) nxt[k] = W16 ? (unsigned)nl16[k * kTile] : (unsigned)nl[k * kTile];
+        for (int k = 0; k < kLJChunk; k++) nxt[k] = entry(k);
     }
     for (int k0 = 0; k0 < nn; k0 += kLJChunk) {
         uint4 pjs[kLJChunk];
@@ -1194,9 +1237,7 @@ __device__ __forceinline__ LJSum lj_loop(const RepDesc& d, int lt, int lane, int
         }
         if (k0 + kLJChunk < nn) {
 #pragma unroll
-            for (int k = 0; k < kLJChunk; k++)
-                nxt[k] = W16 ? (unsigned)nl16[(k0 + kLJChunk + k) * kTile]
-                             : (unsigned)nl[(k0 + kLJChunk + k) * kTile];
+            for (int k = 0; k < kLJChunk; k++) nxt[k] = entry(k0 + kLJChunk + k);
         }
 #pragma unroll
         for (int k = 0; k < kLJChunk; k++) {
@@ -1222,6 +1263,10 @@ __device__ __forceinline__ LJSum lj_loop(const RepDesc& d, int lt, int lane, int
                 fr = fminf(fmaxf(fr, -cap), cap);
             }
             gx += fr * ax; gy += fr * ay; gz += fr * az;
+            // keep the type word live so the neighbor read stays 16 bytes:
+            // gfx950 serves randomly addressed 12-byte LDS reads at about half
+            // the rate of 16-byte ones (the empty asm emits no instruction)
+            if constexpr (LDS && ONE_TYPE) asm volatile("" ::"v"(pj.w));
             if (OBS) {
                 const float hx = fr * ax, hy = fr * ay, hz = fr * az;
                 pe += c2e * (sr12 - sr6);
