@@ -54,6 +54,26 @@ __device__ inline double wave_sum(double x) {
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
     return x;
 }
+// The sum over the wave in lane 63 (other lanes hold partial sums), by DPP
+// moves on the VALU instead of LDS permutes: pairs, quads (quad_perm), 8 and
+// 16 lanes (row_shr 4, 8), then rows 0+1 and 2+3 (row_bcast 15) and all four
+// (row_bcast 31).  Every lane of the wave must be active.
+template <int CTRL, int ROWS>
+__device__ inline double dpp_move(double x) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROWS, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROWS, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ inline double wave_sum_last(double x) {
+    x += dpp_move<0xB1, 0xF>(x);   // quad_perm [1, 0, 3, 2]
+    x += dpp_move<0x4E, 0xF>(x);   // quad_perm [2, 3, 0, 1]
+    x += dpp_move<0x114, 0xF>(x);  // row_shr 4
+    x += dpp_move<0x118, 0xF>(x);  // row_shr 8
+    x += dpp_move<0x142, 0xA>(x);  // row_bcast 15 into rows 1 and 3
+    x += dpp_move<0x143, 0xC>(x);  // row_bcast 31 into rows 2 and 3
+    return x;
+}
 __device__ inline int wave_min_i(int x) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o, 64));
@@ -1507,20 +1527,16 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
     double part[13] = {(double)pe, (double)w0, (double)w1, (double)w2, (double)w3, (double)w4,
                        (double)w5, k0, k1, k2, k3, k4, k5};
 #pragma unroll
-    for (int c = 0; c < 13; c++) part[c] = wave_sum(part[c]);
-    err = wave_or(err);
-    if (lane < 13) {
-        double v = part[0];
+    for (int c = 0; c < 13; c++) part[c] = wave_sum_last(part[c]);
+    if (lane == 63)
 #pragma unroll
-        for (int c = 1; c < 13; c++) v = (lane == c) ? part[c] : v;
-        d.wpart[(size_t)t.lt * kNumPart + lane] = v;
-    }
-    if (err && lane == 0) atomicOr(&st->err, err);
+        for (int c = 0; c < 13; c++) d.wpart[(size_t)t.lt * kNumPart + c] = part[c];
+    if (err) atomicOr(&st->err, err);  // failures only
 }
 
 // First reduction level: one workgroup sums the FP64 partials of kChunkTiles
-// tiles of one replica in a fixed order (thread t takes tile t, then wave
-// shuffles, then the four waves in order), so a large replica's sum is spread
+// tiles of one replica in a fixed order (thread t takes tile t, then the DPP
+// wave sum, then the four waves in order), so a large replica's sum is spread
 // over many CUs.
 __global__ void __launch_bounds__(256) k_chunk(const RepDesc* __restrict__ descs,
                                               const Tile* __restrict__ chunks, int nch, int mode) {
@@ -1536,8 +1552,8 @@ __global__ void __launch_bounds__(256) k_chunk(const RepDesc* __restrict__ descs
 #pragma unroll
         for (int c = 0; c < 13; c++) a[c] = d.wpart[(size_t)tl * kNumPart + c];
 #pragma unroll
-    for (int c = 0; c < 13; c++) a[c] = wave_sum(a[c]);
-    if (lane_id() == 0)
+    for (int c = 0; c < 13; c++) a[c] = wave_sum_last(a[c]);
+    if (lane_id() == 63)
 #pragma unroll
         for (int c = 0; c < 13; c++) red[c][threadIdx.x >> 6] = a[c];
     __syncthreads();
