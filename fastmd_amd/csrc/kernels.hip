@@ -808,7 +808,7 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
     __shared__ int s_red[6][kGroupTiles];           // per-wave window bounds
     __shared__ int s_wsum[kGroupTiles];
     __shared__ int s_c0[3], s_nu, s_nstage;
-    __shared__ int s_len[kGroupSlots];              // padded row length of each atom of the group
+    __shared__ __align__(16) int s_len[kGroupSlots];  // padded row length of each atom of the group
     unsigned short* s_cidx = (unsigned short*)s_pxy;  // after the scan: compact index of a used atom
     const int wv = threadIdx.x >> 6, lane = lane_id(), tid = threadIdx.x;
     for (int gb = blockIdx.x, ng = *nrt / kGroupTiles; gb < ng; gb += gridDim.x) {
@@ -853,8 +853,11 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
         }
         const FMD_GLOBAL int* bt = gview(d.bnd) + (size_t)t.lt * d.bcap * kTile + lane;
         FMD_GLOBAL int* out = gmut(d.nbr) + (size_t)t.lt * d.kcap * kTile + lane;
+        // 16-bit rows are built in the upper half of the tile's row area (which
+        // holds 2 kcap 16-bit entries per lane) and moved to the lower half by
+        // the placement below; 32-bit rows are built in place
         FMD_GLOBAL unsigned short* out16 =
-            (FMD_GLOBAL unsigned short*)(gmut(d.nbr) + (size_t)t.lt * d.kcap * kTile) + lane;
+            (FMD_GLOBAL unsigned short*)(gmut(d.nbr) + (size_t)t.lt * d.kcap * kTile) + (size_t)d.kcap * kTile + lane;
         const int kcap = d.kcap;
         const float rcs2 = d.rcs2;
         const int nx = st->nx;
@@ -1147,26 +1150,29 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
         __syncthreads();
         if (valid && compact) selfent = (int)s_cidx[qself] << 4;
         int rank = 0, maxpad = 0;
-        for (int b = 0; b < kGroupSlots; b++) {
-            const int lb = s_len[b];
-            rank += (lb > padded || (lb == padded && b < tid)) ? 1 : 0;
-            maxpad = max(maxpad, lb);
+        for (int b = 0; b < kGroupSlots; b += 4) {  // four lengths per LDS read
+            const int4 l4 = *(const int4*)(s_len + b);
+            const int lb[4] = {l4.x, l4.y, l4.z, l4.w};
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                rank += (lb[e] > padded || (lb[e] == padded && b + e < tid)) ? 1 : 0;
+                maxpad = max(maxpad, lb[e]);
+            }
         }
         gmut(d.gperm)[(size_t)grp * kGroupSlots + rank] = (unsigned char)tid;
+        auto entry = [&](int q) {  // a staged index or slot as the force kernel reads it
+            if (!staged) return q;
+            if (compact) return (int)s_cidx[q] << 4;
+            return s_cell[s_seg[q]].x + q - s_cell[s_seg[q]].z;
+        };
+        // 32-bit rows, in place: every read of a band precedes every write
         auto place = [&](auto* src, auto* dst) {
             for (int k0 = 0; k0 < maxpad; k0 += kPlaceBand) {
                 int v[kPlaceBand];
 #pragma unroll
                 for (int e = 0; e < kPlaceBand; e++) {
                     const int k = k0 + e;
-                    int x = selfent;
-                    if (k < stored) {
-                        const int q = (int)src[k * kTile];
-                        if (!staged) x = q;
-                        else if (compact) x = (int)s_cidx[q] << 4;
-                        else x = s_cell[s_seg[q]].x + q - s_cell[s_seg[q]].z;
-                    }
-                    v[e] = x;
+                    v[e] = (k < stored) ? entry((int)src[k * kTile]) : selfent;
                 }
                 __syncthreads();
 #pragma unroll
@@ -1177,8 +1183,24 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
         };
         // destination column: the same addressing as out / out16 (tile base in int units)
         FMD_GLOBAL int* dtile = gmut(d.nbr) + (size_t)(grp * kGroupTiles + rank / kTile) * d.kcap * kTile;
-        if (d.idx16) place(out16, (FMD_GLOBAL unsigned short*)dtile + (rank % kTile));
-        else place(out, dtile + (rank % kTile));
+        if (d.idx16) {
+            // from this lane's row in the upper half to its column in the lower
+            // half: no entry is both read and written, so no barriers
+            FMD_GLOBAL unsigned short* dst = (FMD_GLOBAL unsigned short*)dtile + (rank % kTile);
+            for (int k0 = 0; k0 < padded; k0 += kPlaceBand) {
+                int v[kPlaceBand];
+#pragma unroll
+                for (int e = 0; e < kPlaceBand; e++) {
+                    const int k = k0 + e;
+                    v[e] = (k < stored) ? entry((int)out16[k * kTile]) : selfent;
+                }
+#pragma unroll
+                for (int e = 0; e < kPlaceBand; e++)
+                    if (k0 + e < padded) dst[(k0 + e) * kTile] = (unsigned short)v[e];
+            }
+        } else {
+            place(out, dtile + (rank % kTile));
+        }
         if (valid) {
             if (cnt > kcap) err |= ERR_NBR_OVERFLOW;
             gmut(d.nnbr)[i] = stored;
