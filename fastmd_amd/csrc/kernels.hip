@@ -1467,22 +1467,28 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
                 // capped soft push-off, where the clamp below keeps it finite
                 if (!(r2 < R02) && !(d.fcap > 0.0f)) err |= ERR_STRETCH;
                 r2 = fminf(fmaxf(r2, 1e-6f), R02 * 0.9999f);  // src/force/fene.cu:33
-                float ff = -bp.x * R02 / (R02 - r2);
-                float bpe = -0.5f * bp.x * R02 * logf(1.0f - r2 / R02);
+                // divisions as hardware reciprocals (1 ulp), as the reference's
+                // fast-math build does; PE and virial only where observed
+                float ff = -bp.x * R02 * __builtin_amdgcn_rcpf(R02 - r2);
                 const float sig2 = bp.w * bp.w;
-                if (r2 < sig2 * 1.2599210498948732f) {
-                    const float r2inv = 1.0f / r2;
+                const bool wca = r2 < sig2 * 1.2599210498948732f;
+                float sr6 = 0.0f, sr12 = 0.0f;
+                if (wca) {
+                    const float r2inv = __builtin_amdgcn_rcpf(r2);
                     const float sr2 = sig2 * r2inv;
-                    const float sr6 = sr2 * sr2 * sr2;
-                    const float sr12 = sr6 * sr6;
+                    sr6 = sr2 * sr2 * sr2;
+                    sr12 = sr6 * sr6;
                     ff += 24.0f * bp.z * r2inv * (2.0f * sr12 - sr6);
-                    bpe += 4.0f * bp.z * (sr12 - sr6) + bp.z;
                 }
-                const float hx = 0.5f * ff * dx, hy = 0.5f * ff * dy, hz = 0.5f * ff * dz;
                 fx += ff * dx; fy += ff * dy; fz += ff * dz;
-                pe += 0.5f * bpe;
-                w0 += hx * dx; w1 += hx * dy; w2 += hx * dz;
-                w3 += hy * dy; w4 += hy * dz; w5 += hz * dz;
+                if (obs) {
+                    float bpe = -0.5f * bp.x * R02 * logf(1.0f - r2 / R02);
+                    if (wca) bpe += 4.0f * bp.z * (sr12 - sr6) + bp.z;
+                    const float hx = 0.5f * ff * dx, hy = 0.5f * ff * dy, hz = 0.5f * ff * dz;
+                    pe += 0.5f * bpe;
+                    w0 += hx * dx; w1 += hx * dy; w2 += hx * dz;
+                    w3 += hy * dy; w4 += hy * dz; w5 += hz * dz;
+                }
             }
         }
         if (terms & 4) {
@@ -1513,8 +1519,8 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
                 const float c = dot * inv_ab;
                 const float s = fmaxf(cr * inv_ab, 1e-6f);
                 const float dth = theta - ap.y;
-                const float pref = 2.0f * ap.x * dth / s;
-                const float ia2 = 1.0f / ra2, ib2 = 1.0f / rb2;
+                const float pref = 2.0f * ap.x * dth * __builtin_amdgcn_rcpf(s);
+                const float ia2 = __builtin_amdgcn_rcpf(ra2), ib2 = __builtin_amdgcn_rcpf(rb2);
                 const float fix = pref * (bx * inv_ab - c * ax * ia2);
                 const float fiy = pref * (by * inv_ab - c * ay * ia2);
                 const float fiz = pref * (bz * inv_ab - c * az * ia2);
@@ -1524,14 +1530,16 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
                 if (role == 0) { fx += fix; fy += fiy; fz += fiz; }
                 else if (role == 2) { fx += fkx; fy += fky; fz += fkz; }
                 else { fx -= fix + fkx; fy -= fiy + fky; fz -= fiz + fkz; }
-                const float third = 1.0f / 3.0f;
-                pe += ap.x * dth * dth * third;
-                w0 += (fix * ax + fkx * bx) * third;
-                w1 += (fix * ay + fkx * by) * third;
-                w2 += (fix * az + fkx * bz) * third;
-                w3 += (fiy * ay + fky * by) * third;
-                w4 += (fiy * az + fky * bz) * third;
-                w5 += (fiz * az + fkz * bz) * third;
+                if (obs) {
+                    const float third = 1.0f / 3.0f;
+                    pe += ap.x * dth * dth * third;
+                    w0 += (fix * ax + fkx * bx) * third;
+                    w1 += (fix * ay + fkx * by) * third;
+                    w2 += (fix * az + fkx * bz) * third;
+                    w3 += (fiy * ay + fky * by) * third;
+                    w4 += (fiy * az + fky * bz) * third;
+                    w5 += (fiz * az + fkz * bz) * third;
+                }
             }
         }
         d.force[i] = make_float4(fx, fy, fz, pe);
