@@ -1081,7 +1081,7 @@ int32_t Engine::get_rg(int32_t sys, double* rg, double* per_chain, int32_t cap, 
 // ---------------------------------------------------------------------------
 namespace {
 constexpr uint32_t kSnapMagic = 0x534D4446u;  // "FDMS"
-constexpr uint32_t kSnapVersion = 1;
+constexpr uint32_t kSnapVersion = 2;
 struct SnapHeader {
     uint32_t magic, version, desc_bytes, state_bytes;
     uint64_t used;

@@ -71,6 +71,8 @@ struct RepState {
     int err;               // ErrBits
     int pre_ready;         // start-of-step scalars for step+1 are computed
     int want_obs;          // the current step records PE and virial (NPT or thermo step)
+    int kicked;            // Langevin: the last force pass also gave the next step's
+                           // pre-force half kick (vel holds it; force was not stored)
     int64_t nrebuilds;
     // current box and cell grid (float copies for per-atom kernels)
     float L, invL;      // box of the next position update (the barostat advances it after the forces)

@@ -270,15 +270,18 @@ __global__ void __launch_bounds__(256) k_pre(const RepDesc* __restrict__ descs,
     int bad = 0;
     if (i < d.natoms) {
         float4 r = d.pos[i], v = d.vel[i];
-        const float4 f = d.force[i];
         int4 im = d.img[i];
         float4 ref = d.pos_ref[i];
         const float m = d.mass[type_of(r.w)];
         const float hdtm = d.half_dt / m;
         const float L = st->L, invL = st->invL;
         if (d.ensemble == 0) {
-            v.x += hdtm * f.x; v.y += hdtm * f.y; v.z += hdtm * f.z;
-            if (d.vlim > 0.0f) v = limit_speed(v, d.vlim);
+            if (!st->kicked) {  // else the last force pass gave this half kick
+                const float4 f = d.force[i];
+                // explicit FMAs: the force kernel's kick ahead computes the same bits
+                v.x = fmaf(hdtm, f.x, v.x); v.y = fmaf(hdtm, f.y, v.y); v.z = fmaf(hdtm, f.z, v.z);
+                if (d.vlim > 0.0f) v = limit_speed(v, d.vlim);
+            }
             r.x += d.half_dt * v.x; r.y += d.half_dt * v.y; r.z += d.half_dt * v.z;
             const float3 z = normal3(d.seed_lo, d.seed_hi, (uint32_t)im.w, st->step + 1);
             const float ns = d.c2 * sqrtf(st->kT / m);
@@ -287,6 +290,7 @@ __global__ void __launch_bounds__(256) k_pre(const RepDesc* __restrict__ descs,
             r.x += d.half_dt * v.x; r.y += d.half_dt * v.y; r.z += d.half_dt * v.z;
         } else {
             const float sc = st->vscale;
+            const float4 f = d.force[i];
             v.x *= sc; v.y *= sc; v.z *= sc;
             v.x += hdtm * f.x; v.y += hdtm * f.y; v.z += hdtm * f.z;
             if (d.ensemble == 1) {
@@ -1564,17 +1568,24 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
                 }
             }
         }
-        d.force[i] = make_float4(fx, fy, fz, pe);
+        // Langevin steps followed by another step of the run also give that
+        // step's pre-force half kick (the same force, the same operations as
+        // k_pre), so the force need not be stored and read back; the kinetic
+        // partials are taken between the two kicks
+        const bool ahead = kick && mode == FM_STEP && d.ensemble == 0 && st->step + 1 < st->stop;
+        if (!ahead) d.force[i] = make_float4(fx, fy, fz, pe);
         float4 v = d.vel[i];
         const float m = d.mass[ti];
-        if (kick) {
-            const float hdtm = d.half_dt / m;
-            v.x += hdtm * fx; v.y += hdtm * fy; v.z += hdtm * fz;
-            d.vel[i] = v;
-        }
+        const float hdtm = d.half_dt / m;
+        if (kick) { v.x = fmaf(hdtm, fx, v.x); v.y = fmaf(hdtm, fy, v.y); v.z = fmaf(hdtm, fz, v.z); }
         const double md = m, vx = v.x, vy = v.y, vz = v.z;
         k0 = md * vx * vx; k1 = md * vx * vy; k2 = md * vx * vz;
         k3 = md * vy * vy; k4 = md * vy * vz; k5 = md * vz * vz;
+        if (ahead) {  // bit for bit the kick of k_pre (explicit FMAs there too)
+            v.x = fmaf(hdtm, fx, v.x); v.y = fmaf(hdtm, fy, v.y); v.z = fmaf(hdtm, fz, v.z);
+            if (d.vlim > 0.0f) v = limit_speed(v, d.vlim);
+        }
+        if (kick) d.vel[i] = v;
     }
     double part[13] = {(double)pe, (double)w0, (double)w1, (double)w2, (double)w3, (double)w4,
                        (double)w5, k0, k1, k2, k3, k4, k5};
@@ -1741,6 +1752,8 @@ __device__ int boundary_tail(const RepDesc& d, RepState* st, int mode, const dou
     const int sample = d.corr && st->stress_freq > 0 && (cur % st->stress_freq) == 0;
     if (sample)
         for (int c = 0; c < 6; c++) smp[c] = (st->kin[c] + W[c]) / V;
+    // the force pass of this step kicked ahead exactly when another step follows
+    st->kicked = (d.ensemble == 0 && cur < st->stop) ? 1 : 0;
     if (cur >= st->stop) {
         st->active = 0;
         st->pre_ready = 0;

@@ -164,3 +164,26 @@ def test_rg_reference_cases_and_melt(fmd, engine, orc):
     assert abs(rg - ref) <= 1e-6 * ref
     assert 2.0 < rg < 6.0  # ~ sqrt(N/6) b for N = 50 beads
     engine.remove(sid)
+
+
+@pytest.mark.parametrize("cap", [0.0, 50.0])
+def test_one_run_equals_single_steps(fmd, engine, cap):
+    """Within a Langevin run the force pass also gives the next step's
+    pre-force half kick; the last step of a run does not, and the next run's
+    pre-force kernel gives it from the stored force.  Both routes give the
+    same bits: one 30-step run equals 30 one-step runs, including under the
+    push-off force cap (with its speed limit)."""
+    d = relax_kg(fmd, engine, 12, 40, seed=77, T=(1.0, 1.2), nsteps=100)
+    a = engine.add(d)
+    b = engine.add(d)
+    if cap > 0.0:
+        engine.set_force_cap(a, cap)
+        engine.set_force_cap(b, cap)
+    engine.step(a, 30)
+    for _ in range(30):
+        engine.step(b, 1)
+    for x, y in zip(engine.state(a), engine.state(b)):
+        assert np.array_equal(x, y)
+    assert engine.thermo(a) == engine.thermo(b)
+    engine.remove(a)
+    engine.remove(b)
