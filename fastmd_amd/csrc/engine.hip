@@ -377,6 +377,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     D.pos_ref = (float4*)carve(ns * 16);
     D.ipos = (int4*)carve(ns * 16);
     D.img = (int4*)carve(ns * 16);
+    D.gid = (int*)carve(ns * 4);
     D.tmp_pos = (float4*)carve(ns * 16);
     D.tmp_vel = (float4*)carve(ns * 16);
     D.tmp_img = (int4*)carve(ns * 16);
@@ -523,6 +524,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     up(D.slot_of, ident.data(), na * 4);
     zero(D.force, ns * 16);
     zero(D.ipos, ns * 16);
+    zero(D.gid, ns * 4);  // set by the sort of the initial build
     zero(D.cell_cnt, (size_t)S.ncell_cap * 4);
     zero(D.nnbr, ns * 4);
     zero(D.gcount, nt / kGroupTiles * 4);
@@ -1093,7 +1095,7 @@ struct SnapHeader {
 // Applies f to every pointer member of a descriptor.
 template <class F>
 void for_each_ptr(RepDesc& d, F&& f) {
-    f(d.pos); f(d.vel); f(d.force); f(d.pos_ref); f(d.ipos); f(d.img);
+    f(d.pos); f(d.vel); f(d.force); f(d.pos_ref); f(d.ipos); f(d.img); f(d.gid);
     f(d.tmp_pos); f(d.tmp_vel); f(d.tmp_img); f(d.cell_of); f(d.rank); f(d.order); f(d.order2);
     f(d.slot_of); f(d.cell_cnt); f(d.cell_start); f(d.nnbr); f(d.nbr); f(d.gcount); f(d.gstage); f(d.gperm); f(d.bhdr); f(d.bnd); f(d.ang);
     f(d.t_boff); f(d.t_blst); f(d.t_aoff); f(d.t_alst); f(d.t_choff);

@@ -136,6 +136,7 @@ struct RepDesc {
     float4* pos_ref;
     int4* ipos;            // fixed-point copy of pos for the force path (x, y, z, type bits)
     int4* img;             // image counts x, y, z and the global id in w
+    int* gid;              // the global id alone (img.w), for the per-step noise key
     float4* tmp_pos;
     float4* tmp_vel;
     int4* tmp_img;

@@ -270,7 +270,6 @@ __global__ void __launch_bounds__(256) k_pre(const RepDesc* __restrict__ descs,
     int bad = 0;
     if (i < d.natoms) {
         float4 r = d.pos[i], v = d.vel[i];
-        int4 im = d.img[i];
         float4 ref = d.pos_ref[i];
         const float m = d.mass[type_of(r.w)];
         const float hdtm = d.half_dt / m;
@@ -283,7 +282,7 @@ __global__ void __launch_bounds__(256) k_pre(const RepDesc* __restrict__ descs,
                 if (d.vlim > 0.0f) v = limit_speed(v, d.vlim);
             }
             r.x += d.half_dt * v.x; r.y += d.half_dt * v.y; r.z += d.half_dt * v.z;
-            const float3 z = normal3(d.seed_lo, d.seed_hi, (uint32_t)im.w, st->step + 1);
+            const float3 z = normal3(d.seed_lo, d.seed_hi, (uint32_t)d.gid[i], st->step + 1);
             const float ns = d.c2 * sqrtf(st->kT / m);
             v.x = d.c1 * v.x + ns * z.x; v.y = d.c1 * v.y + ns * z.y; v.z = d.c1 * v.z + ns * z.z;
             if (d.vlim > 0.0f) v = limit_speed(v, d.vlim);
@@ -305,7 +304,7 @@ __global__ void __launch_bounds__(256) k_pre(const RepDesc* __restrict__ descs,
         const int ix = (int)floorf(r.x * invL), iy = (int)floorf(r.y * invL),
                   iz = (int)floorf(r.z * invL);
         r.x -= (float)ix * L; r.y -= (float)iy * L; r.z -= (float)iz * L;
-        im.x += ix; im.y += iy; im.z += iz;
+
         float dx = r.x - ref.x, dy = r.y - ref.y, dz = r.z - ref.z;
         dx -= L * roundf(dx * invL); dy -= L * roundf(dy * invL); dz -= L * roundf(dz * invL);
         d2 = dx * dx + dy * dy + dz * dz;
@@ -313,7 +312,11 @@ __global__ void __launch_bounds__(256) k_pre(const RepDesc* __restrict__ descs,
                 isfinite(v.y) && isfinite(v.z));
         d.pos[i] = r;
         d.vel[i] = v;
-        if (ix | iy | iz) d.img[i] = im;  // image counts change only on a wrap
+        if (ix | iy | iz) {  // image counts change only on a wrap
+            int4 im = d.img[i];
+            im.x += ix; im.y += iy; im.z += iz;
+            d.img[i] = im;
+        }
         const double sc = 4294967296.0 / (double)L;
         d.ipos[i] = make_int4((int)fix_coord(r.x, sc), (int)fix_coord(r.y, sc), (int)fix_coord(r.z, sc),
                               __float_as_int(r.w));
@@ -641,6 +644,7 @@ __global__ void __launch_bounds__(256) k_gather(REBUILD_ARGS) {
             d.pos_ref[i] = p;
             d.vel[i] = d.tmp_vel[src];
             d.img[i] = im;
+            d.gid[i] = im.w;
             d.slot_of[im.w] = i;
             const double sc = 4294967296.0 / (double)st->Lp;
             d.ipos[i] = make_int4((int)fix_coord(p.x, sc), (int)fix_coord(p.y, sc), (int)fix_coord(p.z, sc),
