@@ -816,7 +816,7 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
     __shared__ int s_red[6][kGroupTiles];           // per-wave window bounds
     __shared__ int s_wsum[kGroupTiles];
     __shared__ int s_c0[3], s_nu, s_nstage;
-    __shared__ __align__(16) int s_len[kGroupSlots];  // padded row length of each atom of the group
+    __shared__ int s_len[kGroupSlots];              // padded row length of each atom of the group
     unsigned short* s_cidx = (unsigned short*)s_pxy;  // after the scan: compact index of a used atom
     const int wv = threadIdx.x >> 6, lane = lane_id(), tid = threadIdx.x;
     for (int gb = blockIdx.x, ng = *nrt / kGroupTiles; gb < ng; gb += gridDim.x) {
@@ -1158,14 +1158,10 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
         __syncthreads();
         if (valid && compact) selfent = (int)s_cidx[qself] << 4;
         int rank = 0, maxpad = 0;
-        for (int b = 0; b < kGroupSlots; b += 4) {  // four lengths per LDS read
-            const int4 l4 = *(const int4*)(s_len + b);
-            const int lb[4] = {l4.x, l4.y, l4.z, l4.w};
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                rank += (lb[e] > padded || (lb[e] == padded && b + e < tid)) ? 1 : 0;
-                maxpad = max(maxpad, lb[e]);
-            }
+        for (int b = 0; b < kGroupSlots; b++) {
+            const int lb = s_len[b];
+            rank += (lb > padded || (lb == padded && b < tid)) ? 1 : 0;
+            maxpad = max(maxpad, lb);
         }
         gmut(d.gperm)[(size_t)grp * kGroupSlots + rank] = (unsigned char)tid;
         auto entry = [&](int q) {  // a staged index or slot as the force kernel reads it
@@ -1194,17 +1190,23 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
         if (d.idx16) {
             // from this lane's row in the upper half to its column in the lower
             // half: no entry is both read and written, so no barriers
+            // (the next band's loads are in flight while a band is converted)
             FMD_GLOBAL unsigned short* dst = (FMD_GLOBAL unsigned short*)dtile + (rank % kTile);
+            int cur[kPlaceBand], nxt[kPlaceBand];
+            auto load = [&](int k0, int (&to)[kPlaceBand]) {
+#pragma unroll
+                for (int e = 0; e < kPlaceBand; e++) to[e] = (k0 + e < stored) ? (int)out16[(k0 + e) * kTile] : 0;
+            };
+            load(0, cur);
             for (int k0 = 0; k0 < padded; k0 += kPlaceBand) {
-                int v[kPlaceBand];
+                if (k0 + kPlaceBand < padded) load(k0 + kPlaceBand, nxt);
 #pragma unroll
                 for (int e = 0; e < kPlaceBand; e++) {
                     const int k = k0 + e;
-                    v[e] = (k < stored) ? entry((int)out16[k * kTile]) : selfent;
+                    if (k < padded) dst[k * kTile] = (unsigned short)((k < stored) ? entry(cur[e]) : selfent);
                 }
 #pragma unroll
-                for (int e = 0; e < kPlaceBand; e++)
-                    if (k0 + e < padded) dst[(k0 + e) * kTile] = (unsigned short)v[e];
+                for (int e = 0; e < kPlaceBand; e++) cur[e] = nxt[e];
             }
         } else {
             place(out, dtile + (rank % kTile));
