@@ -469,53 +469,50 @@ __global__ void __launch_bounds__(1024) k_rebuild_list(const RepDesc* __restrict
                                                        const int* __restrict__ rep_t0, int nr,
                                                        int always, int* __restrict__ rlist,
                                                        int* __restrict__ nrt) {
-    constexpr int kPer = kMaxReplicas / 1024;  // replicas per thread
     __shared__ int wsum[16];
-    __shared__ int s_off[kMaxReplicas + 1];  // exclusive prefix of the listed tile counts
-    int cnt[kPer];
-    int local = 0;
-    for (int k = 0; k < kPer; k++) {
-        const int r = threadIdx.x * kPer + k;
-        cnt[k] = 0;
+    __shared__ int s_off[kMaxReplicas];  // exclusive prefix of the listed tile counts
+    __shared__ int s_carry;
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    // one replica per thread and pass, a block prefix per pass of 1024
+    for (int r0 = 0; r0 < nr; r0 += 1024) {
+        const int r = r0 + threadIdx.x;
+        int c = 0;
         if (r < nr) {
             const RepDesc& d = descs[reps[r]];
             const RepState* st = d.st;
-            if (always || (st->active && st->rebuild)) cnt[k] = d.ntiles;
+            if (always || (st->active && st->rebuild)) c = d.ntiles;
         }
-        local += cnt[k];
-    }
-    int x = local;
-    const int lane = lane_id(), wv = threadIdx.x >> 6;
+        int x = c;
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) wsum[wv] = x;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int k = 0; k < 16; k++) { const int v = wsum[k]; wsum[k] = acc; acc += v; }
-        s_off[nr] = acc;
-        *nrt = acc;
-    }
-    __syncthreads();
-    int off = wsum[wv] + x - local;
-    for (int k = 0; k < kPer; k++) {
-        const int r = threadIdx.x * kPer + k;
-        if (r < nr) s_off[r] = off;
-        off += cnt[k];
-    }
-    __syncthreads();
-    // entry v belongs to the last replica r with s_off[r] <= v
-    const int total = s_off[nr];
-    for (int v = threadIdx.x; v < total; v += blockDim.x) {
-        int lo = 0, hi = nr - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_off[mid] <= v) lo = mid; else hi = mid - 1;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
         }
-        rlist[v] = rep_t0[lo] + (v - s_off[lo]);
+        if (lane == 63) wsum[wv] = x;
+        __syncthreads();
+        int before = 0, all = 0;
+        for (int w = 0; w < 16; w++) {
+            const int ws = wsum[w];
+            before += (w < wv) ? ws : 0;
+            all += ws;
+        }
+        const int carry = s_carry;
+        if (r < nr) s_off[r] = carry + before + x - c;
+        __syncthreads();
+        if (threadIdx.x == 0) s_carry = carry + all;
+        __syncthreads();
+    }
+    const int total = s_carry;
+    if (threadIdx.x == 0) *nrt = total;
+    // each wave writes the tiles of replicas wv, wv + 16, ... (coalesced runs)
+    for (int r = wv; r < nr; r += 16) {
+        const int o = s_off[r];
+        const int n = ((r + 1 < nr) ? s_off[r + 1] : total) - o;
+        if (n == 0) continue;
+        const int t0 = rep_t0[r];
+        for (int v = lane; v < n; v += 64) rlist[o + v] = t0 + v;
     }
 }
 
