@@ -660,7 +660,9 @@ __global__ void __launch_bounds__(256) k_gather(REBUILD_ARGS) {
 // One workgroup of kGroupTiles waves per group of tiles.  The group's atoms lie
 // in a few Morton-consecutive cells; the union of those cells' 27-neighborhoods
 // (typically 90-140 cells, about 2000-2900 atoms) is staged in LDS with
-// coalesced loads, and each lane scans its own 27 cells from LDS in
+// coalesced loads (the top 16 bits of each grid coordinate: a prefilter that
+// sends the few candidates near the radius to the exact FP32 test), and each
+// lane scans its own 27 cells from LDS in
 // (dz, dy, dx) order, candidates in ascending slot order, exactly as the
 // global-memory scan that serves groups whose union does not fit.  Both give
 // the same rows.  The atoms the group's rows reference (its neighbor set, about
