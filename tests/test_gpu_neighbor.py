@@ -108,5 +108,15 @@ def test_pairs_and_forces_with_32bit_rows(fmd, engine, orc, monkeypatch):
     assert np.array_equal(engine.pairs(sids[True]), ref)
     for a, b in zip(f16, f32):
         assert np.array_equal(np.asarray(a), np.asarray(b))
+    # both layouts place the same rows, so 60 steps (several rebuilds through
+    # the step path) give the same bits, and the lists stay the exact pair set
+    engine.step([sids[False], sids[True]], 60)
+    assert engine.rebuilds(sids[False]) >= 3
+    for a, b in zip(engine.state(sids[False]), engine.state(sids[True])):
+        assert np.array_equal(a, b)
+    p4, _, _ = engine.state(sids[True])
+    engine.build_neighbors(sids[True])
+    ref = orc.pair_set(p4, np.float32(engine.state_box(sids[True])), np.float32(2.5) + np.float32(0.3), excl)
+    assert np.array_equal(engine.pairs(sids[True]), ref)
     for s in sids.values():
         engine.remove(s)
