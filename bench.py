@@ -258,7 +258,11 @@ def main():
                      "kernel": "k_force<1> (LJ over the Verlet list)", "ms_per_launch": ms_lj,
                      "alg_bytes_per_launch": bytes_lj, "stored_bytes_per_launch": stored_lj,
                      "listed_pairs": pairs,
-                     "alg_bytes_rule": "36 B per bead + 4 B per listed pair (rows stored as 16-bit slots)"},
+                     "alg_bytes_rule": "36 B per bead + 4 B per listed pair (rows stored as 16-bit slots)",
+                     "limiter": ("VALU issue: about 90% of the cycles per CU issue a VALU instruction "
+                                 "(23 per pair plus one quarter-rate reciprocal); HBM traffic is 1.12 GB "
+                                 "per launch" if traffic else None),
+                     "limiter_source": (os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None)},
         "cpu_baseline": cpu,
         "detail": {"plain_bead_steps_per_s": plain, "batched_over_plain": batched / plain,
                    "plain_steps_per_replica": args.plain_steps,
