@@ -1455,6 +1455,10 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
     const int4* sp = nst >= 0 ? s_pos : nullptr;
     float fx = 0.f, fy = 0.f, fz = 0.f, pe = 0.f;
     float w0 = 0.f, w1 = 0.f, w2 = 0.f, w3 = 0.f, w4 = 0.f, w5 = 0.f;
+    // Langevin steps that record nothing need none of the FP64 partials (no
+    // thermostat reads the kinetic energy; the host reevaluates unobserved
+    // steps on request), and neither does a forces-only launch
+    const bool partials = mode == FM_OBS || (mode == FM_STEP && (d.ensemble != 0 || st->want_obs));
     double k0 = 0, k1 = 0, k2 = 0, k3 = 0, k4 = 0, k5 = 0;
     int err = 0;
     if (i < d.natoms) {
@@ -1583,22 +1587,26 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
         const float m = d.mass[ti];
         const float hdtm = d.half_dt / m;
         if (kick) { v.x = fmaf(hdtm, fx, v.x); v.y = fmaf(hdtm, fy, v.y); v.z = fmaf(hdtm, fz, v.z); }
-        const double md = m, vx = v.x, vy = v.y, vz = v.z;
-        k0 = md * vx * vx; k1 = md * vx * vy; k2 = md * vx * vz;
-        k3 = md * vy * vy; k4 = md * vy * vz; k5 = md * vz * vz;
+        if (partials) {
+            const double md = m, vx = v.x, vy = v.y, vz = v.z;
+            k0 = md * vx * vx; k1 = md * vx * vy; k2 = md * vx * vz;
+            k3 = md * vy * vy; k4 = md * vy * vz; k5 = md * vz * vz;
+        }
         if (ahead) {  // bit for bit the kick of k_pre (explicit FMAs there too)
             v.x = fmaf(hdtm, fx, v.x); v.y = fmaf(hdtm, fy, v.y); v.z = fmaf(hdtm, fz, v.z);
             if (d.vlim > 0.0f) v = limit_speed(v, d.vlim);
         }
         if (kick) d.vel[i] = v;
     }
-    double part[13] = {(double)pe, (double)w0, (double)w1, (double)w2, (double)w3, (double)w4,
-                       (double)w5, k0, k1, k2, k3, k4, k5};
+    if (partials) {
+        double part[13] = {(double)pe, (double)w0, (double)w1, (double)w2, (double)w3, (double)w4,
+                           (double)w5, k0, k1, k2, k3, k4, k5};
 #pragma unroll
-    for (int c = 0; c < 13; c++) part[c] = wave_sum_last(part[c]);
-    if (lane == 63)
+        for (int c = 0; c < 13; c++) part[c] = wave_sum_last(part[c]);
+        if (lane == 63)
 #pragma unroll
-        for (int c = 0; c < 13; c++) d.wpart[(size_t)t.lt * kNumPart + c] = part[c];
+            for (int c = 0; c < 13; c++) d.wpart[(size_t)t.lt * kNumPart + c] = part[c];
+    }
     if (err) atomicOr(&st->err, err);  // failures only
 }
 
@@ -1611,7 +1619,7 @@ __global__ void __launch_bounds__(256) k_chunk(const RepDesc* __restrict__ descs
     __shared__ double red[13][4];
     const Tile ch = chunks[blockIdx.x];
     const RepDesc& d = descs[ch.rep];
-    if (mode == BND_STEP && !d.st->active) return;
+    if (mode == BND_STEP && (!d.st->active || (d.ensemble == 0 && !d.st->want_obs))) return;  // nothing to sum
     const int tl = ch.lt * kChunkTiles + threadIdx.x;
     double a[13];
 #pragma unroll
@@ -1732,7 +1740,8 @@ __device__ int boundary_tail(const RepDesc& d, RepState* st, int mode, const dou
     }
     st->v_pending = (float)ks;
     const double ks2 = (double)st->v_pending * (double)st->v_pending;
-    for (int c = 0; c < 6; c++) st->kin[c] = K[c] * ks2;
+    if (d.ensemble != 0 || st->want_obs)  // else this step reduced no partials
+        for (int c = 0; c < 6; c++) st->kin[c] = K[c] * ks2;
     if (st->want_obs) {
         st->pe = pe;
         for (int c = 0; c < 6; c++) st->vir[c] = W[c];
@@ -1790,9 +1799,11 @@ __global__ void __launch_bounds__(kBndThreads) k_boundary(const RepDesc* __restr
     // second level: lane l of a wave reads component l % 16 of chunk 4 w + l / 16
     // (+ 4 x waves per pass), the chunks in a fixed order
     {
+        // a Langevin step that records nothing reduced no partials (k_force)
+        const bool none = mode == BND_STEP && d.ensemble == 0 && !st->want_obs;
         const int lane = lane_id(), wv = threadIdx.x >> 6, c = lane & 15;
         double a = 0.0;
-        for (int ck = 4 * wv + (lane >> 4); ck < d.nchunks; ck += 4 * (kBndThreads / 64))
+        for (int ck = 4 * wv + (lane >> 4); !none && ck < d.nchunks; ck += 4 * (kBndThreads / 64))
             a += d.cpart[(size_t)ck * kNumPart + c];
         a += __shfl_xor(a, 16, 64);
         a += __shfl_xor(a, 32, 64);
