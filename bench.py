@@ -259,9 +259,9 @@ def main():
                      "alg_bytes_per_launch": bytes_lj, "stored_bytes_per_launch": stored_lj,
                      "listed_pairs": pairs,
                      "alg_bytes_rule": "36 B per bead + 4 B per listed pair (rows stored as 16-bit slots)",
-                     "limiter": ("VALU issue: about 90% of the cycles per CU issue a VALU instruction "
-                                 "(23 per pair plus one quarter-rate reciprocal); HBM traffic is 1.12 GB "
-                                 "per launch" if traffic else None),
+                     "limiter": ("VALU issue: a VALU instruction issues in about 86% of the cycles per CU, "
+                                 "94% counting the quarter-rate reciprocal (23 instructions per pair); "
+                                 "HBM traffic is 1.11 GB per launch" if traffic else None),
                      "limiter_source": (os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None)},
         "cpu_baseline": cpu,
         "detail": {"plain_bead_steps_per_s": plain, "batched_over_plain": batched / plain,
