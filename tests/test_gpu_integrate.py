@@ -187,3 +187,30 @@ def test_one_run_equals_single_steps(fmd, engine, cap):
     assert engine.thermo(a) == engine.thermo(b)
     engine.remove(a)
     engine.remove(b)
+
+
+def test_thermo_after_unobserved_steps(fmd, engine):
+    """Langevin steps that record nothing sum no FP64 partials; the last step
+    of a run always does, so the thermo read after a run with no thermo
+    output is that of the final state: its kinetic energy and kinetic stress
+    equal sums over the returned velocities, and its step is the last one."""
+    d = relax_kg(fmd, engine, 12, 40, seed=78, T=(1.0, 1.2), nsteps=100, thermo_freq=0)
+    sid = engine.add(d)
+    s0 = engine.steps_done(sid)
+    engine.step(sid, 37)
+    t = engine.thermo(sid)
+    _, v, _ = engine.state(sid)
+    vd = v.astype(np.float64)
+    n = len(vd)
+    assert t["step"] == s0 + 37
+    assert abs(t["ke"] - 0.5 * np.sum(vd * vd) / n) < 1e-9 * max(1.0, t["ke"])
+    # PE and pressure tensor: the final step's virial equals a fresh evaluation
+    # at the returned state, plus the kinetic tensor of the returned velocities
+    _, vir, pe = engine.forces(sid)
+    V = float(engine.box(sid)) ** 3
+    kin = np.array([vd[:, 0] @ vd[:, 0], vd[:, 0] @ vd[:, 1], vd[:, 0] @ vd[:, 2],
+                    vd[:, 1] @ vd[:, 1], vd[:, 1] @ vd[:, 2], vd[:, 2] @ vd[:, 2]])
+    assert abs(t["pe"] - pe / n) <= 1e-12 * abs(pe / n)
+    scale = np.max(np.abs(kin) + np.abs(np.asarray(vir))) / V
+    assert np.max(np.abs(np.asarray(t["press"]) - (kin + np.asarray(vir)) / V)) <= 1e-9 * scale
+    engine.remove(sid)
