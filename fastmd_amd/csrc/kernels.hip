@@ -1486,96 +1486,120 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
             }
         }
         const int hdr = (terms & 6) ? d.bhdr[i] : 0;
-        if (terms & 2) {
-            const int nb = hdr & 0xFF;
-            const int* bt = d.bnd + (size_t)(i / kTile) * d.bcap * kTile + (i % kTile);
-            for (int b = 0; b < nb; b++) {
-                const int e = bt[b * kTile];
-                const int4 pj = d.ipos[e & kSlotMask];
-                const float4 bp = d.bparm[(unsigned)e >> kSlotBits];
-                const float dx = fix_diff(pi.x, pj.x, ulen);
-                const float dy = fix_diff(pi.y, pj.y, ulen);
-                const float dz = fix_diff(pi.z, pj.z, ulen);
-                float r2 = sq3(dx, dy, dz);
-                const float R02 = bp.y * bp.y;
-                // a bond at or beyond R0 is a failure of the run, except during the
-                // capped soft push-off, where the clamp below keeps it finite
-                if (!(r2 < R02) && !(d.fcap > 0.0f)) err |= ERR_STRETCH;
-                r2 = fminf(fmaxf(r2, 1e-6f), R02 * 0.9999f);  // src/force/fene.cu:33
-                // divisions as hardware reciprocals (1 ulp), as the reference's
-                // fast-math build does; PE and virial only where observed
-                float ff = -bp.x * R02 * __builtin_amdgcn_rcpf(R02 - r2);
-                const float sig2 = bp.w * bp.w;
-                const bool wca = r2 < sig2 * 1.2599210498948732f;
-                float sr6 = 0.0f, sr12 = 0.0f;
-                if (wca) {
-                    const float r2inv = __builtin_amdgcn_rcpf(r2);
-                    const float sr2 = sig2 * r2inv;
-                    sr6 = sr2 * sr2 * sr2;
-                    sr12 = sr6 * sr6;
-                    ff += 24.0f * bp.z * r2inv * (2.0f * sr12 - sr6);
-                }
-                fx += ff * dx; fy += ff * dy; fz += ff * dz;
-                if (obs) {
-                    float bpe = -0.5f * bp.x * R02 * logf(1.0f - r2 / R02);
-                    if (wca) bpe += 4.0f * bp.z * (sr12 - sr6) + bp.z;
-                    const float hx = 0.5f * ff * dx, hy = 0.5f * ff * dy, hz = 0.5f * ff * dz;
-                    pe += 0.5f * bpe;
-                    w0 += hx * dx; w1 += hx * dy; w2 += hx * dz;
-                    w3 += hy * dy; w4 += hy * dz; w5 += hz * dz;
-                }
+        // The bonded terms in the order bonds, then angles, each over its rows;
+        // the entries and partner positions of the first kBondAhead bonds and
+        // kAngleAhead angles (all of a Kremer-Grest bead's) are requested
+        // together, so their latencies overlap instead of forming one chain per
+        // term.
+        const int nb = (terms & 2) ? (hdr & 0xFF) : 0;
+        const int na = (terms & 4) ? ((hdr >> 8) & 0xFF) : 0;
+        const int* bt = d.bnd + (size_t)(i / kTile) * d.bcap * kTile + (i % kTile);
+        const int2* at = d.ang + (size_t)(i / kTile) * d.acap * kTile + (i % kTile);
+        auto bond_term = [&](int e, int4 pj) __attribute__((always_inline)) {
+            const float4 bp = d.bparm[(unsigned)e >> kSlotBits];
+            const float dx = fix_diff(pi.x, pj.x, ulen);
+            const float dy = fix_diff(pi.y, pj.y, ulen);
+            const float dz = fix_diff(pi.z, pj.z, ulen);
+            float r2 = sq3(dx, dy, dz);
+            const float R02 = bp.y * bp.y;
+            // a bond at or beyond R0 is a failure of the run, except during the
+            // capped soft push-off, where the clamp below keeps it finite
+            if (!(r2 < R02) && !(d.fcap > 0.0f)) err |= ERR_STRETCH;
+            r2 = fminf(fmaxf(r2, 1e-6f), R02 * 0.9999f);  // src/force/fene.cu:33
+            // divisions as hardware reciprocals (1 ulp), as the reference's
+            // fast-math build does; PE and virial only where observed
+            float ff = -bp.x * R02 * __builtin_amdgcn_rcpf(R02 - r2);
+            const float sig2 = bp.w * bp.w;
+            const bool wca = r2 < sig2 * 1.2599210498948732f;
+            float sr6 = 0.0f, sr12 = 0.0f;
+            if (wca) {
+                const float r2inv = __builtin_amdgcn_rcpf(r2);
+                const float sr2 = sig2 * r2inv;
+                sr6 = sr2 * sr2 * sr2;
+                sr12 = sr6 * sr6;
+                ff += 24.0f * bp.z * r2inv * (2.0f * sr12 - sr6);
             }
+            fx += ff * dx; fy += ff * dy; fz += ff * dz;
+            if (obs) {
+                float bpe = -0.5f * bp.x * R02 * logf(1.0f - r2 / R02);
+                if (wca) bpe += 4.0f * bp.z * (sr12 - sr6) + bp.z;
+                const float hx = 0.5f * ff * dx, hy = 0.5f * ff * dy, hz = 0.5f * ff * dz;
+                pe += 0.5f * bpe;
+                w0 += hx * dx; w1 += hx * dy; w2 += hx * dz;
+                w3 += hy * dy; w4 += hy * dz; w5 += hz * dz;
+            }
+        };
+        auto angle_term = [&](int2 e, int4 q1, int4 q2) __attribute__((always_inline)) {
+            const int role = (unsigned)e.x >> kSlotBits;
+            const float2 ap = d.aparm[(unsigned)e.y >> kSlotBits];
+            // angle atoms (A, B=center, C) from this atom's role
+            // (assignments, not conditionals of two lvalues: a select of
+            // addresses would keep pi in scratch memory)
+            int4 pa = q1, pb = q1, pc = q2;
+            if (role == 0) pa = pi;
+            if (role == 1) pb = pi;
+            else if (role == 2) pb = q2;
+            if (role == 2) pc = pi;
+            const float ax = fix_diff(pa.x, pb.x, ulen);
+            const float ay = fix_diff(pa.y, pb.y, ulen);
+            const float az = fix_diff(pa.z, pb.z, ulen);
+            const float bx = fix_diff(pc.x, pb.x, ulen);
+            const float by = fix_diff(pc.y, pb.y, ulen);
+            const float bz = fix_diff(pc.z, pb.z, ulen);
+            const float ra2 = ax * ax + ay * ay + az * az;
+            const float rb2 = bx * bx + by * by + bz * bz;
+            const float cxp = ay * bz - az * by, cyp = az * bx - ax * bz, czp = ax * by - ay * bx;
+            const float cr = sqrtf(cxp * cxp + cyp * cyp + czp * czp);
+            const float dot = ax * bx + ay * by + az * bz;
+            const float theta = angle_atan2(cr, dot);
+            const float inv_ab = rsqrtf(ra2) * rsqrtf(rb2);
+            const float c = dot * inv_ab;
+            const float s = fmaxf(cr * inv_ab, 1e-6f);
+            const float dth = theta - ap.y;
+            const float pref = 2.0f * ap.x * dth * __builtin_amdgcn_rcpf(s);
+            const float ia2 = __builtin_amdgcn_rcpf(ra2), ib2 = __builtin_amdgcn_rcpf(rb2);
+            const float fix = pref * (bx * inv_ab - c * ax * ia2);
+            const float fiy = pref * (by * inv_ab - c * ay * ia2);
+            const float fiz = pref * (bz * inv_ab - c * az * ia2);
+            const float fkx = pref * (ax * inv_ab - c * bx * ib2);
+            const float fky = pref * (ay * inv_ab - c * by * ib2);
+            const float fkz = pref * (az * inv_ab - c * bz * ib2);
+            if (role == 0) { fx += fix; fy += fiy; fz += fiz; }
+            else if (role == 2) { fx += fkx; fy += fky; fz += fkz; }
+            else { fx -= fix + fkx; fy -= fiy + fky; fz -= fiz + fkz; }
+            if (obs) {
+                const float third = 1.0f / 3.0f;
+                pe += ap.x * dth * dth * third;
+                w0 += (fix * ax + fkx * bx) * third;
+                w1 += (fix * ay + fkx * by) * third;
+                w2 += (fix * az + fkx * bz) * third;
+                w3 += (fiy * ay + fky * by) * third;
+                w4 += (fiy * az + fky * bz) * third;
+                w5 += (fiz * az + fkz * bz) * third;
+            }
+        };
+        constexpr int kBondAhead = 2, kAngleAhead = 3;
+        const int eb0 = (nb > 0) ? bt[0] : 0, eb1 = (nb > 1) ? bt[kTile] : 0;
+        const int2 ea0 = (na > 0) ? at[0] : make_int2(0, 0), ea1 = (na > 1) ? at[kTile] : make_int2(0, 0),
+                   ea2 = (na > 2) ? at[2 * kTile] : make_int2(0, 0);
+        int4 pb0 = pi, pb1 = pi, qa0 = pi, qc0 = pi, qa1 = pi, qc1 = pi, qa2 = pi, qc2 = pi;
+        if (nb > 0) pb0 = d.ipos[eb0 & kSlotMask];
+        if (nb > 1) pb1 = d.ipos[eb1 & kSlotMask];
+        if (na > 0) { qa0 = d.ipos[ea0.x & kSlotMask]; qc0 = d.ipos[ea0.y & kSlotMask]; }
+        if (na > 1) { qa1 = d.ipos[ea1.x & kSlotMask]; qc1 = d.ipos[ea1.y & kSlotMask]; }
+        if (na > 2) { qa2 = d.ipos[ea2.x & kSlotMask]; qc2 = d.ipos[ea2.y & kSlotMask]; }
+        if (nb > 0) bond_term(eb0, pb0);
+        if (nb > 1) bond_term(eb1, pb1);
+        for (int b = kBondAhead; b < nb; b++) {
+            const int e = bt[b * kTile];
+            bond_term(e, d.ipos[e & kSlotMask]);
         }
-        if (terms & 4) {
-            const int na = (hdr >> 8) & 0xFF;
-            const int2* at = d.ang + (size_t)(i / kTile) * d.acap * kTile + (i % kTile);
-            for (int a = 0; a < na; a++) {
-                const int2 e = at[a * kTile];
-                const int role = (unsigned)e.x >> kSlotBits;
-                const float2 ap = d.aparm[(unsigned)e.y >> kSlotBits];
-                const int4 q1 = d.ipos[e.x & kSlotMask], q2 = d.ipos[e.y & kSlotMask];
-                // angle atoms (A, B=center, C) from this atom's role
-                const int4 pa = (role == 0) ? pi : q1;
-                const int4 pb = (role == 1) ? pi : ((role == 0) ? q1 : q2);
-                const int4 pc = (role == 2) ? pi : q2;
-                const float ax = fix_diff(pa.x, pb.x, ulen);
-                const float ay = fix_diff(pa.y, pb.y, ulen);
-                const float az = fix_diff(pa.z, pb.z, ulen);
-                const float bx = fix_diff(pc.x, pb.x, ulen);
-                const float by = fix_diff(pc.y, pb.y, ulen);
-                const float bz = fix_diff(pc.z, pb.z, ulen);
-                const float ra2 = ax * ax + ay * ay + az * az;
-                const float rb2 = bx * bx + by * by + bz * bz;
-                const float cxp = ay * bz - az * by, cyp = az * bx - ax * bz, czp = ax * by - ay * bx;
-                const float cr = sqrtf(cxp * cxp + cyp * cyp + czp * czp);
-                const float dot = ax * bx + ay * by + az * bz;
-                const float theta = angle_atan2(cr, dot);
-                const float inv_ab = rsqrtf(ra2) * rsqrtf(rb2);
-                const float c = dot * inv_ab;
-                const float s = fmaxf(cr * inv_ab, 1e-6f);
-                const float dth = theta - ap.y;
-                const float pref = 2.0f * ap.x * dth * __builtin_amdgcn_rcpf(s);
-                const float ia2 = __builtin_amdgcn_rcpf(ra2), ib2 = __builtin_amdgcn_rcpf(rb2);
-                const float fix = pref * (bx * inv_ab - c * ax * ia2);
-                const float fiy = pref * (by * inv_ab - c * ay * ia2);
-                const float fiz = pref * (bz * inv_ab - c * az * ia2);
-                const float fkx = pref * (ax * inv_ab - c * bx * ib2);
-                const float fky = pref * (ay * inv_ab - c * by * ib2);
-                const float fkz = pref * (az * inv_ab - c * bz * ib2);
-                if (role == 0) { fx += fix; fy += fiy; fz += fiz; }
-                else if (role == 2) { fx += fkx; fy += fky; fz += fkz; }
-                else { fx -= fix + fkx; fy -= fiy + fky; fz -= fiz + fkz; }
-                if (obs) {
-                    const float third = 1.0f / 3.0f;
-                    pe += ap.x * dth * dth * third;
-                    w0 += (fix * ax + fkx * bx) * third;
-                    w1 += (fix * ay + fkx * by) * third;
-                    w2 += (fix * az + fkx * bz) * third;
-                    w3 += (fiy * ay + fky * by) * third;
-                    w4 += (fiy * az + fky * bz) * third;
-                    w5 += (fiz * az + fkz * bz) * third;
-                }
-            }
+        if (na > 0) angle_term(ea0, qa0, qc0);
+        if (na > 1) angle_term(ea1, qa1, qc1);
+        if (na > 2) angle_term(ea2, qa2, qc2);
+        for (int a = kAngleAhead; a < na; a++) {
+            const int2 e = at[a * kTile];
+            angle_term(e, d.ipos[e.x & kSlotMask], d.ipos[e.y & kSlotMask]);
         }
         // Langevin steps followed by another step of the run also give that
         // step's pre-force half kick (the same force, the same operations as
