@@ -54,6 +54,27 @@ class Thermo(C.Structure):
                 "press": list(self.press), "L": self.L}
 
 
+MAX_CHAIN = 10
+
+
+class Thermostat(C.Structure):
+    _fields_ = [("ensemble", C.c_int32), ("M", C.c_int32), ("step", C.c_int64),
+                ("Q1", C.c_double), ("Qr", C.c_double), ("W", C.c_double), ("V0", C.c_double),
+                ("eps", C.c_double), ("veps", C.c_double),
+                ("xi", C.c_double * MAX_CHAIN), ("vxi", C.c_double * MAX_CHAIN), ("ke_carry", C.c_double)]
+
+    def as_dict(self):
+        return {"ensemble": self.ensemble, "M": self.M, "step": self.step, "Q1": self.Q1, "Qr": self.Qr,
+                "W": self.W, "V0": self.V0, "eps": self.eps, "veps": self.veps,
+                "xi": np.array(self.xi[:]), "vxi": np.array(self.vxi[:]), "ke_carry": self.ke_carry}
+
+
+class ChainRun(C.Structure):
+    _fields_ = [("M", C.c_int32), ("natoms", C.c_int32), ("carry", C.c_int32), ("n", C.c_int32),
+                ("Q1", C.c_double), ("Qr", C.c_double), ("kT", C.c_double), ("half_dt", C.c_double),
+                ("ke", _dp)]
+
+
 class ConfExtra(C.Structure):
     _fields_ = [("thermo_on", C.c_int32), ("thermo_freq", C.c_int32), ("thermo_file", C.c_char * 256),
                 ("restart_freq", C.c_int32), ("restart_file", C.c_char * 256),
@@ -104,6 +125,7 @@ SIGNATURES = [
     ("fmd_step_plain", C.c_int32, [C.c_void_p, C.c_int32, C.c_int64]),
     ("fmd_system_step", C.c_int64, [C.c_void_p, C.c_int32]),
     ("fmd_system_natoms", C.c_int32, [C.c_void_p, C.c_int32]),
+    ("fmd_system_error", C.c_int32, [C.c_void_p, C.c_int32]),
     ("fmd_get_state", C.c_int32, [C.c_void_p, C.c_int32, _fp, _fp, _ip]),
     ("fmd_get_box", C.c_int32, [C.c_void_p, C.c_int32, _dp]),
     ("fmd_get_state_box", C.c_int32, [C.c_void_p, C.c_int32, _dp]),
@@ -117,6 +139,8 @@ SIGNATURES = [
     ("fmd_get_thermo", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(Thermo)]),
     ("fmd_drain_thermo", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(Thermo), C.c_int32, _ip,
                                      C.POINTER(C.c_int64)]),
+    ("fmd_get_thermostat", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(Thermostat)]),
+    ("fmd_nh_chain_run", C.c_int32, [C.c_void_p, C.POINTER(ChainRun), _dp, _dp, _dp]),
     ("fmd_system_rebuilds", C.c_int64, [C.c_void_p, C.c_int32]),
     ("fmd_autotune", C.c_int32, [C.c_void_p, _ip, C.c_int32, C.c_int32]),
     ("fmd_tuned_block", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32]),
@@ -402,6 +426,10 @@ class Engine:
     def steps_done(self, sid):
         return lib().fmd_system_step(self.h, sid)
 
+    def error(self, sid):
+        """The error code that stopped the system (OK while healthy)."""
+        return lib().fmd_system_error(self.h, sid)
+
     def rebuilds(self, sid):
         return lib().fmd_system_rebuilds(self.h, sid)
 
@@ -470,6 +498,27 @@ class Engine:
         t = Thermo()
         self._chk(lib().fmd_get_thermo(self.h, sid, C.byref(t)))
         return t.as_dict()
+
+    def thermostat(self, sid):
+        """Nose-Hoover chain and barostat state (FP64) after the last step."""
+        t = Thermostat()
+        self._chk(lib().fmd_get_thermostat(self.h, sid, C.byref(t)))
+        return t.as_dict()
+
+    def nh_chain_run(self, M, Q1, Qr, xi, vxi, ke, kT, natoms, half_dt, carry=False):
+        """The device chain applied to a stream of kinetic energies, state
+        carried from draw to draw; returns (scales, xi, vxi)."""
+        ke = np.ascontiguousarray(ke, np.float64).reshape(-1)
+        xi = np.ascontiguousarray(np.resize(np.asarray(xi, np.float64), MAX_CHAIN))
+        vxi = np.ascontiguousarray(np.resize(np.asarray(vxi, np.float64), MAX_CHAIN))
+        sc = np.zeros(len(ke) * (2 if carry else 1), np.float64)
+        c = ChainRun()
+        c.M, c.natoms, c.carry, c.n = M, natoms, 1 if carry else 0, len(ke)
+        c.Q1, c.Qr, c.kT, c.half_dt = Q1, Qr, kT, half_dt
+        c.ke = _ptr(ke, C.c_double)
+        self._chk(lib().fmd_nh_chain_run(self.h, C.byref(c), _ptr(xi, C.c_double), _ptr(vxi, C.c_double),
+                                         _ptr(sc, C.c_double)))
+        return sc, xi, vxi
 
     def drain_thermo(self, sid, cap=4096):
         buf = (Thermo * cap)()

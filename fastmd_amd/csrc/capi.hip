@@ -141,6 +141,7 @@ int32_t fmd_step_plain(fmd_engine* e, int32_t sys, int64_t nsteps) {
     return e ? e->e->step(&sys, 1, nsteps, false) : FMD_ERR_ARG;
 }
 int64_t fmd_system_step(fmd_engine* e, int32_t sys) { return e ? e->e->step_count(sys) : -1; }
+int32_t fmd_system_error(fmd_engine* e, int32_t sys) { return e ? e->e->replica_error(sys) : FMD_ERR_ARG; }
 int32_t fmd_system_natoms(fmd_engine* e, int32_t sys) {
     fmd::Replica* r = e ? e->e->rep(sys) : nullptr;
     return r ? r->natoms : -1;
@@ -175,6 +176,12 @@ int32_t fmd_get_state_box(fmd_engine* e, int32_t sys, double* L) {
 }
 int32_t fmd_get_thermo(fmd_engine* e, int32_t sys, fmd_thermo* out) {
     return e ? e->e->get_thermo(sys, out) : FMD_ERR_ARG;
+}
+int32_t fmd_get_thermostat(fmd_engine* e, int32_t sys, fmd_thermostat* out) {
+    return e ? e->e->get_thermostat(sys, out) : FMD_ERR_ARG;
+}
+int32_t fmd_nh_chain_run(fmd_engine* e, const fmd_chain_run* c, double* xi, double* vxi, double* scales) {
+    return e ? e->e->nh_chain_run(c, xi, vxi, scales) : FMD_ERR_ARG;
 }
 int32_t fmd_drain_thermo(fmd_engine* e, int32_t sys, fmd_thermo* out, int32_t cap, int32_t* n,
                          int64_t* lost) {

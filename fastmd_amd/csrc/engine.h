@@ -89,6 +89,8 @@ class Engine {
     int32_t get_state(int32_t sys, float* pos4, float* vel3, int32_t* img3);
     int32_t get_box(int32_t sys, double* L, bool of_positions = false);
     int32_t get_thermo(int32_t sys, fmd_thermo* out);
+    int32_t get_thermostat(int32_t sys, fmd_thermostat* out);
+    int32_t nh_chain_run(const fmd_chain_run* c, double* xi, double* vxi, double* scales);
     int32_t get_rg(int32_t sys, double* rg, double* per_chain, int32_t cap, int32_t* nchains);
     int32_t snapshot_size(int32_t sys, uint64_t* bytes);
     int32_t snapshot(int32_t sys, void* buf, uint64_t cap);

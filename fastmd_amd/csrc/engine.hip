@@ -1258,6 +1258,69 @@ int32_t Engine::get_thermo(int32_t sys, fmd_thermo* out) {
     return rc;
 }
 
+int32_t Engine::get_thermostat(int32_t sys, fmd_thermostat* out) {
+    Replica* r = rep(sys);
+    if (!r || !out) return fail(FMD_ERR_ARG, "bad arguments");
+    RepState s;
+    int32_t rc = read_state(*r, s);
+    if (rc) return rc;
+    *out = fmd_thermostat{};
+    out->ensemble = r->ensemble;
+    out->M = s.M;
+    out->step = s.step;
+    out->Q1 = s.Q1;
+    out->Qr = s.Qr;
+    out->W = s.W;
+    out->V0 = s.V0;
+    out->eps = s.eps;
+    out->veps = s.veps;
+    for (int k = 0; k < kMaxChain && k < FMD_MAX_CHAIN; k++) {
+        out->xi[k] = s.xi[k];
+        out->vxi[k] = s.vxi[k];
+    }
+    out->ke_carry = s.ke_carry;
+    return FMD_OK;
+}
+
+int32_t Engine::nh_chain_run(const fmd_chain_run* c, double* xi, double* vxi, double* scales) {
+    if (!c || !xi || !vxi || !scales || (c->n > 0 && !c->ke) || c->n < 0 || c->M < 1 ||
+        c->M > FMD_MAX_CHAIN || c->natoms < 1)
+        return fail(FMD_ERR_ARG, "bad chain-run arguments");
+    if (c->n == 0) return FMD_OK;
+    const size_t nsc = (size_t)c->n * (c->carry ? 2 : 1);
+    RepState h{};
+    h.M = c->M;
+    h.Q1 = c->Q1;
+    h.Qr = c->Qr;
+    for (int k = 0; k < kMaxChain; k++) {
+        h.xi[k] = xi[k];
+        h.vxi[k] = vxi[k];
+    }
+    void* buf = nullptr;
+    const size_t bytes = sizeof(RepState) + (size_t)c->n * 8 + nsc * 8;
+    int32_t rc = hip_check(hipMalloc(&buf, bytes), "hipMalloc(chain run)");
+    if (rc) return rc;
+    RepState* dst = (RepState*)buf;
+    double* dke = (double*)((char*)buf + sizeof(RepState));
+    double* dsc = dke + c->n;
+    rc = hip_check(hipMemcpyAsync(dst, &h, sizeof(RepState), hipMemcpyHostToDevice, stream_), "copy chain");
+    if (!rc) rc = hip_check(hipMemcpyAsync(dke, c->ke, (size_t)c->n * 8, hipMemcpyHostToDevice, stream_), "copy ke");
+    if (!rc) {
+        launch_nh_chain_run(dst, dke, c->n, c->carry ? 1 : 0, c->kT, c->natoms, c->half_dt, dsc, stream_);
+        rc = hip_check(hipGetLastError(), "chain run launch");
+    }
+    if (!rc) rc = hip_check(hipMemcpyAsync(scales, dsc, nsc * 8, hipMemcpyDeviceToHost, stream_), "copy scales");
+    if (!rc) rc = hip_check(hipMemcpyAsync(&h, dst, sizeof(RepState), hipMemcpyDeviceToHost, stream_), "copy chain");
+    if (!rc) rc = sync();
+    (void)hipFree(buf);
+    if (rc) return rc;
+    for (int k = 0; k < kMaxChain; k++) {
+        xi[k] = h.xi[k];
+        vxi[k] = h.vxi[k];
+    }
+    return FMD_OK;
+}
+
 int32_t Engine::drain_thermo(int32_t sys, fmd_thermo* out, int32_t cap, int32_t* n, int64_t* lost) {
     Replica* r = rep(sys);
     if (!r || !n) return fail(FMD_ERR_ARG, "bad arguments");

@@ -42,6 +42,9 @@ void launch_rg(const RepDesc* d, int nchains, double* out, hipStream_t s);
 // One stress sample (xx, xy, xz, yy, yz, zz), passed by value.
 struct Sample6 { double v[6]; };
 void launch_corr_push(const RepDesc* d, const Sample6& x, hipStream_t s);
+// Chain applications of nh_chain on a stream of kinetic energies (one lane).
+void launch_nh_chain_run(RepState* st, const double* ke, int n, int carry, double kT, int natoms,
+                         double half_dt, double* scales, hipStream_t s);
 // Per replica of the list: (err, active, step lo, step hi).
 void launch_status(const Launch& l, int4* out);
 // Reference cell ids / Morton keys by global id for one replica.

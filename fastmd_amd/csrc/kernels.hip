@@ -264,7 +264,7 @@ __global__ void __launch_bounds__(256) k_pre(const RepDesc* __restrict__ descs,
     const Tile t = tiles[w];
     const RepDesc& d = descs[t.rep];
     RepState* st = d.st;
-    if (!st->active) return;
+    if (!st->active || st->err) return;
     const int i = t.lt * kTile + lane_id();
     float d2 = 0.0f;
     int bad = 0;
@@ -1710,6 +1710,27 @@ __global__ void __launch_bounds__(128) k_corr_push(const RepDesc* __restrict__ d
     corr_push(d->corr, smp);
 }
 
+// The step's chain applications on a stream of kinetic energies, one lane,
+// state carried from draw to draw (the device chain of nh_propagate_chain_kernel,
+// src/integrate/nose_hoover.cu:86-178, as tests/test_nose_hoover.cu:343-439
+// drives it).  carry = 0: one application per draw with KE ke[t]; carry = 1:
+// two per draw, the post-force one with ke[t] and then the next pre-force one
+// with the scaled kinetic energy ke[t] s^2 it leaves (chain_KE_carry).
+__global__ void __launch_bounds__(64) k_nh_chain_run(RepState* st, const double* __restrict__ ke, int n,
+                                                     int carry, double kT, int natoms, double half_dt,
+                                                     double* __restrict__ scales) {
+    if (threadIdx.x != 0) return;
+    for (int t = 0; t < n; t++) {
+        const double s = nh_chain(st, ke[t], kT, natoms, half_dt);
+        if (carry) {
+            scales[2 * t] = s;
+            scales[2 * t + 1] = nh_chain(st, ke[t] * s * s, kT, natoms, half_dt);
+        } else {
+            scales[t] = s;
+        }
+    }
+}
+
 // Scalar tail of the per-replica boundary (thread 0): the post-force chain /
 // barostat of this step, start-of-step scalars of the next, thermo record,
 // step counter.  Returns 1 when this step's pressure tensor is a correlator
@@ -1792,7 +1813,9 @@ __device__ int boundary_tail(const RepDesc& d, RepState* st, int mode, const dou
         for (int c = 0; c < 6; c++) smp[c] = (st->kin[c] + W[c]) / V;
     // the force pass of this step kicked ahead exactly when another step follows
     st->kicked = (d.ensemble == 0 && cur < st->stop) ? 1 : 0;
-    if (cur >= st->stop) {
+    // a neighbor overflow (list build) or an overstretched bond (force pass)
+    // found during this step stops the replica at the end of it
+    if (cur >= st->stop || st->err) {
         st->active = 0;
         st->pre_ready = 0;
     } else {
@@ -1892,6 +1915,11 @@ void launch_rg(const RepDesc* d, int nchains, double* out, hipStream_t s) {
 }
 void launch_corr_push(const RepDesc* d, const Sample6& x, hipStream_t s) {
     k_corr_push<<<1, 128, 0, s>>>(d, x);
+}
+void launch_nh_chain_run(RepState* st, const double* ke, int n, int carry, double kT, int natoms,
+                         double half_dt, double* scales, hipStream_t s) {
+    if (n <= 0) return;
+    k_nh_chain_run<<<1, 64, 0, s>>>(st, ke, n, carry, kT, natoms, half_dt, scales);
 }
 void launch_status(const Launch& l, int4* out) {
     if (l.nr <= 0) return;

@@ -202,6 +202,21 @@ int32_t fmd_step(fmd_engine* e, const int32_t* sys, int32_t nsys, int64_t nsteps
 int32_t fmd_step_plain(fmd_engine* e, int32_t sys, int64_t nsteps);
 /* Steps completed so far. */
 int64_t fmd_system_step(fmd_engine* e, int32_t sys);
+/* The error that stopped a system (FMD_OK while it is healthy).  No reference
+ * counterpart: the reference truncates the neighbor list
+ * (src/neighbor/verlet_list.cu:281-285), clamps an overstretched FENE bond
+ * (src/force/fene.cu:33) and carries NaNs on.  Here a system stops on its own,
+ * the others of its batch continue unchanged:
+ *   FMD_ERR_NAN: a non-finite position or velocity after the position update
+ *     of step s; the system stops before that step's list build and forces
+ *     (fmd_system_step = s - 1, the state as that update left it);
+ *   FMD_ERR_NBR_OVERFLOW (list build of step s) and FMD_ERR_STRETCH (FENE
+ *     r >= R0 in the forces of step s): the system stops at the end of step s
+ *     (fmd_system_step = s).  Within a Langevin run the velocities of such a
+ *     state already hold the next step's first half kick, which the force
+ *     pass applies together with the forces.
+ * A stopped system is not stepped again; it can be read and destroyed. */
+int32_t fmd_system_error(fmd_engine* e, int32_t sys);
 /* Bead count of a system (-1 if there is no such system). */
 int32_t fmd_system_natoms(fmd_engine* e, int32_t sys);
 /* State by global id: pos4 (x, y, z, type bits), vel3, image3; any may be NULL. */
@@ -247,6 +262,42 @@ int32_t fmd_get_thermo(fmd_engine* e, int32_t sys, fmd_thermo* out);
  * drain (oldest first); *n = number written; *lost = records overwritten. */
 int32_t fmd_drain_thermo(fmd_engine* e, int32_t sys, fmd_thermo* out, int32_t cap, int32_t* n,
                          int64_t* lost);
+/* Extended-system state of the Nose-Hoover chain and barostat (FP64), as
+ * NoseHooverState holds it on the host (src/integrate/nose_hoover.cuh:14-45):
+ * chain masses Q1 = Nf kT Tdamp^2 and Qr = kT Tdamp^2, barostat mass
+ * W = (Nf + 3) kT Pdamp^2, V0, eps (V = V0 exp(3 eps)), v_eps, xi[k], v_xi[k]
+ * (src/integrate/nose_hoover.cu:309-346).  Read between runs it is the state
+ * after the last step's post-force chain / barostat update, which with the
+ * velocities and energies makes up the conserved quantity of
+ * tests/test_nose_hoover.cu:19-40.  ke_carry is the kinetic energy the next
+ * pre-force chain application will use (chain_KE_carry). */
+#define FMD_MAX_CHAIN 10
+typedef struct fmd_thermostat {
+    int32_t ensemble, M;
+    int64_t step;
+    double Q1, Qr, W, V0;
+    double eps, veps;
+    double xi[FMD_MAX_CHAIN], vxi[FMD_MAX_CHAIN];
+    double ke_carry;
+} fmd_thermostat;
+int32_t fmd_get_thermostat(fmd_engine* e, int32_t sys, fmd_thermostat* out);
+
+/* The device chain on a stream of kinetic energies: replaces
+ * nh_propagate_chain_kernel (src/integrate/nose_hoover.cu:86-178) as its
+ * GPUChainMatchesHost test drives it (tests/test_nose_hoover.cu:343-439).
+ * Runs the step's FP64 chain code on the device, one draw after another with
+ * the state carried: carry = 0 applies the chain once per draw with KE ke[t]
+ * (scales[t]); carry = 1 applies it twice per draw, with ke[t] (the post-force
+ * call, scales[2t]) and then with the KE that call leaves, ke[t] s^2 (the next
+ * pre-force call on chain_KE_carry, scales[2t+1]).  xi and vxi (FMD_MAX_CHAIN
+ * each) are the initial state in and the final state out. */
+typedef struct fmd_chain_run {
+    int32_t M, natoms, carry, n;
+    double Q1, Qr, kT, half_dt;
+    const double* ke;
+} fmd_chain_run;
+int32_t fmd_nh_chain_run(fmd_engine* e, const fmd_chain_run* c, double* xi, double* vxi, double* scales);
+
 /* Neighbor-list rebuilds performed so far. */
 int64_t fmd_system_rebuilds(fmd_engine* e, int32_t sys);
 

@@ -649,11 +649,34 @@ Real nh_chain_step(NHChain<Real>& c, Real KE, Real kT, int natoms, Real half_dt)
     return scale;
 }
 
-// Conserved quantity of the chain (tests/test_nose_hoover.cu:19-40).
+// Chain energy of the reference's conserved quantity (tests/test_nose_hoover.cu:19-40):
+// 1/2 Q1 v0^2 + Nf kT xi0 + sum_{k>=1} (1/2 Qr vk^2 + kT xik).
 template <typename Real> Real nh_chain_energy(const NHChain<Real>& c, Real kT, int natoms) {
     Real Nf = (Real)3 * natoms;
     Real e = (Real)0.5 * c.Q1 * c.vxi[0] * c.vxi[0] + Nf * kT * c.xi[0];
     for (int k = 1; k < c.M; k++) e += (Real)0.5 * c.Qr * c.vxi[k] * c.vxi[k] + kT * c.xi[k];
+    return e;
+}
+
+// Chain energy that the dynamics of nh_chain_step actually conserves.
+// Within one application the backward sweep gives the elements k >= 1 no
+// force term (their acceleration array starts at zero and is only filled in
+// the forward sweep, src/integrate/nose_hoover.cu:365,385-393,431-441), so in
+// the small-dt limit element k >= 1 obeys
+//   Qr dvk/dt = 1/2 (Q_{k-1} v_{k-1}^2 - kT) - Qr vk v_{k+1}
+// with half the force of a textbook chain, while element 0 obeys
+//   Q1 dv0/dt = (2 KE - Nf kT) - Q1 v0 v1  and particles dv/dt = F/m - v0 v.
+// d/dt of KE + U + 1/2 Q1 v0^2 + Nf kT xi0 + sum_k (a_k 1/2 Qr vk^2 + b_k xik)
+// vanishes term by term iff a_1 = 2, a_k = 2 a_{k-1} and b_k = a_k kT / 2, i.e.
+// weights a_k = 2^k, b_k = 2^(k-1) kT.  For M = 1 both forms are equal.
+template <typename Real> Real nh_chain_energy_dyn(const NHChain<Real>& c, Real kT, int natoms) {
+    Real Nf = (Real)3 * natoms;
+    Real e = (Real)0.5 * c.Q1 * c.vxi[0] * c.vxi[0] + Nf * kT * c.xi[0];
+    Real a = (Real)1;
+    for (int k = 1; k < c.M; k++) {
+        a *= (Real)2;
+        e += a * ((Real)0.5 * c.Qr * c.vxi[k] * c.vxi[k]) + (Real)0.5 * a * kT * c.xi[k];
+    }
     return e;
 }
 

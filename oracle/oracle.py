@@ -75,7 +75,12 @@ def lib():
         L.orc_nh_chain.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, C.c_double,
                                    C.c_double, C.c_int, C.c_double]
         L.orc_md_run.restype = C.c_int
-        L.orc_md_run.argtypes = [C.POINTER(OrcSys), C.c_long, _dp]
+        L.orc_md_run.argtypes = [C.POINTER(OrcSys), C.c_long, _dp, _dp]
+        L.orc_lj_cut_energy.restype = C.c_double
+        L.orc_lj_cut_energy.argtypes = [_fp, C.c_int, C.c_int, _fp, C.c_float, C.c_float, _ip, C.c_int]
+        L.orc_nh_chain_energy.restype = C.c_double
+        L.orc_nh_chain_energy.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, C.c_double,
+                                          C.c_int]
         L.orc_thermo.argtypes = [C.c_int, _fp, _fp, _fp, _dp, _dp, C.c_float, _dp]
         L.orc_cpu_md_langevin.restype = C.c_long
         L.orc_cpu_md_langevin.argtypes = [C.c_int, _fp, _fp, _ip, _fp, C.c_float, C.c_float, C.c_float,
@@ -286,8 +291,14 @@ def corr_run(samples):
     return c, n
 
 
-def md_run(sysd, nsteps):
-    """Runs the oracle MD driver on a dict describing the system (see tests)."""
+EXT = 40  # doubles per extended-system record of md_run(..., ext=True)
+
+
+def md_run(sysd, nsteps, ext=False):
+    """Runs the oracle MD driver on a dict describing the system (see tests).
+    Returns (pos4, vel3, img3, L, thermo[nsteps, 11]) and, with ext=True, the
+    extended-system records [nsteps + 1, EXT] (row 0 = the initial state; the
+    column meanings are listed at orc_md_run in oracle_capi.cpp)."""
     n = sysd["pos4"].shape[0]
     keep = {k: (f32(v) if v.dtype.kind == "f" else i32(v)) for k, v in sysd.items() if isinstance(v, np.ndarray)}
     keep["pos4"] = keep["pos4"].copy()
@@ -309,8 +320,60 @@ def md_run(sysd, nsteps):
     s.chain, s.seed, s.nsteps_total = sysd.get("chain", 3), sysd.get("seed", 42), sysd.get("nsteps_total", nsteps)
     s.clamp_lj, s.clamp_fene, s.fcap = 1, 1, sysd.get("fcap", 0.0)
     thermo = np.zeros((nsteps, 11), np.float64)
-    lib().orc_md_run(C.byref(s), nsteps, _p(thermo, C.c_double))
+    xr = np.zeros((nsteps + 1, EXT), np.float64) if ext else None
+    lib().orc_md_run(C.byref(s), nsteps, _p(thermo, C.c_double), _p(xr, C.c_double) if ext else None)
+    if ext:
+        return keep["pos4"], keep["vel3"], keep["img3"], float(s.L), thermo, xr
     return keep["pos4"], keep["vel3"], keep["img3"], float(s.L), thermo
+
+
+def lj_cut_energy(pos4, L, lj, bonds=None, ntypes=1, rc=2.5):
+    """Sum of the LJ energy at the cutoff over the non-excluded pairs inside
+    it: U - lj_cut_energy is the energy of the cut-and-shifted potential."""
+    pos4 = f32(pos4).reshape(-1, 4)
+    b = i32(np.zeros((0, 2)) if bonds is None else bonds).reshape(-1, 2)
+    lj = f32(lj)
+    return lib().orc_lj_cut_energy(_p(pos4, C.c_float), len(pos4), ntypes, _p(lj, C.c_float), rc, L,
+                                   _p(b, C.c_int32), len(b))
+
+
+def nh_chain_energy(form, M, Q1, Qr, xi, vxi, kT, natoms):
+    """Chain energy: form 0 = the chain terms of the reference's conserved
+    quantity (tests/test_nose_hoover.cu:19-40); form 1 = the weights the
+    chain's dynamics conserves (oracle.hpp nh_chain_energy_dyn)."""
+    xi = np.ascontiguousarray(np.resize(np.asarray(xi, np.float64), 10))
+    vxi = np.ascontiguousarray(np.resize(np.asarray(vxi, np.float64), 10))
+    return lib().orc_nh_chain_energy(form, int(M), float(Q1), float(Qr), _p(xi, C.c_double), _p(vxi, C.c_double),
+                                     float(kT), int(natoms))
+
+
+def extended_energy(xr, natoms, dt, form=1, shifted=True):
+    """Extended-system energy of each md_run ext record:
+    K + U + chain energy (form as in nh_chain_energy), and under NPT
+    + P V + v_eps^2 / (2 W) with V the barostat's volume.  The reference's
+    barostat scales velocities by exp(-(1 + 3/Nf) v_eps/W dt/2) but its
+    v_eps force lacks the matching (3/Nf) 2K term of the MTK equations
+    (src/integrate/nose_hoover.cu:206-222,236-238), so under NPT the
+    energy changes at the rate -(6K/Nf) v_eps/W; the NPT value adds the
+    trapezoid integral of that rate, which makes it a constant of the
+    continuous dynamics.  With shifted=True the potential energy is that of
+    the cut-and-shifted LJ (same forces; the truncated LJ of
+    src/force/lj.cu:37-50 jumps by U(rc) whenever a pair crosses the cutoff,
+    which no integrator conserves).  Returns one value per record."""
+    out = np.zeros(len(xr))
+    Nf = 3.0 * natoms
+    acc = 0.0
+    for r, x in enumerate(xr):
+        M = int(x[9])
+        e = x[0] + x[1] - (x[32] if shifted else 0.0) + nh_chain_energy(form, M, x[4], x[5], x[10:20],
+                                                                         x[20:30], x[30], natoms)
+        if x[6] > 0.0:
+            if r > 0:
+                p = xr[r - 1]
+                acc += 0.5 * dt * ((6.0 * p[0] / Nf) * p[8] / p[6] + (6.0 * x[0] / Nf) * x[8] / x[6])
+            e += x[31] * x[3] + 0.5 * x[8] * x[8] / x[6] + acc
+        out[r] = e
+    return out
 
 
 def cpu_md_langevin(pos4, vel3, img3, masses, L, nsteps, bonds, bp, ang4, ap, T=1.0, Tdamp=1.0, seed=42,

@@ -122,6 +122,17 @@ double orc_nh_chain(int prec, int M, double Q1, double Qr, double* xi, double* v
     return s;
 }
 
+// Chain energy from a state: form 0 = the reference's conserved-quantity
+// chain terms (tests/test_nose_hoover.cu:19-40), 1 = the form the chain's
+// dynamics conserves (nh_chain_energy_dyn).
+double orc_nh_chain_energy(int form, int M, double Q1, double Qr, const double* xi, const double* vxi,
+                           double kT, int natoms) {
+    NHChain<double> c;
+    c.M = M; c.Q1 = Q1; c.Qr = Qr;
+    for (int k = 0; k < 10; k++) { c.xi[k] = xi[k]; c.vxi[k] = vxi[k]; }
+    return form == 0 ? nh_chain_energy<double>(c, kT, natoms) : nh_chain_energy_dyn<double>(c, kT, natoms);
+}
+
 // ---------------------------------------------------------------------------
 // MD driver in the reference's operator order (src/main.cu:403-630).
 // Positions/velocities in FP32 (as on the device), forces from forces_f64
@@ -185,7 +196,50 @@ static void wrap_atom(float* r, int* img, float L, float invL) {
     }
 }
 
-int orc_md_run(OrcSys* sp, long nsteps, double* thermo) {
+// Extended-system record (ext, nullable): kExt doubles per row, row 0 before
+// the first step and row k after step k:
+//   0 K (kinetic energy, sum m v^2 / 2), 1 U (potential energy), 2 V of the box
+//   the positions are in, 3 V0*exp(3 eps) (the barostat's volume), 4 Q1, 5 Qr,
+//   6 W, 7 eps, 8 v_eps, 9 M, 10..19 xi[k], 20..29 v_xi[k], 30 target kT of the
+//   step, 31 target P of the step, 32 the LJ energy at the cutoff summed over the
+//   pairs inside it (U - col 32 is the energy of the cut-and-shifted potential,
+//   which has the same forces and no jump when a pair crosses rc).
+constexpr int kExt = 40;
+
+// Sum over unordered non-excluded pairs inside the LJ cutoff (the grid-unit test
+// of the force path) of 4 eps ((sig/rc)^12 - (sig/rc)^6) of the pair's types.
+static double lj_cut_energy_raw(const float* pos4, int n, int ntypes, const float* lj2, float rc, float L,
+                                const int* eoff, const int* elst) {
+    double sum = 0.0;
+    const float rc2 = rc * rc;
+    for (int i = 0; i < n; i++) {
+        const int ti = type_of(pos4[4 * i + 3]);
+        if (ti < 0) continue;
+        for (int j = i + 1; j < n; j++) {
+            const int tj = type_of(pos4[4 * j + 3]);
+            if (tj < 0 || excluded(eoff, elst, i, j)) continue;
+            if (!lj_inside_grid(pos4 + 4 * i, pos4 + 4 * j, rc2, L)) continue;
+            const double eps = lj2[2 * (ti * ntypes + tj)], sig = lj2[2 * (ti * ntypes + tj) + 1];
+            const double sr6 = std::pow(sig / (double)rc, 6.0);
+            sum += 4.0 * eps * (sr6 * sr6 - sr6);
+        }
+    }
+    return sum;
+}
+static double lj_cut_energy(const OrcSys& s, float L, const std::vector<int>& eoff,
+                            const std::vector<int>& elst) {
+    return lj_cut_energy_raw(s.pos4, s.n, s.ntypes, s.lj2, s.rc, L, eoff.data(), elst.data());
+}
+
+// The same sum for a given state (exclusions from the bond list).
+double orc_lj_cut_energy(const float* pos4, int n, int ntypes, const float* lj2, float rc, float L,
+                         const int* bonds, int nbonds) {
+    std::vector<int> eoff, elst;
+    build_exclusions(n, bonds, nbonds, eoff, elst);
+    return lj_cut_energy_raw(pos4, n, ntypes, lj2, rc, L, eoff.data(), elst.data());
+}
+
+int orc_md_run(OrcSys* sp, long nsteps, double* thermo, double* ext) {
     OrcSys& s = *sp;
     const int n = s.n;
     std::vector<int> eoff, elst;
@@ -211,9 +265,22 @@ int orc_md_run(OrcSys* sp, long nsteps, double* thermo) {
     double k6[6];
     kin_tensor(s, k6);
     double carry = 0.5 * (k6[0] + k6[3] + k6[5]);
+    auto record = [&](long row, double K, double U, float Lpos, double kT, double Pt) {
+        if (!ext) return;
+        double* x = ext + (size_t)kExt * row;
+        x[0] = K; x[1] = U; x[2] = (double)Lpos * Lpos * Lpos;
+        x[3] = baro.V0 * std::exp(3.0 * baro.eps);
+        x[4] = ch.Q1; x[5] = ch.Qr; x[6] = baro.W; x[7] = baro.eps; x[8] = baro.veps; x[9] = ch.M;
+        for (int k = 0; k < 10; k++) { x[10 + k] = ch.xi[k]; x[20 + k] = ch.vxi[k]; }
+        x[30] = kT; x[31] = Pt;
+        x[32] = lj_cut_energy(s, Lpos, eoff, elst);
+    };
+    record(0, carry, pe, L, s.T_start, s.P_start);
     for (long step = 1; step <= nsteps; step++) {
         float frac = (float)(step - 1) / (float)s.nsteps_total;
         float kT = s.T_start + (s.T_stop - s.T_start) * frac;
+        float Pt_step = s.P_start + (s.P_stop - s.P_start) * frac;
+        float Lpos = L;  // box the positions of this step are placed in
         float invL = 1.0f / L;
         if (s.ensemble == 0) {
             for (int i = 0; i < n; i++) {
@@ -246,6 +313,7 @@ int orc_md_run(OrcSys* sp, long nsteps, double* thermo) {
                 fpos = dt_d * std::exp(h) * sinch(h);
                 L = (float)o.L;
                 invL = 1.0f / L;
+                Lpos = L;
             }
             float sc = (float)(nh * bs);
             float ew = (float)exp_vW, fp = (float)fpos;
@@ -299,6 +367,7 @@ int orc_md_run(OrcSys* sp, long nsteps, double* thermo) {
             for (int c = 0; c < 6; c++) t[4 + c] = (k6[c] + vir[c]) / V;
             t[10] = L;
         }
+        record(step, 0.5 * (k6[0] + k6[3] + k6[5]), pe, Lpos, (double)kT, (double)Pt_step);
     }
     s.L = L;
     return 0;

@@ -78,3 +78,23 @@ def rel_err_floor(got, ref, floor):
     num = np.linalg.norm(got - ref, axis=-1)
     den = np.maximum(np.linalg.norm(ref, axis=-1), floor)
     return float(np.max(num / den))
+
+
+def lattice_chains(nside, rho=0.85, seed=1, T=1.0):
+    """A dense bead-spring system with no overlaps, built without a push-off:
+    nside^3 beads on a simple cubic lattice at density rho (spacing about
+    1.06 sigma at 0.85), each lattice row along x one straight chain of nside
+    beads (bonds between neighbours in the row, angles of 180 degrees), and
+    Maxwell velocities at T with zero total momentum.  Returns
+    (pos4, vel, L, bonds, angles)."""
+    n = nside ** 3
+    L = np.float32((n / rho) ** (1.0 / 3.0))
+    a = float(L) / nside
+    g = np.arange(n)
+    xyz = np.stack([g % nside, (g // nside) % nside, g // nside ** 2], 1) * a + 0.5 * a
+    bonds = np.array([[i, i + 1] for i in range(n) if (i % nside) != nside - 1], np.int32)
+    angles = np.array([[i, i + 1, i + 2, 0] for i in range(n) if (i % nside) < nside - 2], np.int32)
+    rng = np.random.default_rng(seed)
+    v = rng.normal(0.0, np.sqrt(T), (n, 3))
+    v -= v.mean(0)
+    return pos4_of(xyz), v.astype(np.float32), L, bonds, angles

@@ -4,7 +4,7 @@ different algorithms for the same set, and finite differences of the energy."""
 import numpy as np
 import pytest
 
-from conftest import KG_AP, KG_BP, KG_LJ, pos4_of
+from conftest import KG_AP, KG_BP, KG_LJ, lattice_chains, pos4_of
 
 F = np.float32
 
@@ -342,6 +342,114 @@ def test_oracle_md_nve_conserves_energy(orc, golden):
     _, _, _, _, th = orc.md_run(sysd, 300)
     e = (th[:, 1] + th[:, 2]) * 64
     assert np.max(np.abs(e - e[0])) < 0.01 * abs(e[0])
+
+
+# --- Nose-Hoover conserved quantities
+def _nh_sys(p4, v, L, bonds, angles, ensemble, dt, nsteps, Tdamp=0.1, P=1.0, Pdamp=1.0, masses=None):
+    return {"pos4": p4.copy(), "vel3": v.copy(), "img3": np.zeros((len(p4), 3), np.int32),
+            "masses": np.ones(16, F) if masses is None else masses, "lj": KG_LJ,
+            "bonds": bonds, "btype": np.zeros(len(bonds), np.int32), "bp": KG_BP, "ang4": angles, "ap": KG_AP,
+            "L": float(L), "ensemble": ensemble, "T_start": 1.0, "T_stop": 1.0, "Tdamp": Tdamp,
+            "P_start": P, "P_stop": P, "Pdamp": Pdamp, "chain": 3, "nsteps_total": nsteps, "dt": dt}
+
+
+def test_nh_reference_conserved_quantity(orc, golden):
+    """tests/test_nose_hoover.cu:43-192 (NVTLimitEnergyConservation): the
+    64-atom lattice of tests/test_integrator.cu, L=8, rc 2.5, skin 0.5,
+    dt 0.001, M=3, Tdamp=1000, 500 steps; the reference's conserved quantity
+    KE + PE + 1/2 Q1 v0^2 + Nf kT xi0 + sum (1/2 Qr vk^2 + kT xik)
+    (tests/test_nose_hoover.cu:19-40) drifts by less than 1%."""
+    sysd = {"pos4": pos4_of(golden["nve64_xyz"]), "vel3": golden["nve64_vel"].copy(),
+            "img3": np.zeros((64, 3), np.int32), "masses": np.ones(16, F), "lj": KG_LJ, "L": 8.0,
+            "skin": 0.5, "Tdamp": 1000.0, "ensemble": 1, "chain": 3, "nsteps_total": 500}
+    *_, xr = orc.md_run(sysd, 500, ext=True)
+    H = orc.extended_energy(xr, 64, 0.001, form=0, shifted=False)
+    assert abs(H[-1] - H[0]) / abs(H[0]) < 0.01
+    # the chain masses of NoseHooverState::init (src/integrate/nose_hoover.cu:336-339)
+    assert xr[0, 4] == 3 * 64 * 1.0 * 1000.0 ** 2 and xr[0, 5] == 1000.0 ** 2
+
+
+def _drifts(orc, ensemble, t_end=0.5, form=1, shifted=True, **kw):
+    p4, v, L, b, a = lattice_chains(7)
+    out = []
+    for dt in (0.002, 0.001):
+        ns = int(round(t_end / dt))
+        *_, xr = orc.md_run(_nh_sys(p4, v, L, b, a, ensemble, dt, ns, **kw), ns, ext=True)
+        H = orc.extended_energy(xr, len(p4), dt, form=form, shifted=shifted)
+        out.append((np.max(np.abs(H - H[0])), xr))
+    return out
+
+
+def test_nvt_extended_energy_converges(orc):
+    """Moderate damping (Tdamp = 0.1, M = 3) on a dense 343-bead KG system
+    with every force term: the extended energy of the chain's own dynamics
+    (element k >= 1 weighted 2^k, oracle.hpp nh_chain_energy_dyn) is a
+    constant of the continuous dynamics, so its drift over a fixed time
+    falls with dt, at the first order of this chain splitting (the backward
+    sweep leaves elements k >= 1 without their force term).  A plausible
+    slip fails this: weighting the chain as the reference's test formula does
+    leaves a drift of about 150 that does not shrink with dt, and so does an
+    energy that ignores the jump of the unshifted LJ at the cutoff."""
+    (d1, xr), (d2, _) = _drifts(orc, 1)
+    K0 = xr[0, 0]
+    assert 0.4 < d2 / d1 < 0.62, (d1, d2)
+    assert d2 < 0.01 * K0
+    (w1, _), (w2, _) = _drifts(orc, 1, form=0)
+    assert w2 > 10 * d2 and w2 / w1 > 0.9
+    (u1, _), (u2, _) = _drifts(orc, 1, shifted=False)
+    assert u2 / u1 > 0.9
+    # Q1 = Nf kT Tdamp^2, Qr = kT Tdamp^2 (src/integrate/nose_hoover.cu:336-339)
+    tau2 = float(F(0.1)) ** 2  # Tdamp is an FP32 run.conf value
+    assert abs(xr[0, 4] - 3 * 343 * tau2) < 1e-12 and abs(xr[0, 5] - tau2) < 1e-15
+
+
+def test_npt_extended_energy_converges(orc):
+    """NPT-NH at P = 1 (Pdamp = 1, Tdamp = 0.1, M = 3) from the same state:
+    K + U(shifted) + P V + v_eps^2/(2W) + chain + the integral of the missing
+    MTK term (6K/Nf) v_eps/W is a constant of the reference's barostat
+    dynamics; its drift falls with dt at first order.  The barostat moves:
+    eps swings by several percent of its range over the window."""
+    (d1, xr), (d2, _) = _drifts(orc, 2, t_end=0.6)
+    assert 0.4 < d2 / d1 < 0.62, (d1, d2)
+    assert d2 < 0.03 * xr[0, 0]
+    assert np.max(np.abs(xr[:, 7])) > 0.02
+    # W = (Nf + 3) kT Pdamp^2 (src/integrate/nose_hoover.cu:341-343)
+    assert abs(xr[0, 6] - (3 * 343 + 3)) < 1e-9
+
+
+def test_two_mass_langevin_equipartition(orc):
+    """tests/test_mass.cu restated through the driver: an ideal gas (LJ
+    epsilon 0) of beads with masses 1 and 4 under Langevin at T = 1.5 reaches
+    <m v^2> = kT per component for both species (the kick's 1/m and the
+    noise's sqrt(kT/m), src/integrate/langevin.cu:48-66); KE is mass
+    weighted (the thermo tensor sum m v v)."""
+    n = 2000
+    rng = np.random.default_rng(3)
+    xyz = rng.uniform(0, 60.0, (n, 3))
+    types = np.arange(n) % 2
+    sysd = {"pos4": pos4_of(xyz, types), "vel3": np.zeros((n, 3), F), "masses": np.array([1.0, 4.0] + [1.0] * 14, F),
+            "lj": np.zeros(8, F), "ntypes": 2, "L": 60.0, "Tdamp": 0.05, "T_start": 1.5, "T_stop": 1.5,
+            "ensemble": 0, "nsteps_total": 600, "dt": 0.002}
+    _, v, _, _, th = orc.md_run(sysd, 600)
+    for t, m in ((0, 1.0), (1, 4.0)):
+        mv2 = m * np.mean(v[types == t].astype(np.float64) ** 2)
+        assert abs(mv2 - 1.5) < 0.06, (t, mv2)
+    ke = 0.5 * np.sum(np.where(types == 0, 1.0, 4.0)[:, None] * v.astype(np.float64) ** 2) / n
+    assert abs(th[-1, 1] - ke) < 1e-9 * ke
+
+
+def test_two_mass_momentum_exchange(orc):
+    """tests/test_mass.cu:21-63: the half kicks are dt/2 F/m.  Two beads of
+    masses 1 and 2 pushed apart by their LJ repulsion for one step of the
+    driver (NVE limit): both kicks come from the same equal and opposite
+    forces, so m1 dv1 = -m2 dv2 and dv1 = -2 dv2 (to FP32 rounding)."""
+    p4 = pos4_of([[5.0, 5.0, 5.0], [6.0, 5.0, 5.0]], [0, 1])
+    sysd = {"pos4": p4, "vel3": np.zeros((2, 3), F), "masses": np.array([1.0, 2.0] + [1.0] * 14, F),
+            "lj": np.ones(8, F), "ntypes": 2, "L": 12.0, "Tdamp": 1e10, "ensemble": 0, "nsteps_total": 1}
+    _, v, _, _, _ = orc.md_run(sysd, 1)
+    assert v[0, 0] < 0 < v[1, 0]
+    assert abs(v[0, 0] + 2.0 * v[1, 0]) < 1e-6 * abs(v[0, 0])
+    assert np.all(v[:, 1:] == 0)
 
 
 # --- radius of gyration (tests/test_rg.cu:5-77)
