@@ -68,6 +68,66 @@ def cpu_baseline(engine, sids, specs, target_s=15.0):
                       f"({int(done_bs)} bead-steps, {t_used:.1f} s) with the OpenMP cell-list restatement"}
 
 
+def cpu_literal(eng, sid, desc, target_s=15.0):
+    """Times the literal CPU path on one core: the brute-force FP32 forces of
+    reference/cpu_reference.hpp plus a serial BAOAB step, on the 30,000-bead
+    melt's relaxed state, for as many steps as fit about target_s seconds
+    (O(N^2) pairs: a few seconds per step)."""
+    import numpy as np
+
+    from oracle import oracle as orc
+
+    p, v, im = eng.state(sid)
+    d = desc
+    t0 = time.perf_counter()
+    orc.literal_md_langevin(p, v, im, np.ones(16, np.float32), float(eng.box(sid)), 0, d.lj, d.bonds, d.bond_types,
+                            d.bond_params, d.angles, d.angle_params)
+    t_init = time.perf_counter() - t0  # the initial force evaluation alone
+    nst = max(1, int(target_s / max(t_init, 1e-3)) - 1)
+    t0 = time.perf_counter()
+    orc.literal_md_langevin(p, v, im, np.ones(16, np.float32), float(eng.box(sid)), nst, d.lj, d.bonds,
+                            d.bond_types, d.bond_params, d.angles, d.angle_params)
+    t = time.perf_counter() - t0 - t_init
+    return {"value": len(p) * nst / t, "unit": "bead-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{nst} Langevin steps of the {len(p)}-bead melt ({t:.1f} s after the initial forces): "
+                      "brute-force FP32 LJ/FENE/angle of reference/cpu_reference.hpp and a serial BAOAB step"}
+
+
+def nh_reference_runs(eng, log, steps=100000):
+    """The reference's own NVT/NPT Nose-Hoover benchmark (benchmarks/benchmark.json):
+    30,000 beads (300 x 100) at density 0.70 (box 34.995), chain 3, T = 1,
+    Tdamp = 1, NPT P = 0 -> 0, Pdamp = 1, thermo every 100 steps, 100,000 steps;
+    timed as one hipGraph-launched run on the GPU (inputs resident)."""
+    import fastmd_amd as fmd
+    from fastmd_amd.workloads import relaxed_replicas, state_desc
+
+    spec = {"nchains": 300, "chain_len": 100, "T": 1.0, "seed": 42, "gen_seed": 77}
+    gen = []
+    sid = relaxed_replicas(eng, [spec], descs=gen, rho=0.70)[0]
+    p, v, im = eng.state(sid)
+    L = eng.box(sid)
+    eng.remove(sid)
+    out = {"system": f"300 x 100 beads, box {L:.4f} (density 0.70), chain 3, T 1, Tdamp 1, thermo every 100 steps",
+           "steps": steps}
+    for name, ens, pub in (("nvt_nh", fmd.NVT_NH, 8251.0), ("npt_nh", fmd.NPT_NH, 6423.0)):
+        d = state_desc(gen[0], p, v, im, L, ensemble=ens, T=1.0, Tdamp=1.0, seed=42, nsteps=steps)
+        d.c.chain_length = 3
+        d.c.thermo_freq = 100
+        d.c.P_start = d.c.P_stop = 0.0
+        d.c.Pdamp = 1.0
+        s = eng.add(d)
+        eng.step(s, 200)
+        t0 = time.perf_counter()
+        eng.step(s, steps)
+        t = time.perf_counter() - t0
+        eng.drain_thermo(s)
+        eng.remove(s)
+        out[name] = {"steps_per_s": steps / t, "bead_steps_per_s": 30000 * steps / t,
+                     "published_steps_per_s_rtx4090d": pub, "ratio": steps / t / pub}
+        log(f"{name}: {steps} steps in {t:.2f} s -> {steps / t:.0f} steps/s (published {pub:.0f} on an RTX 4090 D)")
+    return out
+
+
 def served_stream(eng, sids, descs, rate, tuned, window_s, steps, log_every=10.0):
     """Poisson stream of simulation jobs through the continuous-batching
     server: jobs start from the relaxed replicas (picked uniformly), run a
@@ -127,6 +187,108 @@ def served_stream(eng, sids, descs, rate, tuned, window_s, steps, log_every=10.0
             "graph_launches": st["graph_launches"], "elapsed_s": t_all}
 
 
+def served_c5(eng, log, rate_batched, tuned, window_s=600.0, warm_s=120.0, pool_n=64, seed=777, log_every=30.0):
+    """The served job stream at its stated configuration: Poisson arrivals over
+    window_s seconds at a fixed rate of 0.8 x the measured batched rate /
+    E[bead-steps per job]; per job beads log-uniform in [1,000, 200,000] (the
+    nearest of pool_n melts of log-spaced sizes, each generated and relaxed
+    before the window), chain length log-uniform in [20, 200], run length
+    log-uniform in [1,000, 100,000] steps, 70% Langevin / 30% Nose-Hoover NVT,
+    T uniform in [0.8, 1.2], a fresh seed per job.  Reports p50/p99 completion
+    latency over all jobs, the offered load, and the bead-steps the server
+    executed per second over the steady part of the window (after warm_s)."""
+    import numpy as np
+
+    import fastmd_amd as fmd
+    from fastmd_amd.workloads import relaxed_replicas, state_desc
+
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(np.linspace(np.log(1000.0), np.log(200000.0), pool_n))
+    specs = []
+    for k, b in enumerate(sizes):
+        clen = int(round(np.exp(rng.uniform(np.log(20), np.log(200)))))
+        specs.append({"nchains": max(1, int(round(b / clen))), "chain_len": clen, "T": 1.0, "seed": 1 + k,
+                      "gen_seed": 5000 + k})
+    gen = []
+    sids = relaxed_replicas(eng, specs, descs=gen)
+    pool = []
+    for g, sid in zip(gen, sids):
+        p, v, im = eng.state(sid)
+        pool.append((g, p, v, im, eng.box(sid), len(p)))
+        eng.remove(sid)
+    logsz = np.log([x[5] for x in pool])
+    lo, hi = np.log(1000.0), np.log(200000.0)
+    e_beads = (200000.0 - 1000.0) / (hi - lo)
+    e_steps = (100000.0 - 1000.0) / np.log(100.0)
+    lam = 0.8 * rate_batched / (e_beads * e_steps)
+    jobs, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / lam)
+        if t >= window_s:
+            break
+        k = int(np.argmin(np.abs(logsz - rng.uniform(lo, hi))))
+        jobs.append({"t": t, "pool": k, "steps": int(round(np.exp(rng.uniform(np.log(1000.0), np.log(100000.0))))),
+                     "nvt": bool(rng.random() < 0.3), "T": float(rng.uniform(0.8, 1.2)),
+                     "seed": int(rng.integers(1, 2**31 - 1))})
+    offered = sum(pool[j["pool"]][5] * j["steps"] for j in jobs) / window_s
+    log(f"served C5: {len(jobs)} jobs at {lam:.3f}/s over {window_s:.0f} s, offered {offered:.4g} bead-steps/s "
+        f"(0.8 x {rate_batched:.4g})")
+    srv = fmd.Server(0)
+    srv.set_blocks(tuned)
+    jd = []
+    t_start = time.perf_counter()
+    next_log = log_every
+    marks = {}
+    for j in jobs:
+        g, p, v, im, L, n = pool[j["pool"]]
+        ens = fmd.NVT_NH if j["nvt"] else fmd.LANGEVIN
+        d = state_desc(g, p, v, im, L, ensemble=ens, T=j["T"], Tdamp=0.1 if j["nvt"] else 1.0, seed=j["seed"],
+                       nsteps=j["steps"])
+        while True:
+            el = time.perf_counter() - t_start
+            if "warm" not in marks and el >= warm_s:
+                marks["warm"] = (el, srv.stats()["bead_steps"])
+            if el >= next_log:
+                st = srv.stats()
+                log(f"serve t={el:.0f}s submitted {len(jd)} resident {st['resident']} queued {st['queued']} "
+                    f"bead-steps {st['bead_steps']:.4g}")
+                next_log += log_every
+            wait = j["t"] - el
+            if wait <= 0:
+                break
+            time.sleep(min(wait, 1.0))
+        jd.append((srv.submit(d, j["steps"]), n, j["steps"]))
+    while time.perf_counter() - t_start < window_s:
+        time.sleep(0.2)
+    marks["end"] = (time.perf_counter() - t_start, srv.stats()["bead_steps"])
+    st_end = srv.stats()
+    srv.drain()
+    t_all = time.perf_counter() - t_start
+    lat, failed, bs = [], 0, 0.0
+    for jid, n, k in jd:
+        r = srv.result(jid)
+        if r["status"] != 0:
+            failed += 1
+            continue
+        lat.append(r["t_done"] - r["t_submit"])
+        bs += float(n) * k
+    st = srv.stats()
+    srv.close()
+    lat = np.asarray(lat)
+    (tw, bw), (te, be) = marks["warm"], marks["end"]
+    steady = (be - bw) / (te - tw)
+    return {"config": "C5: Poisson stream, beads log-uniform [1000, 200000], steps log-uniform [1000, 100000], "
+                      "70% Langevin / 30% NVT-NH, T U[0.8, 1.2]",
+            "window_s": window_s, "jobs": len(jd), "failed": failed, "rate_jobs_per_s": lam,
+            "pool_melts": pool_n, "offered_bead_steps_per_s": offered,
+            "served_bead_steps_per_s_steady": steady, "steady_window_s": [tw, te],
+            "served_over_offered": steady / offered,
+            "resident_at_window_end": st_end["resident"], "queued_at_window_end": st_end["queued"],
+            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "completed_bead_steps": bs, "elapsed_with_drain_s": t_all,
+            "graph_launches": st["graph_launches"]}
+
+
 def calibrate_copy(mib):
     """A plain copy of known size (read mib MiB, write mib MiB) for scaling
     the HBM byte counters of a profiler against algorithmic bytes."""
@@ -147,10 +309,13 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--replicas", type=int, default=256)
-    ap.add_argument("--workload", choices=("replicas", "single", "small"), default="replicas",
+    ap.add_argument("--workload", choices=("replicas", "single", "small", "served"), default="replicas",
                     help="replicas: the uneven-replica batch (default); single: one 10,000 x 100-bead "
                          "melt (1,000,000 beads); small: one 300 x 100-bead melt (30,000 beads); "
-                         "both Langevin T=1")
+                         "both Langevin T=1; served: the 10-minute served job stream (its own JSON line)")
+    ap.add_argument("--no-nh", action="store_true",
+                    help="skip the reference's 30,000-bead NVT/NPT Nose-Hoover benchmark runs")
+    ap.add_argument("--c5-window", type=float, default=600.0, help="served workload: arrival window (s)")
     ap.add_argument("--plain-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
@@ -173,6 +338,23 @@ def main():
     from fastmd_amd import workloads
 
     eng = fmd.Engine(0)
+    if args.workload == "served":
+        specs = workloads.uneven_replica_specs(args.replicas)
+        sids = workloads.relaxed_replicas(eng, specs, log=log)
+        tuned = eng.autotune(sids, reps=3)
+        eng.step(sids, 100)
+        t0 = time.perf_counter()
+        eng.step(sids, 500)
+        rate = sum(eng.natoms[s] for s in sids) * 500 / (time.perf_counter() - t0)
+        log(f"batched rate of the uneven replicas: {rate:.4g} bead-steps/s")
+        for s in sids:
+            eng.remove(s)
+        res = served_c5(eng, log, rate, tuned, window_s=args.c5_window)
+        res.update({"metric": "served job stream: p50/p99 completion latency and bead-steps/s", "n_gpus": 1,
+                    "batched_rate_bead_steps_per_s": rate, "autotune": {f"{k[0]}:{k[1]}": v for k, v in tuned.items()}})
+        print(json.dumps(res), flush=True)
+        eng.close()
+        return
     single = args.workload in ("single", "small")
     if single:
         nch = 10000 if args.workload == "single" else 300
@@ -229,6 +411,15 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(eng, sids, specs)
     if cpu:
         log(f"cpu baseline: {cpu['value']:.4g} bead-steps/s on {cpu['cores']} threads")
+    lit = None
+    if not args.no_cpu:  # the literal single-core path on the 30,000-bead melt
+        c1 = {"nchains": 300, "chain_len": 100, "T": 1.0, "seed": 42, "gen_seed": 2024}
+        c1d = []
+        c1s = workloads.relaxed_replicas(eng, [c1], descs=c1d)[0]
+        lit = cpu_literal(eng, c1s, c1d[0])
+        eng.remove(c1s)
+        log(f"cpu literal path: {lit['value']:.4g} bead-steps/s on 1 core ({lit['sample']})")
+    nh = None if (args.no_nh or single) else nh_reference_runs(eng, log)
 
     out = {
         "metric": "bead-steps/s on KG melt: batched/served vs plain, 1 GPU; p50/p99 job latency",
@@ -267,7 +458,9 @@ def main():
         "detail": {"plain_bead_steps_per_s": plain, "batched_over_plain": batched / plain,
                    "plain_steps_per_replica": args.plain_steps,
                    "fused_force_ms": ms_all, "fused_force_alg_bytes": bytes_all, "fused_force_GBs": ach_all,
-                   "neighbor_rebuilds_total": rebuilds, "served": served, "autotune": {f"{k[0]}:{k[1]}": v for k, v in tuned.items()}},
+                   "neighbor_rebuilds_total": rebuilds, "served": served,
+                   "autotune": {f"{k[0]}:{k[1]}": v for k, v in tuned.items()},
+                   "cpu_literal_single_core": lit, "reference_nh_benchmark": nh},
     }
     print(json.dumps(out), flush=True)
     eng.close()

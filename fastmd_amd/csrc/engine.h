@@ -53,6 +53,20 @@ struct Batch {
     int tune_version = -1;
 };
 
+// Options of Engine::create beyond the descriptor.
+struct CreateOpts {
+    int kmult0 = 1;          // first neighbor-capacity multiplier to try (1, 2 or 4)
+    bool side = false;       // uploads and the initial build on the side stream
+    // out
+    uint64_t charged = 0;    // bytes charged against the budget (success)
+    uint64_t need = 0;       // bytes the last attempt needed (FMD_ERR_BUDGET)
+    int kmult = 1;           // multiplier of the last attempt
+};
+
+// Device bytes a system is charged: the block its arrays are carved from,
+// in whole 2 MiB allocation granules.
+uint64_t charge_bytes(uint64_t block_bytes);
+
 class Engine {
   public:
     Engine(int device, uint64_t budget);
@@ -61,8 +75,14 @@ class Engine {
     const char* last_error() const { return err_.c_str(); }
     int32_t fail(int32_t code, const std::string& msg);
 
-    int32_t create(const fmd_system_desc* d, int32_t* sys);
-    int32_t destroy(int32_t sys);
+    int32_t create(const fmd_system_desc* d, int32_t* sys, CreateOpts* o = nullptr);
+    // wait = false: the caller guarantees no launch in flight uses the system
+    // (it is in no batch of work still queued on the main stream).
+    int32_t destroy(int32_t sys, bool wait = true);
+    // While deferred, blocks released to the cache are not returned to HIP
+    // (hipFree waits for the whole device); flush_deferred() trims the cache.
+    void set_defer_free(bool on) { defer_free_ = on; }
+    void flush_deferred();
     uint64_t bytes_in_use() const { return in_use_; }
     uint64_t budget() const { return budget_; }
 
@@ -77,6 +97,9 @@ class Engine {
     int32_t step(const int32_t* ids, int32_t n, int64_t nsteps, bool use_graph);
     // Non-blocking pieces used by the server: set run lengths, launch k steps.
     int32_t begin_run(const std::vector<int>& ids, int64_t nsteps);
+    // The same on the side stream, completed before it returns (for systems
+    // created there while the main stream is busy).
+    int32_t begin_run_side(const std::vector<int>& ids, int64_t nsteps);
     int32_t launch_steps(const std::vector<int>& ids, int64_t k, bool use_graph);
     int32_t sync();
     int32_t check_errors(const std::vector<int>& ids, std::vector<int>* failed);
@@ -86,7 +109,9 @@ class Engine {
 
     int64_t step_count(int32_t sys);
     int64_t rebuilds(int32_t sys);
-    int32_t get_state(int32_t sys, float* pos4, float* vel3, int32_t* img3);
+    // wait = false: read without waiting for the main stream (the system is
+    // not in any launch still in flight)
+    int32_t get_state(int32_t sys, float* pos4, float* vel3, int32_t* img3, bool wait = true);
     int32_t get_box(int32_t sys, double* L, bool of_positions = false);
     int32_t get_thermo(int32_t sys, fmd_thermo* out);
     int32_t get_thermostat(int32_t sys, fmd_thermostat* out);
@@ -110,7 +135,8 @@ class Engine {
     Replica* rep(int32_t sys);
 
   private:
-    int32_t create_impl(const fmd_system_desc* d, int32_t* sys, int kmult);
+    int32_t create_impl(const fmd_system_desc* d, int32_t* sys, int kmult, bool side, uint64_t* charged,
+                        uint64_t* need);
     int32_t hip_check(hipError_t e, const char* what);
     void* acquire_block(size_t bytes, size_t* got);
     void release_block(void* p, size_t bytes);
@@ -120,12 +146,15 @@ class Engine {
     void drop_batches_with(int id);
     void launch_step_seq(Batch& b);
     Launch launch_of(const Batch& b) const;
+    Launch launch_of(const Batch& b, hipStream_t s) const;
     int32_t read_state(Replica& r, RepState& s);
 
     int device_ = 0;
     bool init_ok_ = false;
     std::string err_;
     hipStream_t stream_ = nullptr;
+    hipStream_t side_ = nullptr;  // system creation beside the main stream's launches
+    bool defer_free_ = false;
     uint64_t budget_ = 0, in_use_ = 0;
     std::vector<Replica*> reps_;
     RepDesc* d_descs_ = nullptr;

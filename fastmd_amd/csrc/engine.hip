@@ -31,6 +31,7 @@ Engine::Engine(int device, uint64_t budget) : device_(device) {
     }
     if (hip_check(hipSetDevice(device), "hipSetDevice")) return;
     if (hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate")) return;
+    if (hip_check(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "hipStreamCreate")) return;
     size_t fr = 0, tot = 0;
     if (hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo")) return;
     budget_ = budget ? budget : (uint64_t)(0.9 * (double)fr);
@@ -41,6 +42,7 @@ Engine::Engine(int device, uint64_t budget) : device_(device) {
 
 Engine::~Engine() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    if (side_) (void)hipStreamSynchronize(side_);
     for (auto& kv : batches_) {
         free_batch(kv.second);
     }
@@ -52,6 +54,7 @@ Engine::~Engine() {
     flush_cache(0);
     if (d_descs_) (void)hipFree(d_descs_);
     if (stream_) (void)hipStreamDestroy(stream_);
+    if (side_) (void)hipStreamDestroy(side_);
 }
 
 int32_t Engine::fail(int32_t code, const std::string& msg) {
@@ -102,7 +105,13 @@ void Engine::release_block(void* p, size_t bytes) {
     flush_cache(std::min<uint64_t>(budget_ / 8, 8ull << 30));
 }
 
+void Engine::flush_deferred() {
+    defer_free_ = false;
+    flush_cache(std::min<uint64_t>(budget_ / 8, 8ull << 30));
+}
+
 void Engine::flush_cache(uint64_t keep) {
+    if (defer_free_ && keep > 0) return;
     while (cached_bytes_ > keep && !cache_.empty()) {
         auto it = std::prev(cache_.end());  // largest first
         (void)hipFree(it->second);
@@ -231,34 +240,49 @@ static int32_t validate(const fmd_system_desc* d, std::string& msg) {
     return FMD_OK;
 }
 
+uint64_t charge_bytes(uint64_t block_bytes) {
+    const uint64_t g = 2ull << 20;
+    return (block_bytes + g - 1) / g * g;
+}
+
+// The admission unit: what create charges at the first neighbor capacity.
 uint64_t system_bytes(const fmd_system_desc* d) {
     std::string m;
     if (validate(d, m)) return 0;
     int bc, ac;
     incidence_caps(d, &bc, &ac);
-    return sizing(d, bc, ac).bytes;
+    return charge_bytes(sizing(d, bc, ac).bytes);
 }
 
 // ---------------------------------------------------------------------------
 // create / destroy
 // ---------------------------------------------------------------------------
-int32_t Engine::create(const fmd_system_desc* d, int32_t* sys) {
+int32_t Engine::create(const fmd_system_desc* d, int32_t* sys, CreateOpts* o) {
     // an initial configuration denser than the list capacity rule allows is
     // created again with a larger capacity instead of being truncated
+    CreateOpts dflt;
+    if (!o) o = &dflt;
     int32_t rc = FMD_ERR_NBR_OVERFLOW;
-    for (int kmult = 1; kmult <= 4 && rc == FMD_ERR_NBR_OVERFLOW; kmult *= 2) rc = create_impl(d, sys, kmult);
+    int kmult = o->kmult0 == 2 || o->kmult0 == 4 ? o->kmult0 : 1;
+    for (; kmult <= 4 && rc == FMD_ERR_NBR_OVERFLOW; kmult *= 2) {
+        o->kmult = kmult;
+        rc = create_impl(d, sys, kmult, o->side, &o->charged, &o->need);
+    }
     return rc;
 }
 
-int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
+int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult, bool side, uint64_t* charged,
+                            uint64_t* need) {
     std::string msg;
     if (int32_t rc = validate(d, msg)) return fail(rc, msg);
     int bcap, acap;
     incidence_caps(d, &bcap, &acap);
     if (bcap > 255 || acap > 255) return fail(FMD_ERR_ARG, "more than 255 bonds or angles on one atom");
     Sizing S = sizing(d, bcap, acap, kmult);
-    if (in_use_ + S.bytes > budget_)
-        return fail(FMD_ERR_BUDGET, "system needs " + std::to_string(S.bytes) +
+    uint64_t charge = charge_bytes(S.bytes);
+    *need = charge;
+    if (in_use_ + charge > budget_)
+        return fail(FMD_ERR_BUDGET, "system needs " + std::to_string(charge) +
                                         " bytes; budget left " + std::to_string(budget_ - in_use_));
     int id = -1;
     for (int k = 0; k < kMaxReplicas; k++)
@@ -429,6 +453,15 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     };
     layout();
     const size_t block_bytes = used;
+    if (charge_bytes(block_bytes) > charge) {  // the estimate fell short of the carved layout
+        charge = charge_bytes(block_bytes);
+        *need = charge;
+        if (in_use_ + charge > budget_) {
+            delete R;
+            return fail(FMD_ERR_BUDGET, "system needs " + std::to_string(charge) + " bytes; budget left " +
+                                            std::to_string(budget_ - in_use_));
+        }
+    }
     size_t got = 0;
     base = (char*)acquire_block(block_bytes, &got);
     if (!base) {
@@ -438,7 +471,9 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     r.block = base;
     r.block_bytes = got;
     r.block_used = block_bytes;
-    r.bytes = got;
+    // charged by what it needs, whether the block came fresh (whole granules)
+    // or from the cache (up to 1.25x larger): the budget admits by this unit
+    r.bytes = charge;
     used = 0;
     layout();
 
@@ -509,7 +544,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
     S0.stress_freq = std::max(0, d->stress_freq);
     S0.thermo_cap = S.thermo_cap;
 
-    hipStream_t s = stream_;
+    hipStream_t s = side ? side_ : stream_;
     int32_t rc = FMD_OK;
     auto up = [&](void* dst, const void* src, size_t n) {
         if (!rc && n) rc = hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s), "upload");
@@ -563,7 +598,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
         destroy(id);
         return FMD_ERR_HIP;
     }
-    Launch l = launch_of(*b);
+    Launch l = launch_of(*b, s);
     launch_rebuild(l, 1, 1);
     launch_force(l, 0, b->ntl, 256, FMD_TERM_ALL, 0, FM_OBS);
     launch_boundary(l, BND_PRIME);
@@ -583,13 +618,15 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult) {
         return rc;
     }
     *sys = id;
+    *charged = charge;
     return FMD_OK;
 }
 
-int32_t Engine::destroy(int32_t sys) {
+int32_t Engine::destroy(int32_t sys, bool wait) {
     Replica* r = rep(sys);
     if (!r) return fail(FMD_ERR_ARG, "no such system");
-    (void)hipStreamSynchronize(stream_);
+    if (wait) (void)hipStreamSynchronize(stream_);
+    (void)hipStreamSynchronize(side_);
     drop_batches_with(sys);
     release_block(r->block, r->block_bytes);
     in_use_ -= r->bytes;
@@ -689,9 +726,9 @@ void Engine::free_batch(Batch* b) {
     delete b;
 }
 
-Launch Engine::launch_of(const Batch& b) const {
-    return Launch{d_descs_, b.d_tiles, b.ntl, b.d_reps, b.nr, b.d_rep_t0, b.d_rlist, b.d_nrt, b.d_chunks, b.nch,
-                  stream_};
+Launch Engine::launch_of(const Batch& b) const { return launch_of(b, stream_); }
+Launch Engine::launch_of(const Batch& b, hipStream_t s) const {
+    return Launch{d_descs_, b.d_tiles, b.ntl, b.d_reps, b.nr, b.d_rep_t0, b.d_rlist, b.d_nrt, b.d_chunks, b.nch, s};
 }
 
 // One MD step for every replica of the batch.
@@ -715,6 +752,17 @@ int32_t Engine::begin_run(const std::vector<int>& ids, int64_t nsteps) {
     launch_control(l, CTL_RUN, nsteps);
     launch_boundary(l, BND_START);
     return hip_check(hipGetLastError(), "launch(begin_run)");
+}
+
+int32_t Engine::begin_run_side(const std::vector<int>& ids, int64_t nsteps) {
+    Batch* b = batch_for(ids);
+    if (!b) return FMD_ERR_ARG;
+    Launch l = launch_of(*b, side_);
+    launch_control(l, CTL_RUN, nsteps);
+    launch_boundary(l, BND_START);
+    int32_t rc = hip_check(hipGetLastError(), "launch(begin_run)");
+    if (!rc) rc = hip_check(hipStreamSynchronize(side_), "sync(side)");
+    return rc;
 }
 
 int32_t Engine::launch_steps(const std::vector<int>& ids, int64_t k, bool use_graph) {
@@ -983,28 +1031,30 @@ int32_t Engine::compute_forces(int32_t sys, uint32_t terms, float* f4, double* v
     launch_force(l, 0, b->ntl, 256, (int)(terms & 7u), 0, FM_OBS);
     launch_boundary(l, BND_REDUCE);
     int32_t rc = sync();
-    if (rc) return rc;
     RepState s;
-    if ((rc = read_state(*r, s))) return rc;
-    if (f4) {
+    if (!rc) rc = read_state(*r, s);
+    if (!rc && f4) {
         const RepDesc& D = r->desc;
         std::vector<float4> f(D.nslots);
         std::vector<int4> img(D.nslots);
         rc = hip_check(hipMemcpy(f.data(), D.force, f.size() * 16, hipMemcpyDeviceToHost), "copy force");
         if (!rc) rc = hip_check(hipMemcpy(img.data(), D.img, img.size() * 16, hipMemcpyDeviceToHost), "copy img");
-        if (rc) return rc;
-        for (int i = 0; i < D.natoms; i++) {
-            int g = img[i].w;
-            f4[4 * g] = f[i].x; f4[4 * g + 1] = f[i].y; f4[4 * g + 2] = f[i].z; f4[4 * g + 3] = f[i].w;
-        }
+        if (!rc)
+            for (int i = 0; i < D.natoms; i++) {
+                int g = img[i].w;
+                f4[4 * g] = f[i].x; f4[4 * g + 1] = f[i].y; f4[4 * g + 2] = f[i].z; f4[4 * g + 3] = f[i].w;
+            }
     }
-    if (vir)
-        for (int c = 0; c < 6; c++) vir[c] = s.vir[c];
-    if (pe) *pe = s.pe;
-    if ((terms & 7u) != 7u) {  // restore the full forces the next step's kick uses
+    if (!rc) {
+        if (vir)
+            for (int c = 0; c < 6; c++) vir[c] = s.vir[c];
+        if (pe) *pe = s.pe;
+    }
+    if ((terms & 7u) != 7u) {  // on every path: restore the full forces the next step's kick uses
         launch_force(l, 0, b->ntl, 256, FMD_TERM_ALL, 0, FM_OBS);
         launch_boundary(l, BND_REDUCE);
-        rc = sync();
+        const int32_t rc2 = sync();
+        if (!rc) rc = rc2;
     }
     if (!rc && (s.err & ERR_STRETCH)) return fail(FMD_ERR_STRETCH, "bond stretched beyond R0");
     return rc;
@@ -1013,14 +1063,14 @@ int32_t Engine::compute_forces(int32_t sys, uint32_t terms, float* f4, double* v
 // ---------------------------------------------------------------------------
 // observation
 // ---------------------------------------------------------------------------
-int32_t Engine::get_state(int32_t sys, float* pos4, float* vel3, int32_t* img3) {
+int32_t Engine::get_state(int32_t sys, float* pos4, float* vel3, int32_t* img3, bool wait) {
     Replica* r = rep(sys);
     if (!r) return fail(FMD_ERR_ARG, "no such system");
     const RepDesc& D = r->desc;
     std::vector<float4> p(D.nslots), v(D.nslots);
     std::vector<int4> img(D.nslots);
     RepState s;
-    int32_t rc = sync();
+    int32_t rc = wait ? sync() : FMD_OK;
     if (!rc) rc = hip_check(hipMemcpy(p.data(), D.pos, p.size() * 16, hipMemcpyDeviceToHost), "copy pos");
     if (!rc) rc = hip_check(hipMemcpy(v.data(), D.vel, v.size() * 16, hipMemcpyDeviceToHost), "copy vel");
     if (!rc) rc = hip_check(hipMemcpy(img.data(), D.img, img.size() * 16, hipMemcpyDeviceToHost), "copy img");
@@ -1157,7 +1207,7 @@ int32_t Engine::resume(const void* buf, uint64_t size, int32_t* sys) {
     });
     if (bad || h.natoms <= 0 || h.nslots < h.natoms || h.cls < 0 || h.cls >= kNumClasses || !h.desc.st)
         return fail(FMD_ERR_ARG, "corrupt snapshot header");
-    if (in_use_ + h.used > budget_)
+    if (in_use_ + charge_bytes(h.used) > budget_)
         return fail(FMD_ERR_BUDGET, "snapshot needs " + std::to_string(h.used) + " bytes; budget left " +
                                         std::to_string(budget_ - in_use_));
     int id = -1;
@@ -1173,7 +1223,7 @@ int32_t Engine::resume(const void* buf, uint64_t size, int32_t* sys) {
     r.natoms = h.natoms; r.nslots = h.nslots; r.ntiles = h.ntiles; r.cls = h.cls;
     r.ensemble = h.ensemble; r.thermo_cap = h.thermo_cap;
     r.steps_done = h.steps_done; r.thermo_read = h.thermo_read;
-    r.block = base; r.block_bytes = got; r.block_used = h.used; r.bytes = got;
+    r.block = base; r.block_bytes = got; r.block_used = h.used; r.bytes = charge_bytes(h.used);
     r.desc = h.desc;
     for_each_ptr(r.desc, [&](auto*& p) {
         const uintptr_t o = (uintptr_t)p;

@@ -19,6 +19,11 @@ class AdmissionQueue {
     AdmissionQueue(uint64_t budget, int max_resident) : budget_(budget), max_(max_resident) {}
     bool fits_ever(uint64_t bytes) const { return bytes <= budget_; }
     void push(const AdmitItem& it) { q_.push_back(it); }
+    // A job handed back (it needs more than it was admitted with): it keeps its
+    // place at the head of the queue.
+    void push_front(const AdmitItem& it) { q_.push_front(it); }
+    // An admitted job's charge changed by delta bytes (its real footprint).
+    void adjust(int64_t delta) { used_ = (uint64_t)((int64_t)used_ + delta); }
     // Pops the jobs that can start now, in FIFO order.
     std::vector<AdmitItem> admit() {
         std::vector<AdmitItem> out;

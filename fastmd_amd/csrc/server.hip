@@ -3,9 +3,12 @@
 // A worker thread owns the Engine.  Submitted jobs wait in a FIFO admission
 // queue until their device footprint fits the HBM budget; an admitted job's
 // state is created on the device once and stays resident until it finishes.
-// Between batched graph launches (kGraphSteps steps each) the loop admits new
-// jobs and retires finished ones, so jobs that arrive while others run join
-// the very next launch.  Each replica stops on the device at its own step
+// Each turn of the loop launches kGraphSteps steps of the resident set on the
+// main stream and, while they run, reads back the jobs that finished in the
+// previous launch and creates the newly admitted ones (host staging, uploads
+// and their initial list build and forces on a side stream), so the GPU waits
+// only for the status check between launches.  Jobs that arrive while others
+// run join the next launch.  Each replica stops on the device at its own step
 // count, so jobs of different lengths share launches.
 #include <algorithm>
 #include <cstdio>
@@ -61,7 +64,8 @@ struct Job {
     int64_t id;
     std::unique_ptr<DescCopy> desc;
     int64_t nsteps = 0, remaining = 0;
-    uint64_t bytes = 0;
+    uint64_t bytes = 0;   // charged against the budget (system_bytes, then the engine's charge)
+    int kmult = 1;        // neighbor-capacity multiplier to create with
     int state = FMD_JOB_QUEUED;
     int32_t status = FMD_OK;
     int sys = -1;
@@ -88,8 +92,9 @@ class Server {
         if (worker_.joinable()) worker_.join();
         if (const char* v = std::getenv("FMD_SERVER_PROFILE"))
             if (v[0] == '1')
-                std::fprintf(stderr, "[server] create %.3f s, enqueue %.3f s, device wait %.3f s, finish %.3f s\n",
-                             prof_[0], prof_[1], prof_[2], prof_[3]);
+                std::fprintf(stderr, "[server] overlapped host work %.3f s, enqueue %.3f s, device wait %.3f s, "
+                             "between launches %.3f s, requeued %lld\n",
+                             prof_[0], prof_[1], prof_[2], prof_[3], (long long)requeued_);
     }
     bool ok() const { return eng_.ok(); }
     double now() const {
@@ -170,89 +175,136 @@ class Server {
         done_cv_.notify_all();
     }
 
+    // Reads back a finished job and frees it; the caller guarantees no launch
+    // in flight uses it.
+    void retire(Job* j, int32_t status) {
+        fmd_thermo th{};
+        if (eng_.get_thermo(j->sys, &th)) th = fmd_thermo{};
+        std::vector<float> pos(4 * (size_t)j->desc->d.natoms), vel(3 * (size_t)j->desc->d.natoms);
+        eng_.get_state(j->sys, pos.data(), vel.data(), nullptr, false);
+        eng_.destroy(j->sys, false);
+        std::lock_guard<std::mutex> g(m_);
+        j->last = th;
+        j->pos.swap(pos);
+        j->vel.swap(vel);
+        aq_.release(j->bytes);
+        finish(*j, status);
+    }
+
+    // Creates an admitted job on the side stream; returns true if it is
+    // resident (else it failed, or went back to the head of the queue).
+    bool admit_one(Job* j) {
+        int32_t sys = -1;
+        CreateOpts o;
+        o.kmult0 = j->kmult;
+        o.side = true;
+        int32_t rc = eng_.create(&j->desc->d, &sys, &o);
+        if (!rc) rc = eng_.begin_run_side({sys}, j->nsteps);
+        std::lock_guard<std::mutex> g(m_);
+        if (rc == FMD_ERR_BUDGET && sys < 0 && o.need > j->bytes && aq_.fits_ever(o.need)) {
+            // a denser start needs a larger list than admission priced: the job
+            // waits at the head of the queue for that much memory
+            aq_.release(j->bytes);
+            j->bytes = o.need;
+            j->kmult = o.kmult;
+            aq_.push_front(AdmitItem{j->id, j->bytes});
+            requeued_++;
+            return false;
+        }
+        if (rc) {
+            aq_.release(j->bytes);
+            if (sys >= 0) eng_.destroy(sys, false);
+            finish(*j, rc);
+            return false;
+        }
+        if (o.charged != j->bytes) {  // the engine's charge is the one admission tracks
+            aq_.adjust((int64_t)o.charged - (int64_t)j->bytes);
+            j->bytes = o.charged;
+        }
+        j->sys = sys;
+        j->state = FMD_JOB_RUNNING;
+        j->t_admit = now();
+        return true;
+    }
+
     void loop() {
-        std::vector<Job*> running;
+        std::vector<Job*> running;            // resident jobs of the next launch
+        std::vector<std::pair<Job*, int32_t>> done;  // finished in the last launch
         while (true) {
             std::vector<Job*> admitted;
             {
                 std::unique_lock<std::mutex> g(m_);
-                cv_.wait(g, [&] { return stop_ || !running.empty() || aq_.queued() > 0; });
+                cv_.wait(g, [&] { return stop_ || !running.empty() || !done.empty() || aq_.queued() > 0; });
                 if (stop_) break;
-                for (const AdmitItem& it : aq_.admit()) admitted.push_back(jobs_[it.job].get());
                 for (const BlockSet& b : blocks_) eng_.set_tuned(b.kernel, b.cls, b.block);
                 blocks_.clear();
             }
-            // create admitted jobs on the device (outside the lock)
-            double tc0 = now();
-            for (Job* j : admitted) {
-                int32_t sys = -1;
-                int32_t rc = eng_.create(&j->desc->d, &sys);
-                if (!rc) rc = eng_.begin_run({sys}, j->nsteps);
-                std::lock_guard<std::mutex> g(m_);
-                if (rc) {
-                    aq_.release(j->bytes);
-                    if (sys >= 0) eng_.destroy(sys);
-                    finish(*j, rc);
-                    continue;
-                }
-                j->sys = sys;
-                j->state = FMD_JOB_RUNNING;
-                j->t_admit = now();
-                running.push_back(j);
-            }
-            prof_[0] += now() - tc0;
-            if (running.empty()) continue;
+            // 1. launch the resident set (asynchronous)
             std::vector<int> ids;
             for (Job* j : running) ids.push_back(j->sys);
-            int64_t k = graph_steps_;
-            // a hipGraph pays off once the resident set is stable: right after an
-            // admission or a completion the set is new and likely to change
-            // again, so its steps are issued directly instead of captured
-            const bool stable = (ids == prev_ids_);
-            prev_ids_ = ids;
+            const int64_t k = graph_steps_;
+            int32_t rc = FMD_OK;
             const double ts0 = now();
-            int32_t rc = eng_.launch_steps(ids, k, stable);
+            if (!ids.empty()) {
+                // a hipGraph pays off once the resident set is stable: right after
+                // an admission or a completion the set is new and likely to change
+                // again, so its steps are issued directly instead of captured
+                const bool stable = (ids == prev_ids_);
+                prev_ids_ = ids;
+                rc = eng_.launch_steps(ids, k, stable);
+            }
             const double ts1 = now();
-            if (!rc) rc = eng_.sync();
+            // 2. while it runs: read back and free the jobs of the last launch,
+            //    then admit and create new jobs on the side stream
+            eng_.set_defer_free(true);
+            for (auto& jd : done) retire(jd.first, jd.second);
+            done.clear();
+            {
+                std::lock_guard<std::mutex> g(m_);
+                for (const AdmitItem& it : aq_.admit()) admitted.push_back(jobs_[it.job].get());
+            }
+            std::vector<Job*> created;
+            for (Job* j : admitted)
+                if (admit_one(j)) created.push_back(j);
             const double ts2 = now();
+            // 3. end of the launch
+            if (!rc) rc = eng_.sync();
+            eng_.flush_deferred();
+            const double ts3 = now();
+            prof_[0] += ts2 - ts1;
             prof_[1] += ts1 - ts0;
-            prof_[2] += ts2 - ts1;
+            prof_[2] += ts3 - ts2;
             double bs = 0;
-            for (Job* j : running) bs += (double)j->desc->d.natoms * (double)std::min<int64_t>(k, j->remaining);
             std::vector<int> failed;
-            if (!rc) eng_.check_errors(ids, &failed);
+            if (!ids.empty()) {
+                for (Job* j : running) bs += (double)j->desc->d.natoms * (double)std::min<int64_t>(k, j->remaining);
+                if (!rc) eng_.check_errors(ids, &failed);
+            }
             std::vector<Job*> keep;
             for (Job* j : running) {
                 j->remaining -= k;
-                bool bad = rc != FMD_OK || std::find(failed.begin(), failed.end(), j->sys) != failed.end();
+                const bool bad = rc != FMD_OK || std::find(failed.begin(), failed.end(), j->sys) != failed.end();
                 if (bad || j->remaining <= 0) {
                     const int32_t eng_err = bad && !rc ? eng_.replica_error(j->sys) : FMD_OK;
-                    fmd_thermo th{};
-                    eng_.get_thermo(j->sys, &th);
-                    std::vector<float> pos(4 * (size_t)j->desc->d.natoms), vel(3 * (size_t)j->desc->d.natoms);
-                    eng_.get_state(j->sys, pos.data(), vel.data(), nullptr);
-                    eng_.destroy(j->sys);
-                    std::lock_guard<std::mutex> g(m_);
-                    j->last = th;
-                    j->pos.swap(pos);
-                    j->vel.swap(vel);
-                    aq_.release(j->bytes);
-                    finish(*j, bad ? (rc ? rc : eng_err) : FMD_OK);
+                    done.push_back({j, bad ? (rc ? rc : eng_err) : FMD_OK});
                 } else {
                     keep.push_back(j);
                 }
             }
+            for (Job* j : created) keep.push_back(j);
             {
                 std::lock_guard<std::mutex> g(m_);
                 bead_steps_ += bs;
-                launches_++;
+                if (!ids.empty()) launches_++;
             }
             running.swap(keep);
+            // batches of sets that are gone (the device is idle here)
             std::vector<int> rids;
             for (Job* j : running) rids.push_back(j->sys);
             eng_.trim_batches(rids);
-            prof_[3] += now() - ts2;
+            prof_[3] += now() - ts3;
         }
+        for (auto& jd : done) retire(jd.first, jd.second);
     }
 
     struct BlockSet { int kernel, cls, block; };
@@ -260,7 +312,8 @@ class Server {
     int graph_steps_;
     std::vector<BlockSet> blocks_;  // block shapes to apply before the next launch
     std::vector<int> prev_ids_;     // resident set of the previous launch
-    double prof_[4] = {0, 0, 0, 0}; // seconds in create, enqueue, device wait, finish
+    double prof_[4] = {0, 0, 0, 0}; // seconds: host work beside a launch, enqueue, device wait, between launches
+    int64_t requeued_ = 0;          // jobs handed back to the queue needing more memory
     AdmissionQueue aq_;
     std::chrono::steady_clock::time_point t0_;
     std::mutex m_;

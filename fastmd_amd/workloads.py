@@ -78,13 +78,14 @@ def job_stream(pool_natoms, rate_bead_steps, window_s, steps=(1000, 100000), loa
 
 
 def relaxed_replicas(engine, specs, stages=PUSH_OFF, cool=COOL, final=FINAL, Tdamp=1.0, log=None,
-                     descs=None):
-    """Creates the replicas on the engine, relaxes them in batched steps and
-    returns their system ids (Langevin at each replica's T, Tdamp = 1).  If
-    `descs` is a list, the generated descriptors are appended to it."""
+                     descs=None, rho=0.85):
+    """Creates the replicas (melts at number density rho) on the engine,
+    relaxes them in batched steps and returns their system ids (Langevin at
+    each replica's T, Tdamp = 1).  If `descs` is a list, the generated
+    descriptors are appended to it."""
     sids = []
     for s in specs:
-        d = melt_desc(s, force_cap=stages[0][0], Tdamp=0.1)
+        d = melt_desc(s, rho=rho, force_cap=stages[0][0], Tdamp=0.1)
         sids.append(engine.add(d))
         if descs is not None:
             descs.append(d)

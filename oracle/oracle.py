@@ -388,5 +388,21 @@ def cpu_md_langevin(pos4, vel3, img3, masses, L, nsteps, bonds, bp, ang4, ap, T=
     return pos4, vel3, img3, nreb
 
 
+def literal_md_langevin(pos4, vel3, img3, masses, L, nsteps, lj, bonds, btypes, bp, ang4, ap, T=1.0, Tdamp=1.0,
+                        seed=42, rc=2.5, dt=0.001, ntypes=1):
+    """The literal brute-force CPU path (one core): see orc_literal_md_langevin."""
+    pos4, vel3, img3 = f32(pos4).copy(), f32(vel3).copy(), i32(img3).copy()
+    bonds, ang4 = i32(bonds).reshape(-1, 2), i32(ang4).reshape(-1, 4)
+    btypes, bp, ap, masses, lj = i32(btypes), f32(bp).reshape(-1), f32(ap).reshape(-1), f32(masses), f32(lj)
+    f = lib().orc_literal_md_langevin
+    f.restype = None
+    f.argtypes = [C.c_int, _fp, _fp, _ip, _fp, _fp, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
+                  C.c_float, C.c_uint64, C.c_int, _ip, _ip, _fp, C.c_int, _ip, _fp, C.c_long]
+    f(pos4.shape[0], _p(pos4, C.c_float), _p(vel3, C.c_float), _p(img3, C.c_int32), _p(masses, C.c_float),
+      _p(lj, C.c_float), ntypes, rc, L, dt, T, Tdamp, seed, len(bonds), _p(bonds, C.c_int32), _p(btypes, C.c_int32),
+      _p(bp, C.c_float), len(ang4), _p(ang4, C.c_int32), _p(ap, C.c_float), nsteps)
+    return pos4, vel3, img3
+
+
 def cpu_threads():
     return lib().orc_cpu_threads()

@@ -373,6 +373,57 @@ int orc_md_run(OrcSys* sp, long nsteps, double* thermo, double* ext) {
     return 0;
 }
 
+// The literal CPU path, timed on one core as the reference's own CPU
+// baseline: every step the brute-force FP32 forces of reference/cpu_reference.hpp
+// (lj_literal_f32 over all pairs, fene_literal_f32, angle_literal_f32), summed
+// into one force array, and the serial BAOAB Langevin step of
+// src/integrate/langevin.cu:34-130 with the normal3 noise.  Runs nsteps steps in
+// place; one type of mass masses[0] per type as in OrcSys.
+void orc_literal_md_langevin(int n, float* pos4, float* vel3, int* img3, const float* masses,
+                             const float* lj2, int ntypes, float rc, float L, float dt, float T,
+                             float Tdamp, uint64_t seed, int nbonds, const int* bonds, const int* btype,
+                             const float* bp, int nang, const int* ang4, const float* ap, long nsteps) {
+    std::vector<int> eoff, elst;
+    build_exclusions(n, bonds, nbonds, eoff, elst);
+    const float invL = 1.0f / L, half_dt = 0.5f * dt;
+    const float c1 = std::exp(-(1.0f / Tdamp) * dt), c2 = std::sqrt(1.0f - c1 * c1);
+    std::vector<float> f(4 * (size_t)n), fb(4 * (size_t)n), fa(4 * (size_t)n);
+    float vir[6];
+    auto forces = [&]() {
+        lj_literal_f32(pos4, n, lj2, ntypes, rc, L, invL, eoff.data(), elst.data(), f.data(), vir);
+        if (nbonds) {
+            fene_literal_f32(pos4, n, bonds, btype, nbonds, bp, L, invL, fb.data(), vir);
+            for (size_t q = 0; q < f.size(); q++) f[q] += fb[q];
+        }
+        if (nang) {
+            angle_literal_f32(pos4, n, ang4, nang, ap, L, invL, fa.data(), vir);
+            for (size_t q = 0; q < f.size(); q++) f[q] += fa[q];
+        }
+    };
+    forces();
+    for (long step = 1; step <= nsteps; step++) {
+        for (int i = 0; i < n; i++) {
+            float* r = pos4 + 4 * i;
+            float* v = vel3 + 3 * i;
+            const float m = masses[type_of(r[3])];
+            const float hdtm = half_dt / m;
+            for (int d = 0; d < 3; d++) v[d] += hdtm * f[4 * i + d];
+            for (int d = 0; d < 3; d++) r[d] += half_dt * v[d];
+            float z[3];
+            normal3(seed, (uint32_t)i, step, z);
+            const float ns = c2 * std::sqrt(T / m);
+            for (int d = 0; d < 3; d++) v[d] = c1 * v[d] + ns * z[d];
+            for (int d = 0; d < 3; d++) r[d] += half_dt * v[d];
+            wrap_atom(r, img3 + 3 * i, L, invL);
+        }
+        forces();
+        for (int i = 0; i < n; i++) {
+            const float hdtm = half_dt / masses[type_of(pos4[4 * i + 3])];
+            for (int d = 0; d < 3; d++) vel3[3 * i + d] += hdtm * f[4 * i + d];
+        }
+    }
+}
+
 // Thermo from state (src/analysis/thermo.cu:140-201): KE/N, PE/N, T=2KE/(3N),
 // P_ab = (sum m v_a v_b + W_ab)/V, virial order xx,xy,xz,yy,yz,zz.
 void orc_thermo(int n, const float* pos4, const float* vel3, const float* masses, const double* pe_atom,

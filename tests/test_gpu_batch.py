@@ -91,3 +91,43 @@ def test_server_budget_queues_jobs(fmd, trio):
     for a, c in zip(res, res[1:]):
         assert c["t_admit"] >= a["t_done"] - 1e-6
     srv.close()
+
+
+def test_server_tight_budget_many_small_jobs(fmd, engine):
+    """Admission and the engine charge a job the same bytes: with a budget of
+    three 1,000-bead jobs, twelve such jobs all run (none is refused by the
+    engine after admission), at most three at a time, and each equals its
+    alone run."""
+    d = relax_kg(fmd, engine, 10, 100, seed=71)
+    descs = []
+    for k in range(12):
+        x = fmd.Desc(d.pos4, d.c.L, KG_LJ, vel=d.vel, image=d.image, bonds=d.bonds, bond_params=KG_BP,
+                     angles=d.angles, angle_params=KG_AP, T=(1.0, 1.0), seed=900 + k, nsteps=1000)
+        descs.append(x)
+    b = fmd.system_bytes(descs[0])
+    assert b % (2 << 20) == 0
+    e = fmd.Engine(0, hbm_budget=3 * b)
+    sids = [e.add(x) for x in descs[:3]]
+    assert e.bytes_in_use() == 3 * b
+    with pytest.raises(fmd.FmdError) as ex:
+        e.add(descs[3])
+    assert ex.value.code == fmd.ERR_BUDGET
+    e.remove(sids[0])
+    sids[0] = e.add(descs[3])  # a block from the cache is charged the same
+    assert e.bytes_in_use() == 3 * b
+    e.close()
+    alone = [_run_alone(fmd, x, 60 + 10 * k)[0] for k, x in enumerate(descs)]
+    srv = fmd.Server(0, hbm_budget=3 * b, graph_steps=20)
+    jobs = [srv.submit(x, 60 + 10 * k) for k, x in enumerate(descs)]
+    srv.drain()
+    res = [srv.result(j, state=True) for j in jobs]
+    assert all(r["status"] == 0 for r in res), [r["status"] for r in res]
+    for r, st in zip(res, alone):
+        assert np.array_equal(r["pos4"], st[0]) and np.array_equal(r["vel"], st[1])
+    # never more than three resident: each admission waits for a completion
+    ev = sorted([(r["t_admit"], 1) for r in res] + [(r["t_done"], -1) for r in res])
+    live = 0
+    for _, x in ev:
+        live += x
+        assert live <= 3
+    srv.close()

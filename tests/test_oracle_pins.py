@@ -452,6 +452,26 @@ def test_two_mass_momentum_exchange(orc):
     assert np.all(v[:, 1:] == 0)
 
 
+def test_literal_cpu_path_follows_the_driver(orc):
+    """The single-core literal path (brute-force FP32 forces of
+    reference/cpu_reference.hpp and a serial BAOAB step) is timed as the
+    reference's CPU baseline; it runs the same trajectory as the driver (whose
+    forces are FP64) up to FP32 rounding over 10 Langevin steps of a dense
+    343-bead KG system."""
+    p4, v, L, b, a = lattice_chains(7)
+    n = len(p4)
+    pl, vl, _ = orc.literal_md_langevin(p4, v, np.zeros((n, 3), np.int32), np.ones(16, F), float(L), 10, KG_LJ, b,
+                                        np.zeros(len(b), np.int32), KG_BP, a, KG_AP, seed=9)
+    sysd = {"pos4": p4.copy(), "vel3": v.copy(), "masses": np.ones(16, F), "lj": KG_LJ, "bonds": b,
+            "btype": np.zeros(len(b), np.int32), "bp": KG_BP, "ang4": a, "ap": KG_AP, "L": float(L),
+            "ensemble": 0, "seed": 9, "nsteps_total": 10}
+    po, vo, _, _, _ = orc.md_run(sysd, 10)
+    d = pl[:, :3].astype(np.float64) - po[:, :3]
+    d -= float(L) * np.rint(d / float(L))
+    assert np.max(np.abs(d)) < 1e-5 and np.max(np.abs(vl - vo)) < 1e-4
+    assert np.max(np.abs(vl - v)) > 0.01  # it moved
+
+
 # --- radius of gyration (tests/test_rg.cu:5-77)
 def test_rg_reference_cases(orc):
     p = np.zeros((8, 4), F)
