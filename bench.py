@@ -460,6 +460,11 @@ def main():
                    "fused_force_ms": ms_all, "fused_force_alg_bytes": bytes_all, "fused_force_GBs": ach_all,
                    "neighbor_rebuilds_total": rebuilds, "served": served,
                    "autotune": {f"{k[0]}:{k[1]}": v for k, v in tuned.items()},
+                   "autotune_rule": "kernel:class -> value; kernel 0 = force workgroup threads, 1 = list-build "
+                                    "threads, 2 = neighbor layout (1 staged set, 0 direct gathers); classes "
+                                    "0: < 8192 beads, 1: < 32768, 2: larger",
+                   "autotune_ms": {str(c): [{"layout": a, "block": b, "force_ms": m} for a, b, m in rows]
+                                   for c, rows in (eng.autotune_times().items() if tuned else [])},
                    "cpu_literal_single_core": lit, "reference_nh_benchmark": nh},
     }
     print(json.dumps(out), flush=True)

@@ -144,6 +144,8 @@ SIGNATURES = [
     ("fmd_system_rebuilds", C.c_int64, [C.c_void_p, C.c_int32]),
     ("fmd_autotune", C.c_int32, [C.c_void_p, _ip, C.c_int32, C.c_int32]),
     ("fmd_tuned_block", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32]),
+    ("fmd_autotune_times", C.c_int32, [C.c_void_p, C.c_int32, _ip, _ip, _dp, C.c_int32]),
+    ("fmd_engine_set_tuned", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     ("fmd_time_force", C.c_int32, [C.c_void_p, _ip, C.c_int32, C.c_uint32, C.c_int32, _dp, _dp,
                                    C.POINTER(C.c_int64)]),
     ("fmd_size_class", C.c_int32, [C.c_int32]),
@@ -528,9 +530,29 @@ class Engine:
         return [buf[k].as_dict() for k in range(n.value)], lost.value
 
     def autotune(self, sids, reps=5):
+        """Tunes per size class; returns {(kernel, class): value} (kernel 0 =
+        force block, 1 = list-build block, 2 = neighbor layout)."""
         a = np.asarray(sids, np.int32)
         self._chk(lib().fmd_autotune(self.h, _ptr(a, C.c_int32), len(a), reps))
-        return {(k, c): lib().fmd_tuned_block(self.h, k, c) for k in (0, 1) for c in (0, 1, 2)}
+        return self.tuned()
+
+    def tuned(self):
+        return {(k, c): lib().fmd_tuned_block(self.h, k, c) for k in (0, 1, 2) for c in (0, 1, 2)}
+
+    def set_tuned(self, kernel, cls, value):
+        self._chk(lib().fmd_engine_set_tuned(self.h, kernel, cls, value))
+
+    def autotune_times(self):
+        """{class: [(layout, block, ms), ...]} of the last autotune."""
+        out = {}
+        for c in (0, 1, 2):
+            n = lib().fmd_autotune_times(self.h, c, None, None, None, 0)
+            if n <= 0:
+                continue
+            lay, blk, ms = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n, np.float64)
+            lib().fmd_autotune_times(self.h, c, _ptr(lay, C.c_int32), _ptr(blk, C.c_int32), _ptr(ms, C.c_double), n)
+            out[c] = [(int(a), int(b), float(m)) for a, b, m in zip(lay, blk, ms)]
+        return out
 
     def time_force(self, sids, terms=TERM_LJ, reps=20):
         a = np.asarray(sids, np.int32)

@@ -197,8 +197,15 @@ int32_t fmd_time_force(fmd_engine* e, const int32_t* sys, int32_t nsys, uint32_t
     return e ? e->e->time_force(sys, nsys, terms, reps, avg_ms, alg_bytes, pairs) : FMD_ERR_ARG;
 }
 int32_t fmd_tuned_block(const fmd_engine* e, int32_t kernel, int32_t size_class) {
-    if (!e || kernel < 0 || kernel > 1 || size_class < 0 || size_class >= fmd::kNumClasses) return -1;
+    if (!e || kernel < 0 || kernel > 2 || size_class < 0 || size_class >= fmd::kNumClasses) return -1;
     return e->e->tuned(kernel, size_class);
+}
+int32_t fmd_engine_set_tuned(fmd_engine* e, int32_t kernel, int32_t size_class, int32_t value) {
+    return e ? e->e->set_tuned(kernel, size_class, value) : FMD_ERR_ARG;
+}
+int32_t fmd_autotune_times(const fmd_engine* e, int32_t size_class, int32_t* layout, int32_t* block, double* ms,
+                           int32_t cap) {
+    return e ? e->e->tune_times(size_class, layout, block, ms, cap) : -1;
 }
 int32_t fmd_size_class(int32_t natoms) { return fmd::size_class(natoms); }
 

@@ -129,8 +129,12 @@ class Engine {
     int32_t time_force(const int32_t* ids, int32_t n, uint32_t terms, int reps, double* ms,
                        double* bytes, int64_t* pairs);
     int tuned(int kernel, int cls) const { return tuned_[kernel][cls]; }
-    // Sets one block shape (kernel 0 = force, 1 = list build); 64, 128 or 256.
-    int32_t set_tuned(int kernel, int cls, int block);
+    // Sets one tuned parameter of a size class: kernel 0 = force block shape
+    // (64, 128 or 256), 1 = list build (one shape, kGroupSlots), 2 = the force
+    // pass's neighbor layout (1 = staged neighbor set, 0 = direct gathers).
+    int32_t set_tuned(int kernel, int cls, int value);
+    // Timings of the last autotune: per size class, (layout, block, ms) rows.
+    int32_t tune_times(int cls, int32_t* layout, int32_t* block, double* ms, int32_t cap) const;
     hipStream_t stream() const { return stream_; }
     Replica* rep(int32_t sys);
 
@@ -159,8 +163,11 @@ class Engine {
     std::vector<Replica*> reps_;
     RepDesc* d_descs_ = nullptr;
     std::map<std::vector<int>, Batch*> batches_;
-    int tuned_[2][kNumClasses];
+    int tuned_[3][kNumClasses];
     int tune_version_ = 0;
+    struct TuneRow { int layout, block; double ms; };
+    std::vector<TuneRow> tune_rows_[kNumClasses];
+    void apply_layout(int cls);  // the class's neighbor layout into its systems' descriptors
     std::multimap<size_t, void*> cache_;  // released replica blocks by size
     uint64_t cached_bytes_ = 0;
 };

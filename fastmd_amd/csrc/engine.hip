@@ -28,6 +28,7 @@ Engine::Engine(int device, uint64_t budget) : device_(device) {
     for (int c = 0; c < kNumClasses; c++) {
         tuned_[0][c] = 256;
         tuned_[1][c] = kGroupSlots;  // the list build runs one workgroup per group of tiles
+        tuned_[2][c] = 1;            // staged neighbor sets
     }
     if (hip_check(hipSetDevice(device), "hipSetDevice")) return;
     if (hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate")) return;
@@ -313,6 +314,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult, b
     D.vlim = D.fcap > 0.0f ? 0.02f / d->dt : 0.0f;
     D.clamp_lj = 1;
     D.idx16 = (S.nslots <= 65536) && std::getenv("FMD_IDX32") == nullptr ? 1 : 0;
+    D.nostage = tuned_[2][size_class(S.natoms)] ? 0 : 1;
     D.seed_lo = (uint32_t)d->seed;
     D.seed_hi = (uint32_t)(d->seed >> 32);
 
@@ -737,11 +739,15 @@ void Engine::launch_step_seq(Batch& b) {
     launch_pre(l);
     launch_decide(l);
     launch_rebuild(l, 0, 1);
-    // one launch over every tile (the workgroup shape of the class holding most tiles)
-    int big = 0;
-    for (int c = 1; c < kNumClasses; c++)
-        if (b.cls_nt[c] > b.cls_nt[big]) big = c;
-    launch_force(l, 0, b.ntl, tuned_[0][big], FMD_TERM_ALL, 1, FM_STEP);
+    // the force pass over each size class's tile range with that class's
+    // tuned workgroup shape (adjacent classes of one shape in one launch)
+    for (int c = 0; c < kNumClasses;) {
+        int e = c + 1;
+        while (e < kNumClasses && tuned_[0][e] == tuned_[0][c]) e++;
+        const int t0 = b.cls_t0[c], nt = b.cls_t0[e - 1] + b.cls_nt[e - 1] - t0;
+        launch_force(l, t0, nt, tuned_[0][c], FMD_TERM_ALL, 1, FM_STEP);
+        c = e;
+    }
     launch_boundary(l, BND_STEP);
 }
 
@@ -1464,54 +1470,131 @@ int32_t Engine::time_force(const int32_t* ids, int32_t n, uint32_t terms, int re
 // waves lies inside one group; smaller ones stage the group's set more often).
 // The list build has one shape: one workgroup per group of tiles.
 // ---------------------------------------------------------------------------
+// Per size class: the force pass's neighbor layout (staged neighbor sets or
+// direct gathers; the list build writes the layout, so each is timed after a
+// rebuild in it) and its workgroup shape.  The candidates are timed on copies
+// of the systems (snapshot and resume), so the systems themselves keep their
+// lists and rebuild schedule; every candidate gives the same bits.  The cost
+// of a layout is its force pass plus a tenth of its list build (a system
+// rebuilds about every ten steps).  A new layout reaches each system at its
+// next rebuild.
 int32_t Engine::autotune(const int32_t* ids, int32_t n, int reps) {
     if (!ids || n <= 0) return fail(FMD_ERR_ARG, "empty system list");
-    std::vector<int> v(ids, ids + n);
-    Batch* b = batch_for(v);
-    if (!b) return FMD_ERR_ARG;
-    Launch l = launch_of(*b);
     if (reps < 1) reps = 1;
-    // block shapes the kernels are compiled for (__launch_bounds__(256))
-    const int cand[3] = {64, 128, 256};
-    hipEvent_t e0, e1;
-    if (int32_t rc = hip_check(hipEventCreate(&e0), "event")) return rc;
-    if (int32_t rc = hip_check(hipEventCreate(&e1), "event")) return rc;
-    int32_t rc = FMD_OK;
-    for (int c = 0; c < kNumClasses && !rc; c++) {
-        if (b->cls_nt[c] == 0) continue;
-        for (int kern = 0; kern < 1 && !rc; kern++) {
-            float best = 1e30f;
-            int bestb = 256;
+    std::vector<int> tmp;
+    int32_t rc = sync();
+    for (int k = 0; k < n && !rc; k++) {
+        uint64_t sz = 0;
+        rc = snapshot_size(ids[k], &sz);
+        std::vector<char> buf(sz);
+        if (!rc) rc = snapshot(ids[k], buf.data(), sz);
+        int32_t t = -1;
+        if (!rc) rc = resume(buf.data(), sz, &t);
+        if (!rc) tmp.push_back(t);
+    }
+    Batch* b = rc ? nullptr : batch_for(tmp);
+    if (!rc && !b) rc = FMD_ERR_ARG;
+    const int cand[3] = {64, 128, 256};  // the shapes the kernels are compiled for (__launch_bounds__(256))
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!rc) rc = hip_check(hipEventCreate(&e0), "event");
+    if (!rc) rc = hip_check(hipEventCreate(&e1), "event");
+    auto timed = [&](auto&& run) -> double {
+        run();  // warm
+        (void)hipEventRecord(e0, stream_);
+        for (int k = 0; k < reps; k++) run();
+        (void)hipEventRecord(e1, stream_);
+        if (!rc) rc = hip_check(hipGetLastError(), "autotune launch");
+        if (!rc) rc = hip_check(hipEventSynchronize(e1), "autotune sync");
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        return (double)ms / reps;
+    };
+    int best_layout[kNumClasses], best_block[kNumClasses];
+    double best_ms[kNumClasses];
+    for (int c = 0; c < kNumClasses; c++) {
+        best_layout[c] = tuned_[2][c];
+        best_block[c] = tuned_[0][c];
+        best_ms[c] = 1e300;
+        if (b && b->cls_nt[c] > 0) tune_rows_[c].clear();
+    }
+    for (int layout = 1; layout >= 0 && !rc; layout--) {
+        for (int t : tmp) {
+            Replica* r = rep(t);
+            r->desc.nostage = layout ? 0 : 1;
+            rc = hip_check(hipMemcpy(d_descs_ + t, &r->desc, sizeof(RepDesc), hipMemcpyHostToDevice), "desc");
+            if (rc) break;
+        }
+        if (rc) break;
+        Launch l = launch_of(*b);
+        launch_rebuild(l, 1, 1);
+        rc = sync();
+        // the list build of the whole batch in this layout, shared out by tiles
+        const double build_ms = rc ? 0.0 : timed([&] { launch_rebuild(l, 1, 1); });
+        for (int c = 0; c < kNumClasses && !rc; c++) {
+            if (b->cls_nt[c] == 0) continue;
+            const double share = build_ms * (double)b->cls_nt[c] / (double)b->ntl;
             for (int bi = 0; bi < 3 && !rc; bi++) {
-                auto run = [&]() { launch_force(l, b->cls_t0[c], b->cls_nt[c], cand[bi], FMD_TERM_ALL, 0, FM_OBS); };
-                run();  // warm
-                (void)hipEventRecord(e0, stream_);
-                for (int k = 0; k < reps; k++) run();
-                (void)hipEventRecord(e1, stream_);
-                rc = hip_check(hipGetLastError(), "autotune launch");
-                if (!rc) rc = hip_check(hipEventSynchronize(e1), "autotune sync");
-                float ms = 0;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                if (ms < best) { best = ms; bestb = cand[bi]; }
+                const double ms = timed([&] {
+                    launch_force(l, b->cls_t0[c], b->cls_nt[c], cand[bi], FMD_TERM_ALL, 0, FM_FAST);
+                });
+                tune_rows_[c].push_back(TuneRow{layout, cand[bi], ms});
+                const double cost = ms + 0.1 * share;
+                if (cost < best_ms[c]) {
+                    best_ms[c] = cost;
+                    best_layout[c] = layout;
+                    best_block[c] = cand[bi];
+                }
             }
-            tuned_[kern][c] = bestb;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    for (int t : tmp) destroy(t);
+    if (rc) return rc;
+    for (int c = 0; c < kNumClasses; c++) {
+        if (!b->cls_nt[c]) continue;
+        tuned_[0][c] = best_block[c];
+        if (tuned_[2][c] != best_layout[c]) {
+            tuned_[2][c] = best_layout[c];
+            apply_layout(c);
+        }
+    }
     tune_version_++;
-    return rc;
+    return FMD_OK;
 }
 
-int32_t Engine::set_tuned(int kernel, int cls, int block) {
-    if (kernel < 0 || kernel > 1 || cls < 0 || cls >= kNumClasses)
-        return fail(FMD_ERR_ARG, "kernel must be 0 or 1 and the size class in [0, 3)");
-    if (block != 64 && block != 128 && block != 256)
+void Engine::apply_layout(int cls) {
+    for (Replica* r : reps_)
+        if (r && r->cls == cls) {
+            r->desc.nostage = tuned_[2][cls] ? 0 : 1;
+            (void)hipMemcpy(d_descs_ + r->id, &r->desc, sizeof(RepDesc), hipMemcpyHostToDevice);
+        }
+}
+
+int32_t Engine::tune_times(int cls, int32_t* layout, int32_t* block, double* ms, int32_t cap) const {
+    if (cls < 0 || cls >= kNumClasses) return -1;
+    const auto& rows = tune_rows_[cls];
+    for (int k = 0; k < (int)rows.size() && k < cap; k++) {
+        if (layout) layout[k] = rows[k].layout;
+        if (block) block[k] = rows[k].block;
+        if (ms) ms[k] = rows[k].ms;
+    }
+    return (int32_t)rows.size();
+}
+
+int32_t Engine::set_tuned(int kernel, int cls, int value) {
+    if (kernel < 0 || kernel > 2 || cls < 0 || cls >= kNumClasses)
+        return fail(FMD_ERR_ARG, "kernel must be 0, 1 or 2 and the size class in [0, 3)");
+    if (kernel == 0 && value != 64 && value != 128 && value != 256)
         return fail(FMD_ERR_ARG, "block shape must be 64, 128 or 256");
-    if (kernel == 1 && block != kGroupSlots)
+    if (kernel == 1 && value != kGroupSlots)
         return fail(FMD_ERR_ARG, "the list build runs one workgroup of " + std::to_string(kGroupSlots) +
                                      " threads per group of tiles");
-    tuned_[kernel][cls] = block;
+    if (kernel == 2 && value != 0 && value != 1)
+        return fail(FMD_ERR_ARG, "the neighbor layout is 1 (staged) or 0 (direct)");
+    if (int32_t rc = sync()) return rc;
+    tuned_[kernel][cls] = value;
+    if (kernel == 2) apply_layout(cls);
     tune_version_++;
     return FMD_OK;
 }

@@ -128,6 +128,8 @@ struct RepDesc {
     float vlim;            // push-off speed limit (0 = off): |v| dt <= 0.02 sigma
     int clamp_lj;          // force/r clamp to +-1000 (src/force/lj.cu:48)
     int idx16;             // neighbor rows hold 16-bit slots (nslots <= 65536)
+    int nostage;           // the list build leaves rows as slots (the force kernel gathers
+                           // neighbors from HBM) instead of offsets into a staged neighbor set
     uint32_t seed_lo, seed_hi;
     // per-slot state (slot = position in the cell-sorted order)
     float4* pos;

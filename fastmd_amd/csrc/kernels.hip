@@ -1130,7 +1130,7 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
             for (int q = q0; q < q1; q++) c += s_used[q];
             int ncomp = 0;
             int idx = block_excl_scan(c, s_wsum, &ncomp);
-            compact = ncomp <= kGroupStage;
+            compact = ncomp <= kGroupStage && !d.nostage;
             for (int q = q0; q < q1; q++)
                 if (s_used[q]) {
                     s_cidx[q] = (unsigned short)idx;

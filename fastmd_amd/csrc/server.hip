@@ -150,9 +150,10 @@ class Server {
         return FMD_OK;
     }
     int32_t set_block(int kernel, int cls, int block) {
-        if (kernel < 0 || kernel > 1 || cls < 0 || cls >= kNumClasses) return FMD_ERR_ARG;
-        if (block != 64 && block != 128 && block != 256) return FMD_ERR_ARG;
+        if (kernel < 0 || kernel > 2 || cls < 0 || cls >= kNumClasses) return FMD_ERR_ARG;
+        if (kernel == 0 && block != 64 && block != 128 && block != 256) return FMD_ERR_ARG;
         if (kernel == 1 && block != kGroupSlots) return FMD_ERR_ARG;  // one workgroup per group of tiles
+        if (kernel == 2 && block != 0 && block != 1) return FMD_ERR_ARG;  // neighbor layout
         std::lock_guard<std::mutex> g(m_);
         blocks_.push_back({kernel, cls, block});
         return FMD_OK;

@@ -313,12 +313,27 @@ int32_t fmd_time_force(fmd_engine* e, const int32_t* sys, int32_t nsys, uint32_t
                        double* avg_ms, double* alg_bytes, int64_t* pairs);
 
 /* ----- autotuner -------------------------------------------------------- */
-/* Times the candidate block shapes (64, 128, 256 threads) of the force kernel
- * for each replica size class on the given systems and keeps the fastest; the
- * list build has one shape (256 threads, one group of four tiles). */
+/* Picks, per replica size class, the force pass's workgroup shape (64, 128 or
+ * 256 threads) and neighbor layout (the list build writes rows as offsets into
+ * each group's neighbor set staged in LDS, or as slots gathered from HBM),
+ * timing every candidate on copies of the given systems (force pass plus a
+ * tenth of the list build in that layout).  The systems themselves are not
+ * touched; every candidate computes the same bits.  The step then launches the
+ * force pass per class range with that class's shape; a new layout reaches a
+ * system at its next rebuild.  No reference counterpart (the reference
+ * hard-codes 256-thread blocks, src/force/lj.cu:116-118). */
 int32_t fmd_autotune(fmd_engine* e, const int32_t* sys, int32_t nsys, int32_t reps);
-/* Chosen block size for (kernel 0 = force, 1 = list build, size class). */
+/* Chosen value for (kernel, size class): kernel 0 = force block shape,
+ * 1 = list build block shape (always 256: one group of four tiles),
+ * 2 = neighbor layout (1 = staged set, 0 = direct slots). */
 int32_t fmd_tuned_block(const fmd_engine* e, int32_t kernel, int32_t size_class);
+/* Sets one tuned value (as fmd_tuned_block names them) by hand: the same
+ * bits whatever the value; a layout reaches a system at its next rebuild. */
+int32_t fmd_engine_set_tuned(fmd_engine* e, int32_t kernel, int32_t size_class, int32_t value);
+/* The last autotune's timings of a size class: up to cap rows of (layout,
+ * block, average force-pass ms); returns the row count. */
+int32_t fmd_autotune_times(const fmd_engine* e, int32_t size_class, int32_t* layout, int32_t* block, double* ms,
+                           int32_t cap);
 /* Size class of a system with natoms beads (0: < 8192, 1: < 32768, 2: rest). */
 int32_t fmd_size_class(int32_t natoms);
 
@@ -351,9 +366,10 @@ double fmd_server_now(const fmd_server* s);
 int32_t fmd_server_stats(const fmd_server* s, double* bead_steps, int64_t* graph_launches,
                          int32_t* resident, int32_t* queued);
 void fmd_server_destroy(fmd_server* s);
-/* Block shape the server's engine uses for (kernel 0 = force, 1 = list build,
- * size class), e.g. copied from fmd_tuned_block of an autotuned engine; takes
- * effect at the next batched launch.  No reference counterpart (the reference
+/* Tuned value the server's engine uses for (kernel 0 = force block shape,
+ * 1 = list build block shape, 2 = neighbor layout; size class), e.g. copied
+ * from fmd_tuned_block of an autotuned engine; takes effect at the next
+ * batched launch.  No reference counterpart (the reference
  * hard-codes 256-thread blocks, src/force/lj.cu:116-118). */
 int32_t fmd_server_set_block(fmd_server* s, int32_t kernel, int32_t size_class, int32_t block);
 

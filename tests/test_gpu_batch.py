@@ -131,3 +131,54 @@ def test_server_tight_budget_many_small_jobs(fmd, engine):
         live += x
         assert live <= 3
     srv.close()
+
+
+def test_autotune_per_class_is_bit_identical(fmd, engine):
+    """A batch spanning the three size classes (4,000, 10,000 and 35,000
+    beads).  Autotuning times its candidates on copies of the systems, so the
+    systems are untouched, and picks per class a force workgroup shape and a
+    neighbor layout.  Any setting -- the tuned one, or shapes of 64 / 128
+    threads with direct (unstaged) neighbor gathers set before creation or
+    mid-run -- gives the same bits as the defaults over 100 steps with
+    rebuilds."""
+    descs = [relax_kg(fmd, engine, n, 100, seed=80 + k, T=(1.0, 1.0), nsteps=1000)
+             for k, n in enumerate((40, 100, 350))]
+    assert [fmd.size_class(d.natoms) for d in descs] == [0, 1, 2]
+
+    def run(setup=None, tune=False, mid=None):
+        e = fmd.Engine(0)
+        if setup:
+            setup(e)
+        sids = [e.add(d) for d in descs]
+        info = None
+        if tune:
+            before = [e.state(s) for s in sids]
+            tuned = e.autotune(sids, reps=2)
+            for s, b in zip(sids, before):
+                assert all(np.array_equal(x, y) for x, y in zip(e.state(s), b))
+            info = (tuned, e.autotune_times())
+        e.step(sids, 40)
+        if mid:
+            mid(e)
+        e.step(sids, 60)
+        out = [e.state(s) for s in sids]
+        e.close()
+        return out, info
+
+    ref, _ = run()
+    got, (tuned, times) = run(tune=True)
+    assert sorted(times) == [0, 1, 2] and all(len(times[c]) == 6 for c in times)
+    assert all(tuned[(0, c)] in (64, 128, 256) and tuned[(2, c)] in (0, 1) for c in range(3))
+
+    def shapes(e):
+        for c, b in zip(range(3), (64, 128, 64)):
+            e.set_tuned(0, c, b)
+            e.set_tuned(2, c, 0)
+
+    def mid(e):
+        for c in range(3):
+            e.set_tuned(2, c, 1 - e.tuned()[(2, c)])
+
+    for variant in (got, run(setup=shapes)[0], run(mid=mid)[0]):
+        for a, b in zip(variant, ref):
+            assert all(np.array_equal(x, y) for x, y in zip(a, b))
