@@ -47,11 +47,13 @@ class SystemDesc(C.Structure):
 
 class Thermo(C.Structure):
     _fields_ = [("step", C.c_int64), ("ke", C.c_double), ("pe", C.c_double),
-                ("temp", C.c_double), ("press", C.c_double * 6), ("L", C.c_double)]
+                ("temp", C.c_double), ("press", C.c_double * 6), ("L", C.c_double),
+                ("max_v", C.c_double), ("max_f", C.c_double), ("max_v_id", C.c_int64), ("max_f_id", C.c_int64)]
 
     def as_dict(self):
         return {"step": self.step, "ke": self.ke, "pe": self.pe, "temp": self.temp,
-                "press": list(self.press), "L": self.L}
+                "press": list(self.press), "L": self.L, "max_v": self.max_v, "max_f": self.max_f,
+                "max_v_id": self.max_v_id, "max_f_id": self.max_f_id}
 
 
 MAX_CHAIN = 10

@@ -1293,6 +1293,14 @@ static void thermo_of(const RepState& s, int natoms, fmd_thermo* o) {
     o->temp = 2.0 * o->ke / 3.0;
     for (int c = 0; c < 6; c++) o->press[c] = (s.kin[c] + s.vir[c]) / V;
     o->L = L;
+    float m2;
+    int g;
+    max_key_split(s.vkey, &m2, &g);
+    o->max_v = std::sqrt((double)m2) * (double)s.v_pending;
+    o->max_v_id = g;
+    max_key_split(s.fkey, &m2, &g);
+    o->max_f = std::sqrt((double)m2);
+    o->max_f_id = g;
 }
 
 int32_t Engine::get_thermo(int32_t sys, fmd_thermo* out) {
@@ -1402,6 +1410,7 @@ int32_t Engine::drain_thermo(int32_t sys, fmd_thermo* out, int32_t cap, int32_t*
         o.step = t.step; o.ke = t.ke; o.pe = t.pe; o.temp = t.temp;
         for (int c = 0; c < 6; c++) o.press[c] = t.press[c];
         o.L = t.L;
+        o.max_v = t.maxv; o.max_f = t.maxf; o.max_v_id = t.maxv_id; o.max_f_id = t.maxf_id;
         (*n)++;
     }
     r->thermo_read = to;

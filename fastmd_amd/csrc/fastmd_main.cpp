@@ -119,7 +119,10 @@ int main(int argc, char** argv) {
                                  (long)t.step, t.ke, t.pe, t.temp, t.press[0], t.press[1],
                                  t.press[2], t.press[3], t.press[4], t.press[5]);
                 std::printf("\nStep %6ld  T=%10.4f  KE=%11.2f  PE=%11.2f  Etot=%11.2f  Pxx=%8.2f  maxV=%5.2f  maxF=%5.2f\n",
-                            (long)t.step, t.temp, t.ke, t.pe, t.ke + t.pe, t.press[0], 0.0, 0.0);
+                            (long)t.step, t.temp, t.ke, t.pe, t.ke + t.pe, t.press[0], t.max_v, t.max_f);
+                // the reference's spike diagnostics (src/main.cu:649-656)
+                if (t.max_v > 10.0) std::printf("  ** VELOCITY SPIKE: atom %ld v=%.2f\n", (long)t.max_v_id, t.max_v);
+                if (t.max_f > 100.0) std::printf("  ** FORCE SPIKE: atom %ld f=%.2f\n", (long)t.max_f_id, t.max_f);
             }
         } while (got == (int32_t)rec.size());
         if (tf) std::fflush(tf);

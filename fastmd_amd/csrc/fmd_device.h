@@ -29,7 +29,8 @@ constexpr int kBuildStage = 3072;
 constexpr int kGroupStage = 2048;
 constexpr int kMaxTypes = 16;
 constexpr int kMaxChain = 10;
-constexpr int kNumPart = 16;   // FP64 partials per tile: pe, W[6], K[6], pad
+constexpr int kNumPart = 16;   // FP64 partials per tile: pe, W[6], K[6], |v| and |F| max keys, pad
+constexpr int kNumSum = 13;    // the first kNumSum partials are sums, the next two maxima
 constexpr int kChunkTiles = 256;  // tiles summed by one workgroup in the first reduction level
 constexpr int kRebuildEvery = 20;  // forced rebuild period (src/main.cu:401)
 constexpr int kSlotBits = 26;      // slot / gid field of packed bonded entries
@@ -55,7 +56,30 @@ struct ThermoRec {
     double ke, pe, temp;
     double press[6];
     double L;
+    double maxv, maxf;         // largest |v| and |F| of one atom
+    int64_t maxv_id, maxf_id;  // its global id (the smallest on ties)
 };
+
+// Largest-magnitude key of a wave or replica reduction: the FP32 |x|^2 in the
+// high word and the complement of the global id in the low word, so the
+// largest key is the largest magnitude with the smallest id; read as a
+// double it is finite and positive and orders like the integer (the
+// reductions take fmax of doubles).
+__host__ __device__ inline double max_key(float mag2, int gid) {
+    union { float f; uint32_t u; } m;
+    m.f = mag2;
+    union { uint64_t u; double d; } k;
+    k.u = ((uint64_t)m.u << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)gid);
+    return k.d;
+}
+__host__ __device__ inline void max_key_split(double key, float* mag2, int* gid) {
+    union { uint64_t u; double d; } k;
+    k.d = key;
+    union { float f; uint32_t u; } m;
+    m.u = (uint32_t)(k.u >> 32);
+    *mag2 = m.f;
+    *gid = (k.u == 0) ? -1 : (int)(0xFFFFFFFFu - (uint32_t)k.u);
+}
 
 // Mutable per-replica scalar state.  Written by per-replica kernels (one lane),
 // read by the per-atom kernels of the next phase.
@@ -93,6 +117,7 @@ struct RepState {
     double ke_carry;
     // last reductions (after the post-force kick and scaling)
     double pe, vir[6], kin[6];
+    double vkey, fkey;     // max_key of |v|^2 (before the pending scale) and |F|^2
     int64_t rec_step;      // step of the last reduction
     int64_t obs_step;      // step of the last PE / virial evaluation
     // thermo ring

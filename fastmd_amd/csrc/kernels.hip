@@ -74,6 +74,17 @@ __device__ inline double wave_sum_last(double x) {
     x += dpp_move<0x143, 0xC>(x);  // row_bcast 31 into rows 2 and 3
     return x;
 }
+// The largest value of the wave in lane 63 (non-negative inputs: the lanes a
+// DPP move leaves without a source read 0).
+__device__ inline double wave_max_last(double x) {
+    x = fmax(x, dpp_move<0xB1, 0xF>(x));
+    x = fmax(x, dpp_move<0x4E, 0xF>(x));
+    x = fmax(x, dpp_move<0x114, 0xF>(x));
+    x = fmax(x, dpp_move<0x118, 0xF>(x));
+    x = fmax(x, dpp_move<0x142, 0xA>(x));
+    x = fmax(x, dpp_move<0x143, 0xC>(x));
+    return x;
+}
 __device__ inline int wave_min_i(int x) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o, 64));
@@ -1460,6 +1471,7 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
     // steps on request), and neither does a forces-only launch
     const bool partials = mode == FM_OBS || (mode == FM_STEP && (d.ensemble != 0 || st->want_obs));
     double k0 = 0, k1 = 0, k2 = 0, k3 = 0, k4 = 0, k5 = 0;
+    double vkey = 0.0, fkey = 0.0;  // max_key of |v|^2 and |F|^2 (src/analysis/thermo.cu:26-75)
     int err = 0;
     if (i < d.natoms) {
         const int4 pi = d.ipos[i];
@@ -1615,6 +1627,9 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
             const double md = m, vx = v.x, vy = v.y, vz = v.z;
             k0 = md * vx * vx; k1 = md * vx * vy; k2 = md * vx * vz;
             k3 = md * vy * vy; k4 = md * vy * vz; k5 = md * vz * vz;
+            const int g = d.gid[i];
+            vkey = max_key(v.x * v.x + v.y * v.y + v.z * v.z, g);
+            fkey = max_key(fx * fx + fy * fy + fz * fz, g);
         }
         if (ahead) {  // bit for bit the kick of k_pre (explicit FMAs there too)
             v.x = fmaf(hdtm, fx, v.x); v.y = fmaf(hdtm, fy, v.y); v.z = fmaf(hdtm, fz, v.z);
@@ -1623,13 +1638,18 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
         if (kick) d.vel[i] = v;
     }
     if (partials) {
-        double part[13] = {(double)pe, (double)w0, (double)w1, (double)w2, (double)w3, (double)w4,
-                           (double)w5, k0, k1, k2, k3, k4, k5};
+        double part[kNumSum] = {(double)pe, (double)w0, (double)w1, (double)w2, (double)w3, (double)w4,
+                                (double)w5, k0, k1, k2, k3, k4, k5};
 #pragma unroll
-        for (int c = 0; c < 13; c++) part[c] = wave_sum_last(part[c]);
-        if (lane == 63)
+        for (int c = 0; c < kNumSum; c++) part[c] = wave_sum_last(part[c]);
+        vkey = wave_max_last(vkey);
+        fkey = wave_max_last(fkey);
+        if (lane == 63) {
 #pragma unroll
-            for (int c = 0; c < 13; c++) d.wpart[(size_t)t.lt * kNumPart + c] = part[c];
+            for (int c = 0; c < kNumSum; c++) d.wpart[(size_t)t.lt * kNumPart + c] = part[c];
+            d.wpart[(size_t)t.lt * kNumPart + kNumSum] = vkey;
+            d.wpart[(size_t)t.lt * kNumPart + kNumSum + 1] = fkey;
+        }
     }
     if (err) atomicOr(&st->err, err);  // failures only
 }
@@ -1640,26 +1660,29 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
 // over many CUs.
 __global__ void __launch_bounds__(256) k_chunk(const RepDesc* __restrict__ descs,
                                               const Tile* __restrict__ chunks, int nch, int mode) {
-    __shared__ double red[13][4];
+    constexpr int kN = kNumSum + 2;
+    __shared__ double red[kN][4];
     const Tile ch = chunks[blockIdx.x];
     const RepDesc& d = descs[ch.rep];
     if (mode == BND_STEP && (!d.st->active || (d.ensemble == 0 && !d.st->want_obs))) return;  // nothing to sum
     const int tl = ch.lt * kChunkTiles + threadIdx.x;
-    double a[13];
+    double a[kN];
 #pragma unroll
-    for (int c = 0; c < 13; c++) a[c] = 0.0;
+    for (int c = 0; c < kN; c++) a[c] = 0.0;
     if (tl < d.ntiles)
 #pragma unroll
-        for (int c = 0; c < 13; c++) a[c] = d.wpart[(size_t)tl * kNumPart + c];
+        for (int c = 0; c < kN; c++) a[c] = d.wpart[(size_t)tl * kNumPart + c];
 #pragma unroll
-    for (int c = 0; c < 13; c++) a[c] = wave_sum_last(a[c]);
+    for (int c = 0; c < kN; c++) a[c] = c < kNumSum ? wave_sum_last(a[c]) : wave_max_last(a[c]);
     if (lane_id() == 63)
 #pragma unroll
-        for (int c = 0; c < 13; c++) red[c][threadIdx.x >> 6] = a[c];
+        for (int c = 0; c < kN; c++) red[c][threadIdx.x >> 6] = a[c];
     __syncthreads();
-    if (threadIdx.x < 13) {
+    if (threadIdx.x < kN) {
         const int c = threadIdx.x;
-        d.cpart[(size_t)ch.lt * kNumPart + c] = ((red[c][0] + red[c][1]) + red[c][2]) + red[c][3];
+        d.cpart[(size_t)ch.lt * kNumPart + c] =
+            c < kNumSum ? ((red[c][0] + red[c][1]) + red[c][2]) + red[c][3]
+                        : fmax(fmax(red[c][0], red[c][1]), fmax(red[c][2], red[c][3]));
     }
 }
 
@@ -1737,15 +1760,20 @@ __global__ void __launch_bounds__(64) k_nh_chain_run(RepState* st, const double*
 // sample (written to smp).
 __device__ int boundary_tail(const RepDesc& d, RepState* st, int mode, const double (*red)[kBndThreads / 64],
                              double* smp) {
-    double tot[13];
-    for (int c = 0; c < 13; c++) {
+    double tot[kNumSum + 2];
+    for (int c = 0; c < kNumSum + 2; c++) {
         tot[c] = 0.0;
-        for (int w = 0; w < kBndThreads / 64; w++) tot[c] += red[c][w];
+        for (int w = 0; w < kBndThreads / 64; w++) tot[c] = c < kNumSum ? tot[c] + red[c][w] : fmax(tot[c], red[c][w]);
     }
     const double pe = tot[0];
     double W[6], K[6];
     for (int c = 0; c < 6; c++) { W[c] = tot[1 + c]; K[c] = tot[7 + c]; }
     const double KE = 0.5 * (K[0] + K[3] + K[5]);
+    const bool reduced = !(mode == BND_STEP && d.ensemble == 0 && !st->want_obs);
+    if (reduced) {
+        st->vkey = tot[kNumSum];
+        st->fkey = tot[kNumSum + 1];
+    }
     if (mode == BND_REDUCE) {
         st->obs_step = st->step;
         st->pe = pe;
@@ -1804,6 +1832,14 @@ __device__ int boundary_tail(const RepDesc& d, RepState* st, int mode, const dou
         r.temp = 2.0 * r.ke / 3.0;  // src/analysis/thermo.cu:178
         for (int c = 0; c < 6; c++) r.press[c] = (st->kin[c] + W[c]) / V;
         r.L = Ld;
+        float m2;
+        int g;
+        max_key_split(st->vkey, &m2, &g);
+        r.maxv = sqrt((double)m2) * (double)st->v_pending;
+        r.maxv_id = g;
+        max_key_split(st->fkey, &m2, &g);
+        r.maxf = sqrt((double)m2);
+        r.maxf_id = g;
         d.thermo[st->thermo_written % st->thermo_cap] = r;
         st->thermo_written++;
     }
@@ -1831,7 +1867,7 @@ __device__ int boundary_tail(const RepDesc& d, RepState* st, int mode, const dou
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBndThreads) k_boundary(const RepDesc* __restrict__ descs,
                                                           const int* __restrict__ reps, int nr, int mode) {
-    __shared__ double red[13][kBndThreads / 64];
+    __shared__ double red[kNumSum + 2][kBndThreads / 64];
     __shared__ double smp[6];
     __shared__ int push;
     const RepDesc& d = descs[reps[blockIdx.x]];
@@ -1849,12 +1885,17 @@ __global__ void __launch_bounds__(kBndThreads) k_boundary(const RepDesc* __restr
         // a Langevin step that records nothing reduced no partials (k_force)
         const bool none = mode == BND_STEP && d.ensemble == 0 && !st->want_obs;
         const int lane = lane_id(), wv = threadIdx.x >> 6, c = lane & 15;
+        const bool mx = c >= kNumSum;  // the two maximum keys
         double a = 0.0;
-        for (int ck = 4 * wv + (lane >> 4); !none && ck < d.nchunks; ck += 4 * (kBndThreads / 64))
-            a += d.cpart[(size_t)ck * kNumPart + c];
-        a += __shfl_xor(a, 16, 64);
-        a += __shfl_xor(a, 32, 64);
-        if (lane < 13) red[lane][wv] = a;
+        for (int ck = 4 * wv + (lane >> 4); !none && ck < d.nchunks; ck += 4 * (kBndThreads / 64)) {
+            const double x = d.cpart[(size_t)ck * kNumPart + c];
+            a = mx ? fmax(a, x) : a + x;
+        }
+        double o = __shfl_xor(a, 16, 64);
+        a = mx ? fmax(a, o) : a + o;
+        o = __shfl_xor(a, 32, 64);
+        a = mx ? fmax(a, o) : a + o;
+        if (lane < kNumSum + 2) red[lane][wv] = a;
     }
     __syncthreads();
     if (threadIdx.x == 0) push = boundary_tail(d, st, mode, red, smp);

@@ -99,12 +99,16 @@ typedef struct fmd_system_desc {
 #define FMD_CORR_POINTS 16
 
 /* One thermo sample (src/analysis/thermo.cu:140-201): per-atom KE and PE,
- * T = 2 KE/(3N), pressure tensor P_ab = (sum m v_a v_b + W_ab)/V. */
+ * T = 2 KE/(3N), pressure tensor P_ab = (sum m v_a v_b + W_ab)/V, and the
+ * largest per-atom |v| and |F| with that atom's global id (max_vel_force_kernel,
+ * src/analysis/thermo.cu:26-75; here the smallest id on ties, -1 if none). */
 typedef struct fmd_thermo {
     int64_t step;
     double ke, pe, temp;
     double press[6];           /* xx, xy, xz, yy, yz, zz */
     double L;
+    double max_v, max_f;
+    int64_t max_v_id, max_f_id;
 } fmd_thermo;
 
 /* ----- library ----------------------------------------------------------- */

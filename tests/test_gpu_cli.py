@@ -50,6 +50,14 @@ restart 200 restart.data
     # stdout contract parsed by benchmarks/run.py
     assert re.search(r"Wall time: [0-9.]+ s\s+\|\s+Steps/s: [0-9.]+\s+\|\s+ns/day: [0-9.]+", out)
     assert len(re.findall(r"^Step\s+\d+\s+T=", out, re.M)) == 8
+    # maxV / maxF of the thermo line (src/main.cu:645-648): a relaxed melt at
+    # T = 1 has its fastest bead at a few sigma/tau and its largest force well
+    # above zero; the spike lines appear only above 10 and 100
+    mv = [float(x) for x in re.findall(r"maxV=\s*([0-9.]+)", out)]
+    mf = [float(x) for x in re.findall(r"maxF=\s*([0-9.]+)", out)]
+    assert len(mv) == 8 and len(mf) == 8
+    assert all(2.0 < x < 10.0 for x in mv) and all(10.0 < x for x in mf)
+    assert ("FORCE SPIKE" in out) == any(x > 100.0 for x in mf) and "VELOCITY SPIKE" not in out
     # thermo file: header and 10 columns, every 50 steps
     rows = [l for l in (tmp_path / "thermo.dat").read_text().splitlines()]
     assert rows[0] == "# step  KE  PE  T  Pxx  Pyy  Pzz  Pxy  Pxz  Pyz"

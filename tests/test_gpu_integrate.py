@@ -214,3 +214,27 @@ def test_thermo_after_unobserved_steps(fmd, engine):
     scale = np.max(np.abs(kin) + np.abs(np.asarray(vir))) / V
     assert np.max(np.abs(np.asarray(t["press"]) - (kin + np.asarray(vir)) / V)) <= 1e-9 * scale
     engine.remove(sid)
+
+
+@pytest.mark.parametrize("ensemble", [0, 1])
+def test_thermo_largest_velocity_and_force(fmd, engine, ensemble):
+    """The thermo record's maxV and maxF (max_vel_force_kernel,
+    src/analysis/thermo.cu:26-75, printed by src/main.cu:645-656) are the
+    largest |v| and |F| of one atom at that step, with that atom's global id:
+    checked against the returned velocities and a fresh force evaluation at
+    the returned positions."""
+    d = relax_kg(fmd, engine, 12, 40, seed=79, ensemble=ensemble, T=(1.0, 1.0),
+                 Tdamp=(1.0 if ensemble == 0 else 0.1), nsteps=100, thermo_freq=10)
+    sid = engine.add(d)
+    engine.step(sid, 30)
+    rec, _ = engine.drain_thermo(sid)
+    t = rec[-1]
+    assert t["step"] == engine.steps_done(sid)
+    p, v, _ = engine.state(sid)
+    f, _, _ = engine.forces(sid)
+    engine.remove(sid)
+    sv = np.sqrt(np.sum(v.astype(np.float64) ** 2, axis=1))
+    sf = np.sqrt(np.sum(f[:, :3].astype(np.float64) ** 2, axis=1))
+    assert abs(t["max_v"] - sv.max()) <= 1e-6 * sv.max() and t["max_v_id"] == int(np.argmax(sv))
+    assert abs(t["max_f"] - sf.max()) <= 1e-6 * sf.max() and t["max_f_id"] == int(np.argmax(sf))
+    assert all(r["max_v"] > 0 and r["max_f"] > 0 for r in rec)
