@@ -938,6 +938,23 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
             return (wz * ey_ + wy) * ex_ + wx;
         };
         const int W = ex_ * ey_ * ez_;
+        // The staged index of a bonded partner (slot ps): its cell is one of the
+        // 27 around this atom's (a bond is shorter than a cell), found through
+        // the window; -1 if not (the group then keeps slots everywhere).
+        auto partner_q = [&](int ps) __attribute__((always_inline)) -> int {
+            const float4 pp = gpos[ps];
+            int dx = cell_coord(pp.x, ics, nx) - cx, dy = cell_coord(pp.y, ics, nx) - cy,
+                dz = cell_coord(pp.z, ics, nx) - cz;
+            dx += (dx < -1) ? nx : 0; dx -= (dx > 1) ? nx : 0;
+            dy += (dy < -1) ? nx : 0; dy -= (dy > 1) ? nx : 0;
+            dz += (dz < -1) ? nx : 0; dz -= (dz > 1) ? nx : 0;
+            if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) return -1;
+            const int u = s_wmap[win_of((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1))];
+            if (u >= s_nu) return -1;
+            const int4 ce = s_cell[u];
+            const int o = ps - ce.x;
+            return (o >= 0 && o < ce.y) ? ce.z + o : -1;
+        };
         // staged scan: prefilter in fixed point; boxes of fewer than 3 cells per
         // axis (where a pair can sit near half the box) use the global scan
         bool staged = W <= kMaxWin && nx >= 3;  // uniform over the group
@@ -1131,17 +1148,36 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
         int selfent = i;  // the padding entry: the atom itself (rejected by r^2 = 0)
         bool compact = false;
         if (staged) {
-            // the group's neighbor set: used staged atoms (and every atom itself),
+            // the group's neighbor set: used staged atoms, every atom itself and
+            // its bonded partners (the force kernel reads them from the set too),
             // numbered in staged (slot) order
-            if (valid) s_used[qself] = 1;
-            __syncthreads();  // the scan's last reads of s_pxy precede the s_cidx writes below
+            int pmiss = 0;
+            if (valid) {
+                s_used[qself] = 1;
+                const int hv = gview(d.bhdr)[i], nbk = hv & 0xFF, nak = (hv >> 8) & 0xFF;
+                const FMD_GLOBAL int2* at = gview(d.ang) + (size_t)t.lt * d.acap * kTile + lane;
+                for (int b = 0; b < nbk; b++) {
+                    const int q = partner_q(bt[b * kTile] & kSlotMask);
+                    if (q < 0) pmiss = 1;
+                    else s_used[q] = 1;
+                }
+                for (int a = 0; a < nak; a++) {
+                    const int2 e = at[a * kTile];
+                    const int q1 = partner_q(e.x & kSlotMask), q2 = partner_q(e.y & kSlotMask);
+                    if (q1 < 0 || q2 < 0) pmiss = 1;
+                    if (q1 >= 0) s_used[q1] = 1;
+                    if (q2 >= 0) s_used[q2] = 1;
+                }
+            }
+            // the scan's last reads of s_pxy precede the s_cidx writes below
+            pmiss = __syncthreads_or(pmiss);
             const int per = (nstage + kGroupSlots - 1) / kGroupSlots;
             const int q0 = min(tid * per, nstage), q1 = min(q0 + per, nstage);
             int c = 0;
             for (int q = q0; q < q1; q++) c += s_used[q];
             int ncomp = 0;
             int idx = block_excl_scan(c, s_wsum, &ncomp);
-            compact = ncomp <= kGroupStage && !d.nostage;
+            compact = ncomp <= kGroupStage && !d.nostage && !pmiss;
             for (int q = q0; q < q1; q++)
                 if (s_used[q]) {
                     s_cidx[q] = (unsigned short)idx;
@@ -1166,7 +1202,24 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
         // (or to slots when the neighbor set did not fit).
         s_len[tid] = padded;
         __syncthreads();
-        if (valid && compact) selfent = (int)s_cidx[qself] << 4;
+        if (valid && compact) {
+            selfent = (int)s_cidx[qself] << 4;
+            const int hv = gview(d.bhdr)[i], nbk = hv & 0xFF, nak = (hv >> 8) & 0xFF;
+            const FMD_GLOBAL int2* at = gview(d.ang) + (size_t)t.lt * d.acap * kTile + lane;
+            // bonded entries of a staged group: LDS byte offsets of the partners
+            // in the set instead of slots (role and type bits kept)
+            for (int b = 0; b < nbk; b++) {
+                const int e = bt[b * kTile];
+                gmut(d.bnd)[((size_t)t.lt * d.bcap + b) * kTile + lane] =
+                    ((int)s_cidx[partner_q(e & kSlotMask)] << 4) | (e & ~kSlotMask);
+            }
+            for (int a = 0; a < nak; a++) {
+                const int2 e = at[a * kTile];
+                gmut(d.ang)[((size_t)t.lt * d.acap + a) * kTile + lane] =
+                    make_int2(((int)s_cidx[partner_q(e.x & kSlotMask)] << 4) | (e.x & ~kSlotMask),
+                              ((int)s_cidx[partner_q(e.y & kSlotMask)] << 4) | (e.y & ~kSlotMask));
+            }
+        }
         int rank = 0, maxpad = 0;
         for (int b = 0; b < kGroupSlots; b++) {
             const int lb = s_len[b];
@@ -1498,6 +1551,19 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
             }
         }
         const int hdr = (terms & 6) ? d.bhdr[i] : 0;
+        // a bonded entry of a staged group is the partner's LDS byte offset in
+        // the group's neighbor set (read from the set, or through its slot
+        // list when this variant did not stage it); otherwise a slot
+        const bool bstaged = (terms & 6) && (sp || d.gcount[grp] >= 0);
+        const FMD_GLOBAL unsigned short* gs16 =
+            (const FMD_GLOBAL unsigned short*)gview(d.gstage) + (size_t)grp * kGroupStage;
+        const FMD_GLOBAL int* gs32 = gview(d.gstage) + (size_t)grp * kGroupStage;
+        auto partner = [&](int e) __attribute__((always_inline)) -> int4 {
+            const int v = e & kSlotMask;
+            if (sp) return *(const int4*)((const char*)sp + v);
+            if (bstaged) return d.ipos[d.idx16 ? (int)gs16[v >> 4] : gs32[v >> 4]];
+            return d.ipos[v];
+        };
         // The bonded terms in the order bonds, then angles, each over its rows;
         // the entries and partner positions of the first kBondAhead bonds and
         // kAngleAhead angles (all of a Kremer-Grest bead's) are requested
@@ -1595,23 +1661,23 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
         const int2 ea0 = (na > 0) ? at[0] : make_int2(0, 0), ea1 = (na > 1) ? at[kTile] : make_int2(0, 0),
                    ea2 = (na > 2) ? at[2 * kTile] : make_int2(0, 0);
         int4 pb0 = pi, pb1 = pi, qa0 = pi, qc0 = pi, qa1 = pi, qc1 = pi, qa2 = pi, qc2 = pi;
-        if (nb > 0) pb0 = d.ipos[eb0 & kSlotMask];
-        if (nb > 1) pb1 = d.ipos[eb1 & kSlotMask];
-        if (na > 0) { qa0 = d.ipos[ea0.x & kSlotMask]; qc0 = d.ipos[ea0.y & kSlotMask]; }
-        if (na > 1) { qa1 = d.ipos[ea1.x & kSlotMask]; qc1 = d.ipos[ea1.y & kSlotMask]; }
-        if (na > 2) { qa2 = d.ipos[ea2.x & kSlotMask]; qc2 = d.ipos[ea2.y & kSlotMask]; }
+        if (nb > 0) pb0 = partner(eb0);
+        if (nb > 1) pb1 = partner(eb1);
+        if (na > 0) { qa0 = partner(ea0.x); qc0 = partner(ea0.y); }
+        if (na > 1) { qa1 = partner(ea1.x); qc1 = partner(ea1.y); }
+        if (na > 2) { qa2 = partner(ea2.x); qc2 = partner(ea2.y); }
         if (nb > 0) bond_term(eb0, pb0);
         if (nb > 1) bond_term(eb1, pb1);
         for (int b = kBondAhead; b < nb; b++) {
             const int e = bt[b * kTile];
-            bond_term(e, d.ipos[e & kSlotMask]);
+            bond_term(e, partner(e));
         }
         if (na > 0) angle_term(ea0, qa0, qc0);
         if (na > 1) angle_term(ea1, qa1, qc1);
         if (na > 2) angle_term(ea2, qa2, qc2);
         for (int a = kAngleAhead; a < na; a++) {
             const int2 e = at[a * kTile];
-            angle_term(e, d.ipos[e.x & kSlotMask], d.ipos[e.y & kSlotMask]);
+            angle_term(e, partner(e.x), partner(e.y));
         }
         // Langevin steps followed by another step of the run also give that
         // step's pre-force half kick (the same force, the same operations as
