@@ -369,6 +369,18 @@ def test_nh_reference_conserved_quantity(orc, golden):
     assert xr[0, 4] == 3 * 64 * 1.0 * 1000.0 ** 2 and xr[0, 5] == 1000.0 ** 2
 
 
+def test_lj_cut_energy_closed_form(orc):
+    """The cutoff shift used by the conservation checks: one LJ energy at rc
+    per non-excluded pair inside rc, nothing for a pair outside or a bonded
+    (excluded) pair."""
+    e_rc = 4.0 * (2.5 ** -12 - 2.5 ** -6)
+    p4 = pos4_of([[1.0, 1.0, 1.0], [3.0, 1.0, 1.0], [1.0, 3.6, 1.0], [5.0, 5.0, 5.0]])
+    assert abs(orc.lj_cut_energy(p4, F(10.0), KG_LJ) - e_rc) < 1e-15
+    assert orc.lj_cut_energy(p4, F(10.0), KG_LJ, bonds=[[0, 1]]) == 0.0
+    p4 = pos4_of([[0.5, 1.0, 1.0], [9.0, 1.0, 1.0]])  # 1.5 apart through the boundary
+    assert abs(orc.lj_cut_energy(p4, F(10.0), KG_LJ) - e_rc) < 1e-15
+
+
 def _drifts(orc, ensemble, t_end=0.5, form=1, shifted=True, **kw):
     p4, v, L, b, a = lattice_chains(7)
     out = []
