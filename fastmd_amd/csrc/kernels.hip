@@ -771,12 +771,16 @@ __device__ __forceinline__ void build_chunk_fix(const unsigned (&jxy)[kBuildChun
                                                 int bcap, const FMD_GLOBAL int* bt, FMD_GLOBAL int* out,
                                                 FMD_GLOBAL unsigned short* out16, int kcap, int& cnt, int ebase,
                                                 unsigned char* used) {
+    // the bits come from the sign of q - bound (q >= 0, bounds in [0, 2^31 - 1):
+    // no overflow), plain integer VALU work: no compare-to-mask / select pairs
+    // and their wait states
     unsigned m = 0, band = 0;
+    const int chi1 = chi + 1;
 #pragma unroll
     for (int k = 0; k < kBuildChunk; k++) {
         const int q = coarse_q(pxy, pz, jxy[k], jz[k]);
-        m |= (q < clo) ? (1u << k) : 0u;
-        band |= (q <= chi) ? (1u << k) : 0u;
+        m |= ((unsigned)(q - clo) >> 31) << k;      // q < clo
+        band |= ((unsigned)(q - chi1) >> 31) << k;  // q <= chi
     }
     const unsigned valid = n >= kBuildChunk ? (1u << kBuildChunk) - 1u : (1u << n) - 1u;  // k < n
     m &= valid;
