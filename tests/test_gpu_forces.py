@@ -109,3 +109,49 @@ def test_kg_melt_all_terms(fmd, engine, orc, nch, clen, seed):
     f, vir, pe = engine.forces(sid, fmd.TERM_ALL)
     assert np.all(np.abs(f[:, :3].astype(np.float64).sum(axis=0)) < 1e-5 * np.abs(f8[:, :3]).sum() / len(f8) * 10)
     engine.remove(sid)
+
+
+# three bead types with distinct LJ parameters per type pair (symmetric)
+LJ3 = np.array([[1.0, 1.0], [0.75, 1.1], [1.2, 0.9],
+                [0.75, 1.1], [0.5, 1.2], [0.9, 1.05],
+                [1.2, 0.9], [0.9, 1.05], [1.5, 0.8]], np.float32).reshape(-1)
+
+
+def test_lj_three_types_distinct_parameters(fmd, engine, orc):
+    """Per-type-pair epsilon and sigma (three types, every pair different):
+    a dense bonded system from a lattice start, forces/PE/virial against the
+    FP64 oracle at 1e-5 and the literal FP32 restatement of
+    reference/cpu_reference.hpp:95-153 (ti * ntypes + tj indexing of the
+    parameter table), and 40 Langevin steps with per-type masses against the
+    oracle driver."""
+    from conftest import lattice_chains
+    p4, v, L, b, a = lattice_chains(8, rho=0.7, seed=5)
+    rng = np.random.default_rng(6)
+    types = rng.integers(0, 3, len(p4))
+    p4 = pos4_of(p4[:, :3], types)
+    masses = np.array([1.0, 2.0, 0.5], np.float32)
+    d = fmd.Desc(p4, L, LJ3, ntypes=3, vel=v, bonds=b, bond_params=KG_BP, angles=a, angle_params=KG_AP,
+                 masses=masses, seed=11, T=(1.0, 1.0), Tdamp=1.0, nsteps=1000, thermo_freq=1)
+    sid = engine.add(d)
+    f, vir, pe = engine.forces(sid, fmd.TERM_LJ)
+    f8, v8 = orc.forces_f64(p4, L, LJ3, ntypes=3, terms=1)
+    _f64_check(f, vir, pe, f8, v8)
+    fl, _ = orc.lj_literal(p4, LJ3, 3, 2.5, L)
+    floor = np.sqrt(np.mean(np.sum(f8[:, :3] ** 2, axis=1)))
+    assert rel_err_floor(fl[:, :3], f8[:, :3], floor) < 1e-4
+    f, vir, pe = engine.forces(sid)
+    f8, v8 = orc.forces_f64(p4, L, LJ3, ntypes=3, bonds=b, bp=KG_BP, ang4=a, ap=KG_AP, terms=7)
+    _f64_check(f, vir, pe, f8, v8)
+    engine.step(sid, 40)
+    pd, vd, _ = engine.state(sid)
+    rec, _ = engine.drain_thermo(sid)
+    engine.remove(sid)
+    sysd = {"pos4": p4.copy(), "vel3": v.copy(), "masses": np.resize(masses, 16), "lj": LJ3, "ntypes": 3,
+            "bonds": b, "btype": np.zeros(len(b), np.int32), "bp": KG_BP, "ang4": a, "ap": KG_AP, "L": float(L),
+            "ensemble": 0, "seed": 11, "nsteps_total": 1000}
+    po, vo, _, Lo, th = orc.md_run(sysd, 40)
+    dx = pd[:, :3].astype(np.float64) - po[:, :3]
+    dx -= Lo * np.rint(dx / Lo)
+    assert np.max(np.abs(dx)) < 2e-4 and np.max(np.abs(vd - vo)) < 5e-3
+    for k, r in enumerate(rec):
+        assert abs(r["temp"] - th[k, 3]) <= 2e-4 * th[k, 3]

@@ -295,6 +295,25 @@ def test_literal_lj_matches_f64(orc, golden):
     assert err.max() < 1e-3
 
 
+def test_literal_lj_three_types_matches_f64(orc):
+    """Three types with distinct (eps, sigma) per type pair: the literal FP32
+    brute force and the FP64 cell-list path index the table as
+    lj[ti * ntypes + tj] and agree; swapping two types' parameters changes
+    the forces (the table is read per pair)."""
+    lj3 = np.array([[1.0, 1.0], [0.75, 1.1], [1.2, 0.9], [0.75, 1.1], [0.5, 1.2], [0.9, 1.05],
+                    [1.2, 0.9], [0.9, 1.05], [1.5, 0.8]], F).reshape(-1)
+    p4, _, L, _, _ = lattice_chains(6, rho=0.7, seed=2)
+    types = np.random.default_rng(3).integers(0, 3, len(p4))
+    p4 = pos4_of(p4[:, :3], types)
+    fl, vl = orc.lj_literal(p4, lj3, 3, 2.5, L)
+    f8, v8 = orc.forces_f64(p4, L, lj3, ntypes=3, terms=1)
+    assert np.allclose(fl, f8, rtol=1e-4, atol=1e-4 * np.abs(f8).max())
+    assert np.allclose(vl, v8, rtol=1e-4, atol=1e-4 * np.abs(v8).max())
+    sw = lj3.reshape(3, 3, 2)[[1, 0, 2]][:, [1, 0, 2]].reshape(-1)
+    fs, _ = orc.forces_f64(p4, L, sw, ntypes=3, terms=1)
+    assert np.max(np.abs(fs - f8)) > 1e-2 * np.abs(f8).max()
+
+
 def test_literal_bonded_match_f64(orc):
     p4 = pos4_of([[2.0, 5.0, 5.0], [3.0, 5.0, 5.0], [4.2, 5.0, 5.0]])
     fl, vl = orc.fene_literal(p4, [[0, 1], [1, 2]], [0, 0], KG_BP, F(10.0))
