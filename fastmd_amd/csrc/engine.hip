@@ -983,6 +983,7 @@ int32_t Engine::get_pairs(int32_t sys, int64_t* npairs, int32_t* ij, int64_t cap
                 j = D.idx16 ? (int)gs16[q] : gstage[(size_t)g * kGroupStage + q];
             }
             if (j < 0 || j >= D.natoms) return fail(FMD_ERR_STATE, "corrupt neighbor entry");
+            if (j == i) continue;  // a candidate the list build rejected after writing it
             pairs.emplace_back(img[i].w, img[j].w);
         }
     }

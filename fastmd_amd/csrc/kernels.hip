@@ -696,17 +696,25 @@ __device__ inline int wrap_cell(int c, int o, int nx) {
 // skipping the atom itself and its bonded partners.  Entry k is stored as
 // ebase + k (a staged index or a slot) and, with `used`, marks used[ebase + k].
 // W16 rows hold 16-bit entries (replicas of at most 65536 slots), others 32-bit.
+// The bits of candidates q0 + k that are the atom itself or a bonded partner.
+__device__ __forceinline__ unsigned excluded_mask(unsigned m, int q0, int i, const int (&ex)[4], int nb, int bcap,
+                                                  const FMD_GLOBAL int* bt) {
+    unsigned x = 0u;
+    auto drop = [&](int e) {
+        const unsigned o = (unsigned)(e - q0);
+        x |= (o < (unsigned)kBuildChunk) ? (1u << o) : 0u;
+    };
+    drop(i); drop(ex[0]); drop(ex[1]); drop(ex[2]); drop(ex[3]);
+    for (int b = 4; b < nb && b < bcap; b++) drop(bt[b * kTile] & kSlotMask);
+    return m & x;
+}
+
 template <bool W16>
 __device__ __forceinline__ void append_chunk(unsigned m, int q0, int i, const int (&ex)[4], int nb, int bcap,
                                              const FMD_GLOBAL int* bt, FMD_GLOBAL int* out,
                                              FMD_GLOBAL unsigned short* out16, int kcap, int& cnt, int ebase,
                                              unsigned char* used) {
-    auto drop = [&](int e) {
-        const unsigned o = (unsigned)(e - q0);
-        m &= (o < (unsigned)kBuildChunk) ? ~(1u << o) : ~0u;
-    };
-    drop(i); drop(ex[0]); drop(ex[1]); drop(ex[2]); drop(ex[3]);
-    for (int b = 4; b < nb && b < bcap; b++) drop(bt[b * kTile] & kSlotMask);
+    m &= ~excluded_mask(m, q0, i, ex, nb, bcap, bt);
     // one pass per set bit, lowest first: a wave runs as many passes as its
     // fullest lane has pairs in the chunk (about 4 of 8)
     while (m) {
@@ -763,6 +771,13 @@ __device__ __forceinline__ unsigned coarse_xy(int4 p) {
     return ((unsigned)p.x >> 16) | ((unsigned)p.y & 0xFFFF0000u);
 }
 __device__ __forceinline__ unsigned coarse_z(int4 p) { return (unsigned)p.z >> 16; }
+// A candidate the prefilter leaves undecided needs the reference distance on
+// its float position, a global load.  The first such candidate of a lane is
+// appended as inside and recorded in pend (row position << 12 | staged index);
+// after the scan resolve_pending decides it, with one load round trip for the
+// whole wave, and turns it into the atom's own entry (which every consumer
+// skips) if it lies outside.  Further undecided candidates of the lane are
+// decided at once.
 template <bool W16>
 __device__ __forceinline__ void build_chunk_fix(const unsigned (&jxy)[kBuildChunk], const unsigned (&jz)[kBuildChunk],
                                                 unsigned pxy, unsigned pz, int clo, int chi,
@@ -770,7 +785,7 @@ __device__ __forceinline__ void build_chunk_fix(const unsigned (&jxy)[kBuildChun
                                                 float L, float invL, float rcs2, const int (&ex)[4], int nb,
                                                 int bcap, const FMD_GLOBAL int* bt, FMD_GLOBAL int* out,
                                                 FMD_GLOBAL unsigned short* out16, int kcap, int& cnt, int ebase,
-                                                unsigned char* used) {
+                                                unsigned char* used, int& pend) {
     // the bits come from the sign of q - bound (q >= 0, bounds in [0, 2^31 - 1):
     // no overflow), plain integer VALU work: no compare-to-mask / select pairs
     // and their wait states
@@ -786,6 +801,14 @@ __device__ __forceinline__ void build_chunk_fix(const unsigned (&jxy)[kBuildChun
     m &= valid;
     band &= valid & ~m;
     if (band) {
+        band &= ~excluded_mask(band, q0, i, ex, nb, bcap, bt);
+        if (band && pend < 0) {  // defer the first: inside for now, position recorded
+            const int k = __builtin_ctz(band);
+            band &= band - 1u;
+            m |= 1u << k;
+            const unsigned before = m & ~excluded_mask(m, q0, i, ex, nb, bcap, bt) & ((1u << k) - 1u);
+            pend = ((cnt + __popc(before)) << 12) | (ebase + k);
+        }
         for (int k = 0; k < kBuildChunk; k++)
             if ((band >> k) & 1u)
                 m |= (dist2_ref(pi, gpos[q0 + k], L, invL) < rcs2) ? (1u << k) : 0u;
@@ -1078,7 +1101,7 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
             }
             __syncthreads();
         }
-        int cnt = 0, qself = 0;
+        int cnt = 0, qself = 0, pend = -1;
         if (valid && staged) {
             {  // the atom's own staged index (its own cell is window cell win_of(13))
                 const int us = s_wmap[win_of(13)];
@@ -1110,11 +1133,23 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
                     if (d.idx16)
                         build_chunk_fix<true>(jxy, jz, cxy, cz16, clo, chi, gpos, ce.x + q0, ce.y - q0, i, pi, L,
                                               invL, rcs2, ex, nb, d.bcap, bt, out, out16, kcap, cnt, ce.z + q0,
-                                              s_used);
+                                              s_used, pend);
                     else
                         build_chunk_fix<false>(jxy, jz, cxy, cz16, clo, chi, gpos, ce.x + q0, ce.y - q0, i, pi, L,
                                                invL, rcs2, ex, nb, d.bcap, bt, out, out16, kcap, cnt, ce.z + q0,
-                                               s_used);
+                                               s_used, pend);
+                }
+            }
+            // the deferred candidate: one load for the wave; outside -> the atom's
+            // own entry (its staged index), which the force pass and the pair
+            // readout skip
+            if (pend >= 0) {
+                const int pos = pend >> 12, q = pend & 0xFFF;
+                const int4 ce = s_cell[s_seg[q]];
+                const bool in = dist2_ref(pi, gpos[ce.x + q - ce.z], L, invL) < rcs2;
+                if (!in && pos < kcap) {
+                    if (d.idx16) out16[pos * kTile] = (unsigned short)qself;
+                    else out[pos * kTile] = qself;
                 }
             }
             if (qself < 0 || qself >= nstage) {
