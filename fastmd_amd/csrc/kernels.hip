@@ -774,18 +774,19 @@ __device__ __forceinline__ unsigned coarse_z(int4 p) { return (unsigned)p.z >> 1
 // A candidate the prefilter leaves undecided needs the reference distance on
 // its float position, a global load.  The first such candidate of a lane is
 // appended as inside and recorded in pend (row position << 12 | staged index);
-// after the scan resolve_pending decides it, with one load round trip for the
-// whole wave, and turns it into the atom's own entry (which every consumer
-// skips) if it lies outside.  Further undecided candidates of the lane are
-// decided at once.
-template <bool W16>
-__device__ __forceinline__ void build_chunk_fix(const unsigned (&jxy)[kBuildChunk], const unsigned (&jz)[kBuildChunk],
-                                                unsigned pxy, unsigned pz, int clo, int chi,
-                                                const FMD_GLOBAL float4* gpos, int q0, int n, int i, float4 pi,
-                                                float L, float invL, float rcs2, const int (&ex)[4], int nb,
-                                                int bcap, const FMD_GLOBAL int* bt, FMD_GLOBAL int* out,
-                                                FMD_GLOBAL unsigned short* out16, int kcap, int& cnt, int ebase,
-                                                unsigned char* used, int& pend) {
+// after the scan the lane decides it, with one load round trip for the whole
+// wave, and turns it into the atom's own entry (which every consumer skips) if
+// it lies outside.  Further undecided candidates of the lane are decided at
+// once.  Candidates are staged indices q0 + k (k < n); xq lists the staged
+// indices to leave out (the atom itself and its bonded partners, -1 = none)
+// and slot_of maps a staged index to its slot.
+template <bool W16, class SlotOf>
+__device__ __forceinline__ void build_chunk_q(const unsigned (&jxy)[kBuildChunk], const unsigned (&jz)[kBuildChunk],
+                                              unsigned pxy, unsigned pz, int clo, int chi,
+                                              const FMD_GLOBAL float4* gpos, int q0, int n, float4 pi, float L,
+                                              float invL, float rcs2, const int (&xq)[5], unsigned xmore,
+                                              FMD_GLOBAL int* out, FMD_GLOBAL unsigned short* out16, int kcap,
+                                              int& cnt, unsigned char* used, int& pend, SlotOf slot_of) {
     // the bits come from the sign of q - bound (q >= 0, bounds in [0, 2^31 - 1):
     // no overflow), plain integer VALU work: no compare-to-mask / select pairs
     // and their wait states
@@ -798,22 +799,37 @@ __device__ __forceinline__ void build_chunk_fix(const unsigned (&jxy)[kBuildChun
         band |= ((unsigned)(q - chi1) >> 31) << k;  // q <= chi
     }
     const unsigned valid = n >= kBuildChunk ? (1u << kBuildChunk) - 1u : (1u << n) - 1u;  // k < n
-    m &= valid;
-    band &= valid & ~m;
+    unsigned x = xmore;
+#pragma unroll
+    for (int b = 0; b < 5; b++) {
+        const unsigned o = (unsigned)(xq[b] - q0);
+        x |= (o < (unsigned)kBuildChunk) ? (1u << o) : 0u;
+    }
+    m &= valid & ~x;
+    band &= valid & ~m & ~x;
     if (band) {
-        band &= ~excluded_mask(band, q0, i, ex, nb, bcap, bt);
-        if (band && pend < 0) {  // defer the first: inside for now, position recorded
+        if (pend < 0) {  // defer the first: inside for now, its row position recorded
             const int k = __builtin_ctz(band);
             band &= band - 1u;
+            pend = ((cnt + __popc(m & ((1u << k) - 1u))) << 12) | (q0 + k);
             m |= 1u << k;
-            const unsigned before = m & ~excluded_mask(m, q0, i, ex, nb, bcap, bt) & ((1u << k) - 1u);
-            pend = ((cnt + __popc(before)) << 12) | (ebase + k);
         }
         for (int k = 0; k < kBuildChunk; k++)
             if ((band >> k) & 1u)
-                m |= (dist2_ref(pi, gpos[q0 + k], L, invL) < rcs2) ? (1u << k) : 0u;
+                m |= (dist2_ref(pi, gpos[slot_of(q0 + k)], L, invL) < rcs2) ? (1u << k) : 0u;
     }
-    append_chunk<W16>(m, q0, i, ex, nb, bcap, bt, out, out16, kcap, cnt, ebase, used);
+    // one pass per set bit, lowest first: a wave runs as many passes as its
+    // fullest lane has pairs in the chunk (about 4 of 8)
+    while (m) {
+        const int k = __builtin_ctz(m);
+        m &= m - 1u;
+        if (cnt < kcap) {
+            if (W16) out16[cnt * kTile] = (unsigned short)(q0 + k);
+            else out[cnt * kTile] = q0 + k;
+            used[q0 + k] = 1;
+        }
+        cnt++;
+    }
 }
 
 // Block-wide exclusive prefix of one int per thread (kGroupSlots threads).
@@ -1107,37 +1123,55 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
                 const int us = s_wmap[win_of(13)];
                 if (us < nu) qself = s_cell[us].z + (i - s_cell[us].x);
             }
+            // exclusions as staged indices: the atom itself and its bonded
+            // partners (a partner outside the 27 cells is no candidate)
+            int xq[5] = {qself, -1, -1, -1, -1};
+#pragma unroll
+            for (int b = 0; b < 4; b++) xq[b + 1] = ex[b] >= 0 ? partner_q(ex[b]) : -1;
+            auto slot_of = [&](int q) __attribute__((always_inline)) {
+                const int4 ce = s_cell[s_seg[q]];
+                return ce.x + q - ce.z;
+            };
+            auto scan_range = [&](int z, int e) __attribute__((always_inline)) {
+                for (int q0 = z; q0 < e; q0 += kBuildChunk) {
+                    // a whole chunk from LDS (entries past the range are masked
+                    // out, and the arrays are padded by a chunk)
+                    unsigned jxy[kBuildChunk], jz[kBuildChunk];
+#pragma unroll
+                    for (int k = 0; k < kBuildChunk; k++) {
+                        jxy[k] = s_pxy[q0 + k];
+                        jz[k] = s_pz[q0 + k];
+                    }
+                    unsigned xmore = 0u;  // partners past the first four (not in Kremer-Grest chains)
+                    for (int b = 4; b < nb && b < d.bcap; b++) {
+                        const unsigned o = (unsigned)(partner_q(bt[b * kTile] & kSlotMask) - q0);
+                        xmore |= (o < (unsigned)kBuildChunk) ? (1u << o) : 0u;
+                    }
+                    if (d.idx16)
+                        build_chunk_q<true>(jxy, jz, cxy, cz16, clo, chi, gpos, q0, e - q0, pi, L, invL, rcs2, xq,
+                                            xmore, out, out16, kcap, cnt, s_used, pend, slot_of);
+                    else
+                        build_chunk_q<false>(jxy, jz, cxy, cz16, clo, chi, gpos, q0, e - q0, pi, L, invL, rcs2, xq,
+                                             xmore, out, out16, kcap, cnt, s_used, pend, slot_of);
+                }
+            };
+            // the 27 cells in (dz, dy, dx) order; the three cells of a (dz, dy)
+            // row are consecutive union cells (hence one contiguous staged
+            // range) unless the window wraps that axis
             #pragma unroll 1
-            for (int c = 0; c < 27; c++) {
-                const int w = win_of(c);
-                bool seen = false;
-                if (nx < 3)  // small boxes visit a cell once (verlet_list.cu:154-187)
-                    for (int v = 0; v < c; v++) seen |= (win_of(v) == w);
-                if (seen) continue;
-                const int u = s_wmap[w];
-                if (u >= nu) {  // every neighbor cell was marked; guard the LDS lookups regardless
+            for (int r = 0; r < 9; r++) {
+                const int ua = s_wmap[win_of(3 * r)], ub = s_wmap[win_of(3 * r + 1)], uc = s_wmap[win_of(3 * r + 2)];
+                if (ua >= nu || ub >= nu || uc >= nu) {  // every neighbor cell was marked; guard regardless
                     err |= ERR_INTERNAL;
                     continue;
                 }
-                const int4 ce = s_cell[u];
-                for (int q0 = 0; q0 < ce.y; q0 += kBuildChunk) {
-                    // a whole chunk from LDS (immediate offsets; entries past the
-                    // cell are masked out, and the arrays are padded by a chunk)
-                    unsigned jxy[kBuildChunk], jz[kBuildChunk];
-                    const unsigned* pxy = s_pxy + ce.z + q0;
-                    const unsigned* pz = s_pz + ce.z + q0;
-#pragma unroll
-                    for (int k = 0; k < kBuildChunk; k++) {
-                        jxy[k] = pxy[k]; jz[k] = pz[k];
-                    }
-                    if (d.idx16)
-                        build_chunk_fix<true>(jxy, jz, cxy, cz16, clo, chi, gpos, ce.x + q0, ce.y - q0, i, pi, L,
-                                              invL, rcs2, ex, nb, d.bcap, bt, out, out16, kcap, cnt, ce.z + q0,
-                                              s_used, pend);
-                    else
-                        build_chunk_fix<false>(jxy, jz, cxy, cz16, clo, chi, gpos, ce.x + q0, ce.y - q0, i, pi, L,
-                                               invL, rcs2, ex, nb, d.bcap, bt, out, out16, kcap, cnt, ce.z + q0,
-                                               s_used, pend);
+                const int4 A = s_cell[ua], B = s_cell[ub], Cc = s_cell[uc];
+                const bool one = ub == ua + 1 && uc == ub + 1;
+                #pragma unroll 1
+                for (int t = 0; t < (one ? 1 : 3); t++) {
+                    const int z = t == 0 ? A.z : (t == 1 ? B.z : Cc.z);
+                    const int e = one ? Cc.z + Cc.y : (t == 0 ? A.z + A.y : (t == 1 ? B.z + B.y : Cc.z + Cc.y));
+                    scan_range(z, e);
                 }
             }
             // the deferred candidate: one load for the wave; outside -> the atom's
@@ -1145,8 +1179,7 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
             // readout skip
             if (pend >= 0) {
                 const int pos = pend >> 12, q = pend & 0xFFF;
-                const int4 ce = s_cell[s_seg[q]];
-                const bool in = dist2_ref(pi, gpos[ce.x + q - ce.z], L, invL) < rcs2;
+                const bool in = dist2_ref(pi, gpos[slot_of(q)], L, invL) < rcs2;
                 if (!in && pos < kcap) {
                     if (d.idx16) out16[pos * kTile] = (unsigned short)qself;
                     else out[pos * kTile] = qself;
