@@ -75,15 +75,23 @@ def test_extended_state_matches_oracle(fmd, engine, orc, ensemble):
     th = engine.thermostat(sid)
     engine.remove(sid)
     *_, xr = orc.md_run(_oracle_sys(p4, v, L, b, a, ensemble, 0.002), 40, ext=True)
+    # how far 40 steps carry a start error of the size of the device's FP32
+    # force error: the oracle run again from positions moved by ~1e-6 sigma
+    # (two FP32 roundings at this box size); the device may differ from the
+    # oracle by a few times that spread, or by 2e-5 relative, whichever is larger
+    pp = p4.copy()
+    pp[:, :3] += F(1e-6) * np.random.default_rng(4).standard_normal((len(p4), 3)).astype(F)
+    *_, xp = orc.md_run(_oracle_sys(pp, v, L, b, a, ensemble, 0.002), 40, ext=True)
     assert th0["Q1"] == xr[0, 4] and th0["Qr"] == xr[0, 5] and th0["W"] == xr[0, 6]
     assert th["step"] == 40 and th["M"] == 3
     x = xr[40]
+    tol = lambda c, lo: max(2e-5 * max(lo, abs(x[c])), 4.0 * abs(xp[40, c] - x[c]))
     for k in range(3):
-        assert abs(th["vxi"][k] - x[20 + k]) <= 2e-5 * max(1.0, abs(x[20 + k])), (k, th["vxi"][k], x[20 + k])
-        assert abs(th["xi"][k] - x[10 + k]) <= 2e-5 * max(0.1, abs(x[10 + k])), (k, th["xi"][k], x[10 + k])
+        assert abs(th["vxi"][k] - x[20 + k]) <= tol(20 + k, 1.0), (k, th["vxi"][k], x[20 + k], xp[40, 20 + k])
+        assert abs(th["xi"][k] - x[10 + k]) <= tol(10 + k, 0.1), (k, th["xi"][k], x[10 + k], xp[40, 10 + k])
     if ensemble == 2:
-        assert abs(th["eps"] - x[7]) <= 1e-6 + 2e-5 * abs(x[7])
-        assert abs(th["veps"] - x[8]) <= 2e-5 * max(1.0, abs(x[8]))
+        assert abs(th["eps"] - x[7]) <= 1e-6 + tol(7, 0.0)
+        assert abs(th["veps"] - x[8]) <= tol(8, 1.0)
         assert np.max(np.abs(x[7])) > 0  # the barostat moved
 
 
