@@ -132,6 +132,7 @@ SIGNATURES = [
     ("fmd_get_box", C.c_int32, [C.c_void_p, C.c_int32, _dp]),
     ("fmd_get_state_box", C.c_int32, [C.c_void_p, C.c_int32, _dp]),
     ("fmd_get_rg", C.c_int32, [C.c_void_p, C.c_int32, _dp, _dp, C.c_int32, _ip]),
+    ("fmd_get_rg_noalloc", C.c_int32, [C.c_void_p, C.c_int32, _dp, _dp, C.c_int32, _ip]),
     ("fmd_snapshot_size", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64)]),
     ("fmd_snapshot", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_uint64]),
     ("fmd_system_resume", C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64, _ip]),

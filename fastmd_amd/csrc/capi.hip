@@ -171,6 +171,10 @@ int32_t fmd_get_corr(fmd_engine* e, int32_t sys, double* corr, int32_t* counts, 
 int32_t fmd_get_rg(fmd_engine* e, int32_t sys, double* rg, double* per_chain, int32_t cap, int32_t* nchains) {
     return e ? e->e->get_rg(sys, rg, per_chain, cap, nchains) : FMD_ERR_ARG;
 }
+int32_t fmd_get_rg_noalloc(fmd_engine* e, int32_t sys, double* rg, double* per_chain, int32_t cap,
+                           int32_t* nchains) {
+    return e ? e->e->get_rg_noalloc(sys, rg, per_chain, cap, nchains) : FMD_ERR_ARG;
+}
 int32_t fmd_get_state_box(fmd_engine* e, int32_t sys, double* L) {
     return e ? e->e->get_box(sys, L, true) : FMD_ERR_ARG;
 }

@@ -14,6 +14,7 @@
 namespace fmd {
 
 constexpr int kNumClasses = 3;
+constexpr int kRgScratchChains = 1 << 16;  // chains the allocation-free Rg read holds
 constexpr int kGraphSteps = 20;
 
 int size_class(int natoms);
@@ -117,6 +118,9 @@ class Engine {
     int32_t get_thermostat(int32_t sys, fmd_thermostat* out);
     int32_t nh_chain_run(const fmd_chain_run* c, double* xi, double* vxi, double* scales);
     int32_t get_rg(int32_t sys, double* rg, double* per_chain, int32_t cap, int32_t* nchains);
+    // The same with no device allocation: the per-chain values and their mean
+    // go through the device's file-scope scratch array (at most kRgScratchChains).
+    int32_t get_rg_noalloc(int32_t sys, double* rg, double* per_chain, int32_t cap, int32_t* nchains);
     int32_t snapshot_size(int32_t sys, uint64_t* bytes);
     int32_t snapshot(int32_t sys, void* buf, uint64_t cap);
     int32_t resume(const void* buf, uint64_t size, int32_t* sys);
@@ -158,6 +162,7 @@ class Engine {
     std::string err_;
     hipStream_t stream_ = nullptr;
     hipStream_t side_ = nullptr;  // system creation beside the main stream's launches
+    double* scratch_ = nullptr;   // this device's scratch array (device address), or null
     bool defer_free_ = false;
     uint64_t budget_ = 0, in_use_ = 0;
     std::vector<Replica*> reps_;

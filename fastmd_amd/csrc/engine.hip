@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 
 #include "kernels.h"
@@ -23,6 +24,16 @@ int size_class(int natoms) { return natoms < 8192 ? 0 : (natoms < 32768 ? 1 : 2)
 // slot count: whole groups of kGroupTiles tiles
 static inline int ceil_slots(int n) { return (n + kGroupSlots - 1) / kGroupSlots * kGroupSlots; }
 
+
+// Device scratch of the allocation-free Rg read: the per-chain values, then
+// their mean.  One copy per GPU, at a device address found once per device
+// (hipGetSymbolAddress with that device current); host code uses that address,
+// never the symbol.  Calls that overlap on one GPU share it and race.
+static_assert(kRgScratchChains == FMD_RG_NOALLOC_MAX_CHAINS, "the header states the scratch limit");
+static __device__ __align__(256) double scratch[kRgScratchChains + 32];
+static constexpr int kMaxScratchDevices = 64;
+static double* g_scratch_ptr[kMaxScratchDevices];
+static std::mutex g_scratch_mu;
 
 Engine::Engine(int device, uint64_t budget) : device_(device) {
     for (int c = 0; c < kNumClasses; c++) {
@@ -38,6 +49,17 @@ Engine::Engine(int device, uint64_t budget) : device_(device) {
     budget_ = budget ? budget : (uint64_t)(0.9 * (double)fr);
     if (hip_check(hipMalloc(&d_descs_, sizeof(RepDesc) * kMaxReplicas), "hipMalloc(descs)")) return;
     reps_.assign(kMaxReplicas, nullptr);
+    if (device >= 0 && device < kMaxScratchDevices) {
+        std::lock_guard<std::mutex> g(g_scratch_mu);
+        if (!g_scratch_ptr[device]) {
+            void* p = nullptr;
+            if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess && p)
+                g_scratch_ptr[device] = (double*)p;
+            else
+                (void)hipGetLastError();  // the allocation-free read reports it; the rest works
+        }
+        scratch_ = g_scratch_ptr[device];
+    }
     init_ok_ = true;
 }
 
@@ -1126,6 +1148,28 @@ int32_t Engine::get_rg(int32_t sys, double* rg, double* per_chain, int32_t cap, 
     if (per_chain)
         for (int c = 0; c < nc && c < cap; c++) per_chain[c] = h[c];
     return FMD_OK;
+}
+
+int32_t Engine::get_rg_noalloc(int32_t sys, double* rg, double* per_chain, int32_t cap, int32_t* nchains) {
+    Replica* r = rep(sys);
+    if (!r || !rg) return fail(FMD_ERR_ARG, "bad arguments");
+    const int nc = r->desc.nchains;
+    if (nchains) *nchains = nc;
+    *rg = 0.0;
+    if (nc <= 0) return FMD_OK;
+    if (!scratch_) return fail(FMD_ERR_HIP, "the device scratch array has no address on this device");
+    if (nc > kRgScratchChains)
+        return fail(FMD_ERR_ARG, "system has " + std::to_string(nc) + " chains; the allocation-free Rg read holds " +
+                                     std::to_string(kRgScratchChains));
+    launch_rg(d_descs_ + sys, nc, scratch_, stream_);
+    launch_rg_mean(d_descs_ + sys, scratch_, scratch_ + kRgScratchChains, stream_);
+    int32_t rc = hip_check(hipGetLastError(), "launch(rg)");
+    if (!rc) rc = hip_check(hipMemcpyAsync(rg, scratch_ + kRgScratchChains, 8, hipMemcpyDeviceToHost, stream_), "copy rg");
+    if (!rc && per_chain && cap > 0)
+        rc = hip_check(hipMemcpyAsync(per_chain, scratch_, (size_t)std::min(nc, cap) * 8, hipMemcpyDeviceToHost, stream_),
+                       "copy rg");
+    if (!rc) rc = sync();
+    return rc;
 }
 
 // ---------------------------------------------------------------------------

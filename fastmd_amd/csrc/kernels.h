@@ -39,6 +39,8 @@ void launch_boundary(const Launch& l, int mode);
 void launch_control(const Launch& l, int op, int64_t arg);
 // Radius of gyration of each of a replica's chains.
 void launch_rg(const RepDesc* d, int nchains, double* out, hipStream_t s);
+// Their mean (the replica's chain count of values in vals) into out[0].
+void launch_rg_mean(const RepDesc* d, const double* vals, double* out, hipStream_t s);
 // One stress sample (xx, xy, xz, yy, yz, zz), passed by value.
 struct Sample6 { double v[6]; };
 void launch_corr_push(const RepDesc* d, const Sample6& x, hipStream_t s);

@@ -426,6 +426,17 @@ __global__ void __launch_bounds__(64) k_rg(const RepDesc* __restrict__ d, double
     if (lane_id() == 0) out[c] = sqrt(r2 / len);
 }
 
+// Mean of the per-chain values k_rg wrote to vals, summed in chain order by
+// one lane (the order the host sum of the per-chain read uses).
+__global__ void __launch_bounds__(64) k_rg_mean(const RepDesc* __restrict__ d, const double* __restrict__ vals,
+                                                double* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    const int nc = d->nchains;
+    double s = 0.0;
+    for (int c = 0; c < nc; c++) s += vals[c];
+    out[0] = nc > 0 ? s / nc : 0.0;
+}
+
 // Status of each listed replica for one host read per batch.
 __global__ void k_status(const RepDesc* __restrict__ descs, const int* __restrict__ reps, int nr,
                          int4* __restrict__ out) {
@@ -2091,6 +2102,9 @@ void launch_decide(const Launch& l) {
 void launch_rg(const RepDesc* d, int nchains, double* out, hipStream_t s) {
     if (nchains <= 0) return;
     k_rg<<<nchains, 64, 0, s>>>(d, out);
+}
+void launch_rg_mean(const RepDesc* d, const double* vals, double* out, hipStream_t s) {
+    k_rg_mean<<<1, 64, 0, s>>>(d, vals, out);
 }
 void launch_corr_push(const RepDesc* d, const Sample6& x, hipStream_t s) {
     k_corr_push<<<1, 128, 0, s>>>(d, x);

@@ -239,6 +239,17 @@ int32_t fmd_get_state_box(fmd_engine* e, int32_t sys, double* L);
  * chain values; nchains (if non-NULL) the chain count. */
 int32_t fmd_get_rg(fmd_engine* e, int32_t sys, double* rg, double* per_chain, int32_t cap,
                    int32_t* nchains);
+/* fmd_get_rg with no device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The per-chain values and
+ * their mean (summed on the device in chain order: the same bits as
+ * fmd_get_rg) pass through one file-scope device scratch array per GPU of at
+ * most FMD_RG_NOALLOC_MAX_CHAINS chains (512 KiB, held until the process
+ * ends); a system with more chains gets FMD_ERR_ARG.  Calls that overlap on
+ * one GPU share that array and race. */
+#define FMD_RG_NOALLOC_MAX_CHAINS 65536
+int32_t fmd_get_rg_noalloc(fmd_engine* e, int32_t sys, double* rg, double* per_chain, int32_t cap,
+                           int32_t* nchains);
 /* Exact checkpoint of a system's whole device state: positions, velocities
  * and images in the current sorted order, the Verlet list and its reference
  * positions, thermostat / barostat state, step and rebuild counters, the
