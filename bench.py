@@ -262,6 +262,17 @@ def served_c5(eng, log, rate_batched, tuned, window_s=600.0, warm_s=120.0, pool_
         time.sleep(0.2)
     marks["end"] = (time.perf_counter() - t_start, srv.stats()["bead_steps"])
     st_end = srv.stats()
+    # the jobs still resident or queued run to completion (their latencies
+    # count); progress is logged while they do
+    while True:
+        st = srv.stats()
+        if st["resident"] + st["queued"] == 0:
+            break
+        el = time.perf_counter() - t_start
+        if el >= next_log:
+            log(f"drain t={el:.0f}s resident {st['resident']} queued {st['queued']} bead-steps {st['bead_steps']:.4g}")
+            next_log += log_every
+        time.sleep(0.5)
     srv.drain()
     t_all = time.perf_counter() - t_start
     lat, failed, bs = [], 0, 0.0

@@ -335,6 +335,10 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult, b
     D.fcap = d->force_cap > 0.0f ? d->force_cap : 0.0f;
     D.vlim = D.fcap > 0.0f ? 0.02f / d->dt : 0.0f;
     D.clamp_lj = 1;
+    if (d->ntypes == 1 && d->lj[0] != 0.0f && d->lj[1] != 0.0f) {  // the force pass's one-type form
+        D.lj_kf = 24.0f * d->lj[0] / (d->lj[1] * d->lj[1]);
+        D.lj_fclamp = 1000.0f / D.lj_kf;
+    }
     D.idx16 = (S.nslots <= 65536) && std::getenv("FMD_IDX32") == nullptr ? 1 : 0;
     D.nostage = tuned_[2][size_class(S.natoms)] ? 0 : 1;
     D.seed_lo = (uint32_t)d->seed;
@@ -542,6 +546,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult, b
     S0.invL = 1.0f / d->L;
     S0.Lp = d->L;
     S0.invLp = 1.0f / d->L;
+    set_grid_consts(&S0, D.rc2);
     {
         int n = (int)(d->L / D.rcs);
         int nx = n < 1 ? 1 : n;

@@ -26,7 +26,7 @@ constexpr int kGroupTiles = 4;
 constexpr int kGroupSlots = kGroupTiles * kTile;
 constexpr int kMaxUnion = 256;
 constexpr int kBuildStage = 3072;
-constexpr int kGroupStage = 2048;
+constexpr int kGroupStage = 1696;
 constexpr int kMaxTypes = 16;
 constexpr int kMaxChain = 10;
 constexpr int kNumPart = 16;   // FP64 partials per tile: pe, W[6], K[6], |v| and |F| max keys, pad
@@ -101,6 +101,8 @@ struct RepState {
     // current box and cell grid (float copies for per-atom kernels)
     float L, invL;      // box of the next position update (the barostat advances it after the forces)
     float Lp, invLp;    // box the current positions were placed in
+    float glo, giu2;    // LJ grid-unit constants of Lp (set_grid_consts)
+    unsigned gspan;
     int nx;                // cells per axis
     int cbits;             // Morton bits per axis of the cell key
     float inv_cs;
@@ -152,6 +154,7 @@ struct RepDesc {
     float fcap;            // LJ pair force cap (0 = off)
     float vlim;            // push-off speed limit (0 = off): |v| dt <= 0.02 sigma
     int clamp_lj;          // force/r clamp to +-1000 (src/force/lj.cu:48)
+    float lj_kf, lj_fclamp;  // one (eps, sigma): 24 eps / sigma^2 and 1000 / that (0 otherwise)
     int idx16;             // neighbor rows hold 16-bit slots (nslots <= 65536)
     int nostage;           // the list build leaves rows as slots (the force kernel gathers
                            // neighbors from HBM) instead of offsets into a staged neighbor set
@@ -253,18 +256,39 @@ __device__ inline float fix_diff(unsigned a, unsigned b, float ulen) {
 }
 
 // LJ cutoff test in grid units.  a = (float)(int)(Xi - Xj) per axis (the
-// fixed-point displacement in grid steps u = L * 2^-32, rounded once),
-// q = fma(az, az, fma(ay, ay, ax * ax)) = r^2 / u^2, and a pair is inside when
-// lo_g < q < rc2_g with rc2_g = (float)(rc^2 / u^2) and lo_g =
-// (float)(1e-10f / u^2), both quotients in FP64 (u^2 is exact in FP64).  The
-// lower bound is the r^2 > 1e-10 test of src/force/lj.cu:37.
-struct GridCut { float rc2g, lo_g; };
-__device__ inline GridCut grid_cut(float rc2, float u) {
+// fixed-point displacement in grid steps u = L * 2^-32, rounded once) and
+// s = fma(az, az, fma(ay, ay, fma(ax, ax, -lo_g))), which is
+// (r^2 - 1e-10) / u^2 up to rounding.  A pair is inside when
+// 0 < s < span, with lo_g = (float)(1e-10f / u^2) and
+// span = (float)((rc^2 - 1e-10f) / u^2), both quotients in FP64 (u^2 is exact
+// in FP64): the r^2 < rc^2 && r^2 > 1e-10 test of src/force/lj.cu:37 up to
+// one FP32 rounding of each side.  s stands in for r^2 / u^2 in the force
+// terms (the two differ by a relative 1e-10 / r^2).  As unsigned integers the
+// bit patterns of non-negative floats order as the floats and a negative s
+// (sign bit set) lies above every bound, so 0 < s < span is one subtract and
+// one unsigned compare, and 0 <= s < span a single compare.
+struct GridCut { float lo_g; unsigned span; };
+// The grid-unit constants of a box of length Lp, kept in the replica's state
+// and set wherever Lp changes (creation, each step's rebuild decision), so
+// the force pass reads them instead of dividing: lo_g, span (as its bit
+// pattern) and 1/u^2, quotients in FP64 (the same bits on host and device).
+__host__ __device__ inline void set_grid_consts(RepState* st, float rc2) {
+    const float u = st->Lp * 2.3283064365386963e-10f;  // L * 2^-32, exact
     const double u2 = (double)u * (double)u;
-    return GridCut{(float)((double)rc2 / u2), (float)((double)1e-10f / u2)};
+    const float span = (float)(((double)rc2 - (double)1e-10f) / u2);
+    st->glo = (float)((double)1e-10f / u2);
+    __builtin_memcpy(&st->gspan, &span, sizeof(float));
+    st->giu2 = (float)(1.0 / u2);
 }
+__device__ inline GridCut grid_cut(const RepState* st) { return GridCut{st->glo, st->gspan}; }
 __device__ inline float grid_d(unsigned a, unsigned b) { return (float)(int)(a - b); }
-__device__ inline float grid_q(float x, float y, float z) { return fmaf(z, z, fmaf(y, y, x * x)); }
+__device__ inline float grid_s(float x, float y, float z, float lo_g) {
+    return fmaf(z, z, fmaf(y, y, fmaf(x, x, -lo_g)));
+}
+__device__ inline bool grid_inside(float s, unsigned span) { return __float_as_uint(s) - 1u < span - 1u; }
+// 0 <= s < span: also takes s == 0 (r^2 = 1e-10 to rounding) as inside, for
+// the forces-only LJ loop, whose clamped force/r keeps that pair finite.
+__device__ inline bool grid_inside0(float s, unsigned span) { return __float_as_uint(s) < span; }
 
 // Morton interleave of 10 bits (src/core/morton.cu:4-10)
 __host__ __device__ inline uint32_t expand_bits(uint32_t v) {

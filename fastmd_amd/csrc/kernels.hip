@@ -381,6 +381,7 @@ __global__ void __launch_bounds__(1024) k_decide(const RepDesc* __restrict__ des
     // k_pre of this step placed the positions in the box st->L
     st->Lp = st->L;
     st->invLp = st->invL;
+    set_grid_consts(st, d.rc2);
     const float md = fmaxf(__int_as_float(st->maxdisp_bits), m);  // largest since the last rebuild
     const int since = st->since_rebuild + 1;
     const int rb = (md > d.half_skin2) || (since >= kRebuildEvery);
@@ -1369,8 +1370,9 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
 
 // LJ over this atom's Verlet rows (src/force/lj.cu:27-63), in grid units.
 // With a = (float)(int)(Xi - Xj) the fixed-point displacement (grid step
-// u = L * 2^-32), q = a.a is r^2 / u^2: the cutoff test compares q with
-// thresholds scaled by 1/u^2, 1/q = u^2 / r^2 feeds the LJ terms with
+// u = L * 2^-32), q = a.a - lo_g is (r^2 - 1e-10) / u^2: the cutoff test
+// compares q with thresholds scaled by 1/u^2 (grid_s, grid_inside),
+// 1/q = u^2 / r^2 (to a relative 1e-10 / r^2) feeds the LJ terms with
 // sigma^2 / u^2 and 24 eps / u^2 folded in, and the force and virial sums are
 // kept in grid units and scaled by u and u^2 once per atom.  The rows are
 // padded to whole chunks of kLJChunk with the atom itself (q = 0 is rejected),
@@ -1403,17 +1405,17 @@ __device__ __forceinline__ LJSum lj_loop(const RepDesc& d, int lt, int lane, int
     const __amdgpu_buffer_rsrc_t prs =
         __builtin_amdgcn_make_buffer_rsrc((void*)d.ipos, (short)sizeof(int4), d.nslots, kBufferWord3);
     const float u = L * 2.3283064365386963e-10f;
-    const GridCut gc = grid_cut(d.rc2, u);
-    const float iu2 = (float)(1.0 / ((double)u * (double)u));
+    const GridCut gc = grid_cut(d.st);
+    const float iu2 = d.st->giu2;
     const float2* ljrow = d.lj + pi.w * d.ntypes;
-    const float2 p0 = ljrow[0];
+    const float2 p0 = ONE_TYPE ? d.lj[0] : ljrow[0];
     const float s2g0 = p0.y * p0.y * iu2, c24g0 = 24.0f * p0.x * iu2, c2e0 = 2.0f * p0.x;
     const float capg = CAP ? d.fcap / u : 0.0f;  // fcap / r = (fcap / u) / sqrt(q)
     // one (eps, sigma): force / r = K sr2 sr6 (2 sr6 - 1) with K = 24 eps / sigma^2,
     // summed without K (applied once per atom); the +-1000 clamp becomes +-1000 / K
     constexpr bool fold = ONE_TYPE && !CAP;  // ONE_TYPE only with eps, sigma != 0
-    const float kf = fold ? 24.0f * p0.x / (p0.y * p0.y) : 1.0f;
-    const float fclamp = fold ? 1000.0f / kf : 1000.0f;
+    const float kf = fold ? d.lj_kf : 1.0f;
+    const float fclamp = fold ? d.lj_fclamp : 1000.0f;
     float gx = 0.f, gy = 0.f, gz = 0.f, pe = 0.f;
     float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
     unsigned nxt[kLJChunk];
@@ -1441,8 +1443,10 @@ __device__ __forceinline__ LJSum lj_loop(const RepDesc& d, int lt, int lane, int
         for (int k = 0; k < kLJChunk; k++) {
             const uint4 pj = pjs[k];
             const float ax = grid_d(pi.x, pj.x), ay = grid_d(pi.y, pj.y), az = grid_d(pi.z, pj.z);
-            const float q = grid_q(ax, ay, az);
-            const bool in = (q < gc.rc2g) && (q > gc.lo_g);
+            const float q = grid_s(ax, ay, az, gc.lo_g);
+            // forces only, one (eps, sigma): q == 0 may count as inside (its
+            // force/r is the clamp and finite), one compare fewer per pair
+            const bool in = (fold && !OBS) ? grid_inside0(q, gc.span) : grid_inside(q, gc.span);
             float s2g = s2g0, c24g = c24g0, c2e = c2e0;
             if (!ONE_TYPE) {
                 const float2 p = ljrow[pj.w];
@@ -1503,7 +1507,7 @@ __device__ __forceinline__ LJSum table_loop(const RepDesc& d, int lt, int lane, 
     const int* nl = d.nbr + (size_t)lt * d.kcap * kTile + lane;
     const unsigned short* nl16 = (const unsigned short*)(d.nbr + (size_t)lt * d.kcap * kTile) + lane;
     const float u = L * 2.3283064365386963e-10f;
-    const GridCut gc = grid_cut(d.rc2, u);
+    const GridCut gc = grid_cut(d.st);
     const int* trow = d.tab_idx + pi.w * d.ntypes;
     LJSum s{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int k = 0; k < nn; k++) {
@@ -1512,8 +1516,7 @@ __device__ __forceinline__ LJSum table_loop(const RepDesc& d, int lt, int lane, 
         const int tb = trow[pj.w];
         if (tb < 0) continue;
         const float ax = grid_d(pi.x, pj.x), ay = grid_d(pi.y, pj.y), az = grid_d(pi.z, pj.z);
-        const float q = grid_q(ax, ay, az);
-        if (!(q < gc.rc2g && q > gc.lo_g)) continue;
+        if (!grid_inside(grid_s(ax, ay, az, gc.lo_g), gc.span)) continue;
         const float dx = ax * u, dy = ay * u, dz = az * u;
         const float4 tp = d.tab_parm[tb];
         const int2 tn = d.tab_no[tb];
@@ -1562,7 +1565,8 @@ __device__ __forceinline__ float angle_atan2(float y, float x) {
 // FP64 partials per wave: pe, virial[6], kinetic tensor[6] (after the kick).
 // ---------------------------------------------------------------------------
 template <int TERMS>
-__global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict__ descs,
+__global__ void __launch_bounds__(kMaxBlockK) __attribute__((amdgpu_waves_per_eu(6, 8)))
+k_force(const RepDesc* __restrict__ descs,
                                                const Tile* __restrict__ tiles, int t0, int ntl,
                                                int kick, int mode) {
     constexpr int terms = TERMS;
@@ -1617,9 +1621,9 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
         const bool obs = (mode == FM_OBS) || (mode == FM_STEP && st->want_obs);
         if (terms & 1) {
             LJSum a;
-            if (sp) {
-                if (d.idx16) a = lj_variants<true, true>(d, t, lane, i, pi, L, obs, sp);
-                else a = lj_variants<false, true>(d, t, lane, i, pi, L, obs, sp);
+            if (sp) {  // the array itself: its LDS address is a constant of the kernel
+                if (d.idx16) a = lj_variants<true, true>(d, t, lane, i, pi, L, obs, s_pos);
+                else a = lj_variants<false, true>(d, t, lane, i, pi, L, obs, s_pos);
             } else {
                 if (d.idx16) a = lj_variants<true, false>(d, t, lane, i, pi, L, obs, sp);
                 else a = lj_variants<false, false>(d, t, lane, i, pi, L, obs, sp);
@@ -1643,7 +1647,7 @@ __global__ void __launch_bounds__(kMaxBlockK) k_force(const RepDesc* __restrict_
         const FMD_GLOBAL int* gs32 = gview(d.gstage) + (size_t)grp * kGroupStage;
         auto partner = [&](int e) __attribute__((always_inline)) -> int4 {
             const int v = e & kSlotMask;
-            if (sp) return *(const int4*)((const char*)sp + v);
+            if (sp) return *(const int4*)((const char*)s_pos + v);
             if (bstaged) return d.ipos[d.idx16 ? (int)gs16[v >> 4] : gs32[v >> 4]];
             return d.ipos[v];
         };

@@ -68,20 +68,21 @@ inline float min_image_fixed_f(float a, float b, float L) {
     return fix_diff(fix_coord(a, sc), fix_coord(b, sc), L);
 }
 // The device's LJ cutoff test in grid units (grid step u = L * 2^-32):
-// a = (float)(int32)(Xi - Xj) per axis, q = fma(az, az, fma(ay, ay, ax * ax)),
-// inside when lo_g < q < rc2_g, rc2_g = (float)(rc2 / u^2), lo_g =
-// (float)(1e-10f / u^2), quotients in FP64.  Mathematically r^2 = q u^2, so
-// this is the r^2 < rc^2 && r^2 > 1e-10 test of src/force/lj.cu:37 up to one
-// FP32 rounding of each side.
+// a = (float)(int32)(Xi - Xj) per axis, s = fma(az, az, fma(ay, ay,
+// fma(ax, ax, -lo_g))), inside when 0 < s < span, lo_g = (float)(1e-10f / u^2),
+// span = (float)((rc2 - 1e-10f) / u^2), quotients in FP64.  Mathematically
+// s u^2 = r^2 - 1e-10, so this is the r^2 < rc^2 && r^2 > 1e-10 test of
+// src/force/lj.cu:37 up to one FP32 rounding of each side.
 inline bool lj_inside_grid(const float* pi, const float* pj, float rc2, float L) {
     const double sc = 4294967296.0 / (double)L;
     const float u = L * 2.3283064365386963e-10f;
     const double u2 = (double)u * (double)u;
-    const float rc2g = (float)((double)rc2 / u2), lo_g = (float)((double)1e-10f / u2);
+    const float lo_g = (float)((double)1e-10f / u2);
+    const float span = (float)(((double)rc2 - (double)1e-10f) / u2);
     float a[3];
     for (int k = 0; k < 3; k++) a[k] = (float)(int32_t)(fix_coord(pi[k], sc) - fix_coord(pj[k], sc));
-    const float q = std::fma(a[2], a[2], std::fma(a[1], a[1], a[0] * a[0]));
-    return q < rc2g && q > lo_g;
+    const float s = std::fma(a[2], a[2], std::fma(a[1], a[1], std::fma(a[0], a[0], -lo_g)));
+    return s > 0.0f && s < span;
 }
 // Same displacement carried in FP64 (the image shift m chosen as above).
 inline double min_image_exact_d(float a, float b, float L, float invL) {
