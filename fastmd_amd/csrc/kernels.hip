@@ -789,38 +789,39 @@ __device__ __forceinline__ unsigned coarse_z(int4 p) { return (unsigned)p.z >> 1
 // after the scan the lane decides it, with one load round trip for the whole
 // wave, and turns it into the atom's own entry (which every consumer skips) if
 // it lies outside.  Further undecided candidates of the lane are decided at
-// once.  Candidates are staged indices q0 + k (k < n); xq lists the staged
-// indices to leave out (the atom itself and its bonded partners, -1 = none)
-// and slot_of maps a staged index to its slot.
+// once.  Candidates are staged indices q0 + k (k < n); x holds the bits of
+// the candidates to leave out (the atom itself and its bonded partners),
+// slot_of maps a staged index to its slot, and used[q] marks atom q as in a
+// row.
 template <bool W16, class SlotOf>
 __device__ __forceinline__ void build_chunk_q(const unsigned (&jxy)[kBuildChunk], const unsigned (&jz)[kBuildChunk],
                                               unsigned pxy, unsigned pz, int clo, int chi,
                                               const FMD_GLOBAL float4* gpos, int q0, int n, float4 pi, float L,
-                                              float invL, float rcs2, const int (&xq)[5], unsigned xmore,
-                                              FMD_GLOBAL int* out, FMD_GLOBAL unsigned short* out16, int kcap,
-                                              int& cnt, unsigned char* used, int& pend, SlotOf slot_of) {
-    // the bits come from the sign of q - bound (q >= 0, bounds in [0, 2^31 - 1):
-    // no overflow), plain integer VALU work: no compare-to-mask / select pairs
-    // and their wait states
-    unsigned m = 0, band = 0;
-    const int chi1 = chi + 1;
+                                              float invL, float rcs2, unsigned x, FMD_GLOBAL int* out,
+                                              FMD_GLOBAL unsigned short* out16, int kcap, int& cnt, unsigned char* used,
+                                              int& pend, SlotOf slot_of) {
+    // t = q - clo (q >= 0 and clo in [0, 2^31 - 1): no overflow) is negative
+    // for a candidate certainly inside.  Its sign bit is shifted into m from
+    // the last candidate down (alignbit of (m, t) by 31 is m << 1 | t >> 31),
+    // so bit k of m is candidate k; the smallest t as an unsigned number is
+    // the nearest candidate not certainly inside, and only when it lies in the
+    // band [0, chi - clo] are the band's candidates found one by one.
+    unsigned m = 0u, umin = ~0u;
+    int t[kBuildChunk];
 #pragma unroll
-    for (int k = 0; k < kBuildChunk; k++) {
-        const int q = coarse_q(pxy, pz, jxy[k], jz[k]);
-        m |= ((unsigned)(q - clo) >> 31) << k;      // q < clo
-        band |= ((unsigned)(q - chi1) >> 31) << k;  // q <= chi
+    for (int k = kBuildChunk - 1; k >= 0; k--) {
+        t[k] = coarse_q(pxy, pz, jxy[k], jz[k]) - clo;
+        m = __builtin_amdgcn_alignbit(m, (unsigned)t[k], 31);
+        umin = min(umin, (unsigned)t[k]);
     }
     const unsigned valid = n >= kBuildChunk ? (1u << kBuildChunk) - 1u : (1u << n) - 1u;  // k < n
-    unsigned x = xmore;
-#pragma unroll
-    for (int b = 0; b < 5; b++) {
-        const unsigned o = (unsigned)(xq[b] - q0);
-        x |= (o < (unsigned)kBuildChunk) ? (1u << o) : 0u;
-    }
     m &= valid & ~x;
-    band &= valid & ~m & ~x;
-    if (band) {
-        if (pend < 0) {  // defer the first: inside for now, its row position recorded
+    if (umin <= (unsigned)(chi - clo)) {
+        unsigned band = 0u;
+#pragma unroll
+        for (int k = 0; k < kBuildChunk; k++) band |= ((unsigned)t[k] <= (unsigned)(chi - clo)) ? (1u << k) : 0u;
+        band &= valid & ~x;
+        if (band && pend < 0) {  // defer the first: inside for now, its row position recorded
             const int k = __builtin_ctz(band);
             band &= band - 1u;
             pend = ((cnt + __popc(m & ((1u << k) - 1u))) << 12) | (q0 + k);
@@ -831,7 +832,7 @@ __device__ __forceinline__ void build_chunk_q(const unsigned (&jxy)[kBuildChunk]
                 m |= (dist2_ref(pi, gpos[slot_of(q0 + k)], L, invL) < rcs2) ? (1u << k) : 0u;
     }
     // one pass per set bit, lowest first: a wave runs as many passes as its
-    // fullest lane has pairs in the chunk (about 4 of 8)
+    // fullest lane has pairs in the chunk
     while (m) {
         const int k = __builtin_ctz(m);
         m &= m - 1u;
@@ -1136,15 +1137,28 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
                 if (us < nu) qself = s_cell[us].z + (i - s_cell[us].x);
             }
             // exclusions as staged indices: the atom itself and its bonded
-            // partners (a partner outside the 27 cells is no candidate)
-            int xq[5] = {qself, -1, -1, -1, -1};
+            // partners (a partner outside the 27 cells is no candidate), sorted
+            // ascending (none = INT_MAX); each range consumes them in order
+            int xs[5] = {qself, INT_MAX, INT_MAX, INT_MAX, INT_MAX};
 #pragma unroll
-            for (int b = 0; b < 4; b++) xq[b + 1] = ex[b] >= 0 ? partner_q(ex[b]) : -1;
+            for (int b = 0; b < 4; b++) {
+                const int q = ex[b] >= 0 ? partner_q(ex[b]) : -1;
+                xs[b + 1] = q >= 0 ? q : INT_MAX;
+            }
+            auto cswap = [](int& a, int& b) { const int lo = min(a, b), hi = max(a, b); a = lo; b = hi; };
+            cswap(xs[0], xs[1]); cswap(xs[3], xs[4]); cswap(xs[2], xs[4]); cswap(xs[2], xs[3]); cswap(xs[1], xs[4]);
+            cswap(xs[0], xs[3]); cswap(xs[0], xs[2]); cswap(xs[1], xs[3]); cswap(xs[1], xs[2]);
             auto slot_of = [&](int q) __attribute__((always_inline)) {
                 const int4 ce = s_cell[s_seg[q]];
                 return ce.x + q - ce.z;
             };
             auto scan_range = [&](int z, int e) __attribute__((always_inline)) {
+                // the first exclusion at or past z, and its successors
+                int xp = (xs[0] < z) + (xs[1] < z) + (xs[2] < z) + (xs[3] < z) + (xs[4] < z);
+                auto xat = [&](int p) {
+                    return p == 0 ? xs[0] : (p == 1 ? xs[1] : (p == 2 ? xs[2] : (p == 3 ? xs[3] : (p == 4 ? xs[4] : INT_MAX))));
+                };
+                int xn = xat(xp);
                 for (int q0 = z; q0 < e; q0 += kBuildChunk) {
                     // a whole chunk from LDS (entries past the range are masked
                     // out, and the arrays are padded by a chunk)
@@ -1154,17 +1168,21 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
                         jxy[k] = s_pxy[q0 + k];
                         jz[k] = s_pz[q0 + k];
                     }
-                    unsigned xmore = 0u;  // partners past the first four (not in Kremer-Grest chains)
-                    for (int b = 4; b < nb && b < d.bcap; b++) {
+                    unsigned x = 0u;
+                    while ((unsigned)(xn - q0) < (unsigned)kBuildChunk) {
+                        x |= 1u << (xn - q0);
+                        xn = xat(++xp);
+                    }
+                    for (int b = 4; b < nb && b < d.bcap; b++) {  // partners past the first four (not in KG chains)
                         const unsigned o = (unsigned)(partner_q(bt[b * kTile] & kSlotMask) - q0);
-                        xmore |= (o < (unsigned)kBuildChunk) ? (1u << o) : 0u;
+                        x |= (o < (unsigned)kBuildChunk) ? (1u << o) : 0u;
                     }
                     if (d.idx16)
-                        build_chunk_q<true>(jxy, jz, cxy, cz16, clo, chi, gpos, q0, e - q0, pi, L, invL, rcs2, xq,
-                                            xmore, out, out16, kcap, cnt, s_used, pend, slot_of);
+                        build_chunk_q<true>(jxy, jz, cxy, cz16, clo, chi, gpos, q0, e - q0, pi, L, invL, rcs2, x, out,
+                                            out16, kcap, cnt, s_used, pend, slot_of);
                     else
-                        build_chunk_q<false>(jxy, jz, cxy, cz16, clo, chi, gpos, q0, e - q0, pi, L, invL, rcs2, xq,
-                                             xmore, out, out16, kcap, cnt, s_used, pend, slot_of);
+                        build_chunk_q<false>(jxy, jz, cxy, cz16, clo, chi, gpos, q0, e - q0, pi, L, invL, rcs2, x,
+                                             out, out16, kcap, cnt, s_used, pend, slot_of);
                 }
             };
             // the 27 cells in (dz, dy, dx) order; the three cells of a (dz, dy)
