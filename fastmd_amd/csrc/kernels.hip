@@ -636,8 +636,9 @@ __global__ void __launch_bounds__(256) k_scatter(REBUILD_ARGS) {
     }
 }
 
-// Rank inside the cell by global id: deterministic order independent of the
-// atomic arrival order of k_bin.
+// Rank inside the cell by x, then global id: a deterministic order
+// independent of the atomic arrival order of k_bin, in which the list build
+// finds a neighbor cell's atoms within reach along x by binary search.
 __global__ void __launch_bounds__(256) k_rank(REBUILD_ARGS) {
     REBUILD_LOOP {
         REBUILD_PROLOGUE
@@ -646,8 +647,13 @@ __global__ void __launch_bounds__(256) k_rank(REBUILD_ARGS) {
             const int key = d.cell_of[src];
             const int s = d.cell_start[key], e = d.cell_start[key + 1];
             const int g = d.tmp_img[src].w;
+            const float x = d.tmp_pos[src].x;
             int r = 0;
-            for (int q = s; q < e; q++) r += (d.tmp_img[d.order[q]].w < g);
+            for (int q = s; q < e; q++) {
+                const int o = d.order[q];
+                const float xo = d.tmp_pos[o].x;
+                r += (xo < x) || (xo == x && d.tmp_img[o].w < g);
+            }
             d.order2[s + r] = src;
         }
     }
@@ -1016,7 +1022,7 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
         bool staged = W <= kMaxWin && nx >= 3;  // uniform over the group
         const int4 pf = valid ? (int4)gview(d.ipos)[i] : make_int4(0, 0, 0, 0);
         const unsigned cxy = coarse_xy(pf), cz16 = coarse_z(pf);
-        int clo, chi;
+        int clo, chi, xreach = 0;
         {
             // radius in coarse units (L / 2^16), widened by the band the exact
             // FP32 test may differ by and by sqrt(3) units of truncation
@@ -1024,6 +1030,7 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
             const double band = 3e-5 + 1e-6 * (double)L / (double)d.rcs;  // >> the two forms' difference
             const double rin = rc * sqrt(1.0 - band) - 1.7321 - 0.01, rout = rc * sqrt(1.0 + band) + 1.7321 + 0.01;
             clo = rin > 0.0 ? (int)floor(rin * rin) : 0;
+            xreach = (int)ceil(rout);  // a pair within the list radius has |coarse dx| below this
             chi = (int)ceil(rout * rout);
         }
         int nu = 0, nstage = 0;
@@ -1197,10 +1204,32 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
                 }
                 const int4 A = s_cell[ua], B = s_cell[ub], Cc = s_cell[uc];
                 const bool one = ub == ua + 1 && uc == ub + 1;
+                // a cell's atoms are in ascending x: in the -x neighbor only
+                // those from x_i - reach on, in the +x neighbor those up to
+                // x_i + reach (coarse units; with 5 or more cells per axis the
+                // signed 16-bit difference does not wrap)
+                int za = A.z, ec = Cc.z + Cc.y;
+                if (nx >= 5) {
+                    auto dxc = [&](int q) { return (int)(short)(unsigned short)((s_pxy[q] & 0xFFFFu) - (cxy & 0xFFFFu)); };
+                    int lo = A.z, hi = A.z + A.y;  // first with dxc >= -reach
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (dxc(mid) < -xreach) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    za = lo;
+                    lo = Cc.z; hi = Cc.z + Cc.y;  // first with dxc > reach
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (dxc(mid) <= xreach) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    ec = lo;
+                }
                 #pragma unroll 1
                 for (int t = 0; t < (one ? 1 : 3); t++) {
-                    const int z = t == 0 ? A.z : (t == 1 ? B.z : Cc.z);
-                    const int e = one ? Cc.z + Cc.y : (t == 0 ? A.z + A.y : (t == 1 ? B.z + B.y : Cc.z + Cc.y));
+                    const int z = t == 0 ? za : (t == 1 ? B.z : Cc.z);
+                    const int e = one ? ec : (t == 0 ? A.z + A.y : (t == 1 ? B.z + B.y : ec));
                     scan_range(z, e);
                 }
             }
