@@ -186,7 +186,7 @@ static Sizing sizing(const fmd_system_desc* d, int bcap, int acap, int kmult = 1
     b += (uint64_t)s.ncell_cap * 8 + 4;
     b += (uint64_t)s.ntiles * 64 * ((uint64_t)s.kcap * 4 + (uint64_t)s.bcap * 4 + (uint64_t)s.acap * 8);
     b += (uint64_t)(s.ntiles / kGroupTiles) * (4 + (uint64_t)kGroupStage * 4) + (uint64_t)s.nslots + 768;
-    b += (uint64_t)s.ntiles * (kNumPart * 8 + 4) + (uint64_t)((s.ntiles + kChunkTiles - 1) / kChunkTiles) * kNumPart * 8;
+    b += (uint64_t)s.ntiles * (kNumPart * 8 + 4 + 4) + (uint64_t)((s.ntiles + kChunkTiles - 1) / kChunkTiles) * kNumPart * 8;
     b += (uint64_t)(d->natoms + 1) * 8 + (uint64_t)d->nbonds * 8 + (uint64_t)d->nangles * 48;
     b += sizeof(RepState) + (uint64_t)s.thermo_cap * sizeof(ThermoRec) + 4096;
     for (int k = 0; k < d->ntables; k++) b += 24 + (uint64_t)d->table_n[k] * 8 + 1024;
@@ -443,6 +443,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult, b
     D.nnbr = (int*)carve(ns * 4);
     D.nbr = (int*)carve(nt * S.kcap * 64 * 4);
     D.gcount = (int*)carve(nt / kGroupTiles * 4);
+    D.tlen = (int*)carve(nt * 4);
     D.gstage = (int*)carve(nt / kGroupTiles * kGroupStage * 4);
     D.gperm = (unsigned char*)carve(ns);
     D.bhdr = (int*)carve(ns * 4);
@@ -592,6 +593,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult, b
     zero(D.cell_cnt, (size_t)S.ncell_cap * 4);
     zero(D.nnbr, ns * 4);
     zero(D.gcount, nt / kGroupTiles * 4);
+    zero(D.tlen, nt * 4);
     zero(D.bhdr, ns * 4);
     zero(D.wpart, nt * kNumPart * 8);
     zero(D.tdisp, nt * 4);
@@ -1203,7 +1205,7 @@ template <class F>
 void for_each_ptr(RepDesc& d, F&& f) {
     f(d.pos); f(d.vel); f(d.force); f(d.pos_ref); f(d.ipos); f(d.img); f(d.gid);
     f(d.tmp_pos); f(d.tmp_vel); f(d.tmp_img); f(d.cell_of); f(d.rank); f(d.order); f(d.order2);
-    f(d.slot_of); f(d.cell_cnt); f(d.cell_start); f(d.nnbr); f(d.nbr); f(d.gcount); f(d.gstage); f(d.gperm); f(d.bhdr); f(d.bnd); f(d.ang);
+    f(d.slot_of); f(d.cell_cnt); f(d.cell_start); f(d.nnbr); f(d.nbr); f(d.gcount); f(d.tlen); f(d.gstage); f(d.gperm); f(d.bhdr); f(d.bnd); f(d.ang);
     f(d.t_boff); f(d.t_blst); f(d.t_aoff); f(d.t_alst); f(d.t_choff);
     f(d.lj); f(d.bparm); f(d.aparm); f(d.mass);
     f(d.tab_idx); f(d.tab_parm); f(d.tab_no); f(d.tab_fe); f(d.corr);

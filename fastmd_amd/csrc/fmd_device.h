@@ -182,6 +182,7 @@ struct RepDesc {
                            // offset of the neighbor in its group's staged set, or its slot
                            // when the group's gcount is -1
     int* gcount;           // [ntiles / kGroupTiles] atoms of the group's neighbor set, -1: rows hold slots
+    int* tlen;             // [ntiles] padded length of every row of the tile (its longest)
     int* gstage;           // [ntiles / kGroupTiles][kGroupStage] slots of the neighbor set (16-bit if idx16)
     unsigned char* gperm;  // [nslots] atom (index in its group) whose row is in column (tile, lane);
                            // a group's rows are placed by descending length

@@ -118,6 +118,7 @@ template <class T> __device__ inline FMD_GLOBAL T* gmut(T* p) { return (FMD_GLOB
 
 constexpr int kMaxBlockK = 256;  // launch bound of k_build and k_force
 constexpr int kLJChunk = 8;      // neighbors per software-pipelined chunk (kcap % 8 == 0)
+constexpr int kLJTail = 4;       // rows are padded to multiples of this; a last chunk may be this short
 constexpr int kBuildChunk = 8;   // list-build candidates whose loads are issued together
 constexpr int kBndThreads = 256;  // workgroup of the per-replica boundary reduction
 
@@ -1275,7 +1276,7 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
             }
         }
         const int stored = min(cnt, kcap);
-        const int padded = valid ? ((stored + kLJChunk - 1) / kLJChunk) * kLJChunk : 0;
+        const int padded = valid ? ((stored + kLJTail - 1) / kLJTail) * kLJTail : 0;
         int selfent = i;  // the padding entry: the atom itself (rejected by r^2 = 0)
         bool compact = false;
         if (staged) {
@@ -1358,6 +1359,13 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
             maxpad = max(maxpad, lb);
         }
         gmut(d.gperm)[(size_t)grp * kGroupSlots + rank] = (unsigned char)tid;
+        // every row of a wave is padded to the wave's longest (its first
+        // column's), whose length the force pass reads as one uniform bound
+        __syncthreads();  // every read of s_len above precedes the writes below
+        s_len[rank] = padded;
+        __syncthreads();
+        const int wpad = s_len[rank & ~(kTile - 1)];
+        if ((rank & (kTile - 1)) == 0) gmut(d.tlen)[grp * kGroupTiles + rank / kTile] = wpad;
         auto entry = [&](int q) {  // a staged index or slot as the force kernel reads it
             if (!staged) return q;
             if (compact) return (int)s_cidx[q] << 4;
@@ -1375,7 +1383,7 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
                 __syncthreads();
 #pragma unroll
                 for (int e = 0; e < kPlaceBand; e++)
-                    if (k0 + e < padded) dst[(k0 + e) * kTile] = v[e];
+                    if (k0 + e < wpad) dst[(k0 + e) * kTile] = v[e];
                 __syncthreads();
             }
         };
@@ -1392,12 +1400,12 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
                 for (int e = 0; e < kPlaceBand; e++) to[e] = (k0 + e < stored) ? (int)out16[(k0 + e) * kTile] : 0;
             };
             load(0, cur);
-            for (int k0 = 0; k0 < padded; k0 += kPlaceBand) {
-                if (k0 + kPlaceBand < padded) load(k0 + kPlaceBand, nxt);
+            for (int k0 = 0; k0 < wpad; k0 += kPlaceBand) {
+                if (k0 + kPlaceBand < wpad) load(k0 + kPlaceBand, nxt);
 #pragma unroll
                 for (int e = 0; e < kPlaceBand; e++) {
                     const int k = k0 + e;
-                    if (k < padded) dst[k * kTile] = (unsigned short)((k < stored) ? entry(cur[e]) : selfent);
+                    if (k < wpad) dst[k * kTile] = (unsigned short)((k < stored) ? entry(cur[e]) : selfent);
                 }
 #pragma unroll
                 for (int e = 0; e < kPlaceBand; e++) cur[e] = nxt[e];
@@ -1421,10 +1429,12 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
 // compares q with thresholds scaled by 1/u^2 (grid_s, grid_inside),
 // 1/q = u^2 / r^2 (to a relative 1e-10 / r^2) feeds the LJ terms with
 // sigma^2 / u^2 and 24 eps / u^2 folded in, and the force and virial sums are
-// kept in grid units and scaled by u and u^2 once per atom.  The rows are
-// padded to whole chunks of kLJChunk with the atom itself (q = 0 is rejected),
-// so whole chunks are loaded unconditionally; the next chunk's entries are in
-// flight while the current chunk's positions are read.  The cutoff masks 1/q,
+// kept in grid units and scaled by u and u^2 once per atom.  Every row of a
+// wave is padded with the atom itself (q = 0 is rejected) to the wave's
+// longest row, a multiple of kLJTail (tlen), so the wave runs whole chunks of
+// kLJChunk and at most one of kLJTail, loaded unconditionally, under one
+// uniform bound; the next chunk's entries are in flight while the current
+// chunk's positions are read.  The cutoff masks 1/q,
 // so there is no per-pair branch.  OBS adds the per-atom PE and virial (thermo
 // steps and NPT), CAP the push-off pair-force cap, ONE_TYPE a single
 // (eps, sigma) pair with eps and sigma nonzero.  With LDS the entries are byte
@@ -1438,7 +1448,7 @@ struct LJSum {
 template <bool OBS, bool CAP, bool ONE_TYPE, bool W16, bool LDS>
 __device__ __forceinline__ LJSum lj_loop(const RepDesc& d, int lt, int lane, int i, int4 pi, float L,
                                          const int4* sp) {
-    const int nn = d.nnbr[i];
+    const int nw = __builtin_amdgcn_readfirstlane(d.tlen[lt]);  // uniform: the loop control is scalar
     // the tile's rows as a raw buffer: entry k of this lane sits at byte
     // (k * kTile + lane) * entry size; the row offset goes in the scalar
     // offset, so fetching the next chunk costs no vector address arithmetic
@@ -1465,64 +1475,77 @@ __device__ __forceinline__ LJSum lj_loop(const RepDesc& d, int lt, int lane, int
     const float fclamp = fold ? d.lj_fclamp : 1000.0f;
     float gx = 0.f, gy = 0.f, gz = 0.f, pe = 0.f;
     float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f;
+    auto pair = [&](const uint4 pj) __attribute__((always_inline)) {
+        const float ax = grid_d(pi.x, pj.x), ay = grid_d(pi.y, pj.y), az = grid_d(pi.z, pj.z);
+        const float q = grid_s(ax, ay, az, gc.lo_g);
+        // forces only, one (eps, sigma): q == 0 may count as inside (its
+        // force/r is the clamp and finite), one compare fewer per pair
+        const bool in = (fold && !OBS) ? grid_inside0(q, gc.span) : grid_inside(q, gc.span);
+        float s2g = s2g0, c24g = c24g0, c2e = c2e0;
+        if (!ONE_TYPE) {
+            const float2 p = ljrow[pj.w];
+            s2g = p.y * p.y * iu2; c24g = 24.0f * p.x * iu2; c2e = 2.0f * p.x;
+        }
+        const float qinv = in ? __builtin_amdgcn_rcpf(q) : 0.0f;
+        const float sr2 = s2g * qinv;
+        const float sr6 = sr2 * sr2 * sr2;
+        const float sr12 = sr6 * sr6;
+        float fr;
+        if constexpr (fold) fr = (sr2 * sr6) * fmaf(sr6, 2.0f, -1.0f);  // force / r over K
+        else fr = c24g * qinv * (2.0f * sr12 - sr6);        // force / r in real units
+        fr = __builtin_amdgcn_fmed3f(fr, -fclamp, fclamp);  // src/force/lj.cu:48
+        if (CAP) {
+            const float cap = capg * __builtin_amdgcn_rsqf(q);
+            fr = fminf(fmaxf(fr, -cap), cap);
+        }
+        gx += fr * ax; gy += fr * ay; gz += fr * az;
+        // keep the type word live so the neighbor read stays 16 bytes:
+        // gfx950 serves randomly addressed 12-byte LDS reads at about half
+        // the rate of 16-byte ones (the empty asm emits no instruction)
+        if constexpr (LDS && ONE_TYPE) asm volatile("" ::"v"(pj.w));
+        if (OBS) {
+            const float hx = fr * ax, hy = fr * ay, hz = fr * az;
+            pe += c2e * (sr12 - sr6);
+            v0 += hx * ax; v1 += hx * ay; v2 += hx * az;
+            v3 += hy * ay; v4 += hy * az; v5 += hz * az;
+        }
+    };
+    auto neighbor = [&](unsigned e) __attribute__((always_inline)) -> uint4 {
+        if constexpr (LDS) {  // the row holds the LDS byte offset of the staged neighbor
+            const int4 v = *(const int4*)((const char*)sp + e);
+            return make_uint4((unsigned)v.x, (unsigned)v.y, (unsigned)v.z, (unsigned)v.w);
+        } else {
+            const u32x4 v = struct_load_b128(prs, e, 0, 0, 0);
+            return make_uint4(v[0], v[1], v[2], v[3]);
+        }
+    };
+    // nw, the wave's padded row length (a multiple of kLJTail, the same for
+    // every lane: the list build pads each row of a wave to it), bounds whole
+    // chunks of kLJChunk and a last chunk of kLJTail; entries fetched past nw
+    // (within the row area, or zero from the bounded buffer) are never used
     unsigned nxt[kLJChunk];
-    if (nn > 0) {
+    if (nw > 0) {
 #pragma unroll
         for (int k = 0; k < kLJChunk; k++) nxt[k] = entry(k);
     }
-    for (int k0 = 0; k0 < nn; k0 += kLJChunk) {
+    int k0 = 0;
+    for (; k0 + kLJChunk <= nw; k0 += kLJChunk) {
         uint4 pjs[kLJChunk];
 #pragma unroll
-        for (int k = 0; k < kLJChunk; k++) {
-            if constexpr (LDS) {  // the row holds the LDS byte offset of the staged neighbor
-                const int4 v = *(const int4*)((const char*)sp + nxt[k]);
-                pjs[k] = make_uint4((unsigned)v.x, (unsigned)v.y, (unsigned)v.z, (unsigned)v.w);
-            } else {
-                const u32x4 v = struct_load_b128(prs, nxt[k], 0, 0, 0);
-                pjs[k] = make_uint4(v[0], v[1], v[2], v[3]);
-            }
-        }
-        if (k0 + kLJChunk < nn) {
+        for (int k = 0; k < kLJChunk; k++) pjs[k] = neighbor(nxt[k]);
+        if (k0 + kLJChunk < nw) {
 #pragma unroll
             for (int k = 0; k < kLJChunk; k++) nxt[k] = entry(k0 + kLJChunk + k);
         }
 #pragma unroll
-        for (int k = 0; k < kLJChunk; k++) {
-            const uint4 pj = pjs[k];
-            const float ax = grid_d(pi.x, pj.x), ay = grid_d(pi.y, pj.y), az = grid_d(pi.z, pj.z);
-            const float q = grid_s(ax, ay, az, gc.lo_g);
-            // forces only, one (eps, sigma): q == 0 may count as inside (its
-            // force/r is the clamp and finite), one compare fewer per pair
-            const bool in = (fold && !OBS) ? grid_inside0(q, gc.span) : grid_inside(q, gc.span);
-            float s2g = s2g0, c24g = c24g0, c2e = c2e0;
-            if (!ONE_TYPE) {
-                const float2 p = ljrow[pj.w];
-                s2g = p.y * p.y * iu2; c24g = 24.0f * p.x * iu2; c2e = 2.0f * p.x;
-            }
-            const float qinv = in ? __builtin_amdgcn_rcpf(q) : 0.0f;
-            const float sr2 = s2g * qinv;
-            const float sr6 = sr2 * sr2 * sr2;
-            const float sr12 = sr6 * sr6;
-            float fr;
-            if constexpr (fold) fr = (sr2 * sr6) * fmaf(sr6, 2.0f, -1.0f);  // force / r over K
-            else fr = c24g * qinv * (2.0f * sr12 - sr6);        // force / r in real units
-            fr = __builtin_amdgcn_fmed3f(fr, -fclamp, fclamp);  // src/force/lj.cu:48
-            if (CAP) {
-                const float cap = capg * __builtin_amdgcn_rsqf(q);
-                fr = fminf(fmaxf(fr, -cap), cap);
-            }
-            gx += fr * ax; gy += fr * ay; gz += fr * az;
-            // keep the type word live so the neighbor read stays 16 bytes:
-            // gfx950 serves randomly addressed 12-byte LDS reads at about half
-            // the rate of 16-byte ones (the empty asm emits no instruction)
-            if constexpr (LDS && ONE_TYPE) asm volatile("" ::"v"(pj.w));
-            if (OBS) {
-                const float hx = fr * ax, hy = fr * ay, hz = fr * az;
-                pe += c2e * (sr12 - sr6);
-                v0 += hx * ax; v1 += hx * ay; v2 += hx * az;
-                v3 += hy * ay; v4 += hy * az; v5 += hz * az;
-            }
-        }
+        for (int k = 0; k < kLJChunk; k++) pair(pjs[k]);
+    }
+    if (k0 < nw) {  // nw - k0 == kLJTail
+        uint4 pjs[kLJTail];
+#pragma unroll
+        for (int k = 0; k < kLJTail; k++) pjs[k] = neighbor(nxt[k]);
+#pragma unroll
+        for (int k = 0; k < kLJTail; k++) pair(pjs[k]);
     }
     const float fu = kf * u, hu2 = 0.5f * kf * u * u;
     return LJSum{gx * fu, gy * fu, gz * fu, pe, v0 * hu2, v1 * hu2, v2 * hu2, v3 * hu2, v4 * hu2, v5 * hu2};
