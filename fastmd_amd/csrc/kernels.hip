@@ -639,7 +639,11 @@ __global__ void __launch_bounds__(256) k_scatter(REBUILD_ARGS) {
 
 // Rank inside the cell by x, then global id: a deterministic order
 // independent of the atomic arrival order of k_bin, in which the list build
-// finds a neighbor cell's atoms within reach along x by binary search.
+// finds the atoms of a row of cells within reach along x by binary search.
+// x is the grid coordinate floor(x 2^32 / L) taken modulo 2^32 relative to the
+// cell's lower bound floor(cx 2^32 / nx): the list build compares grid
+// coordinates modulo 2^32, and a position a rounding step outside [0, L) (at
+// -0 or L itself) stands at the cell's end where the grid coordinate puts it.
 __global__ void __launch_bounds__(256) k_rank(REBUILD_ARGS) {
     REBUILD_LOOP {
         REBUILD_PROLOGUE
@@ -648,12 +652,16 @@ __global__ void __launch_bounds__(256) k_rank(REBUILD_ARGS) {
             const int key = d.cell_of[src];
             const int s = d.cell_start[key], e = d.cell_start[key + 1];
             const int g = d.tmp_img[src].w;
+            const int nx = st->nx;
+            const double sc = 4294967296.0 / (double)st->Lp;
             const float x = d.tmp_pos[src].x;
+            const unsigned lo = (unsigned)(((unsigned long long)cell_coord(x, st->inv_cs, nx) << 32) / (unsigned)nx);
+            const int rx = (int)(fix_coord(x, sc) - lo);
             int r = 0;
             for (int q = s; q < e; q++) {
                 const int o = d.order[q];
-                const float xo = d.tmp_pos[o].x;
-                r += (xo < x) || (xo == x && d.tmp_img[o].w < g);
+                const int ro = (int)(fix_coord(d.tmp_pos[o].x, sc) - lo);
+                r += (ro < rx) || (ro == rx && d.tmp_img[o].w < g);
             }
             d.order2[s + r] = src;
         }
@@ -776,57 +784,69 @@ __device__ __forceinline__ void build_chunk(const float4 (&pj)[kBuildChunk], int
 // band the exact FP32 test may differ by).  The few candidates in between --
 // about 1e-3 sigma either side of the radius -- are decided by the reference
 // FP32 formula on their positions.
+template <bool B> struct BoolC { static constexpr bool value = B; };
 typedef short fmd_s16x2 __attribute__((ext_vector_type(2)));
 typedef unsigned short fmd_u16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ fmd_s16x2 coarse_diff(unsigned a, unsigned b) {  // two 16-bit lanes, modulo 2^16
     return __builtin_bit_cast(fmd_s16x2, __builtin_bit_cast(fmd_u16x2, a) - __builtin_bit_cast(fmd_u16x2, b));
 }
-__device__ __forceinline__ int coarse_q(unsigned axy, unsigned az, unsigned bxy, unsigned bz) {
+// qc + acc for the two packed coordinate words of each atom, saturating (a
+// sum past INT_MAX is far outside any radius); acc enters the first product
+__device__ __forceinline__ int coarse_t(unsigned axy, unsigned az, unsigned bxy, unsigned bz, int acc) {
     const fmd_s16x2 dxy = coarse_diff(axy, bxy), dz = coarse_diff(az, bz);
-    // saturating accumulate: a sum past INT_MAX is far outside any radius
-    return __builtin_amdgcn_sdot2(dz, dz, __builtin_amdgcn_sdot2(dxy, dxy, 0, true), true);
+    return __builtin_amdgcn_sdot2(dz, dz, __builtin_amdgcn_sdot2(dxy, dxy, acc, true), true);
 }
 __device__ __forceinline__ unsigned coarse_xy(int4 p) {
     return ((unsigned)p.x >> 16) | ((unsigned)p.y & 0xFFFF0000u);
 }
 __device__ __forceinline__ unsigned coarse_z(int4 p) { return (unsigned)p.z >> 16; }
+// The lowest r set bits of m (none for r <= 0).
+__device__ inline unsigned lowest_bits(unsigned m, int r) {
+    unsigned keep = 0u;
+    for (int j = 0; j < r && m; j++) {
+        const unsigned b = m & (0u - m);
+        keep |= b;
+        m ^= b;
+    }
+    return keep;
+}
 // A candidate the prefilter leaves undecided needs the reference distance on
 // its float position, a global load.  The first such candidate of a lane is
 // appended as inside and recorded in pend (row position << 12 | staged index);
 // after the scan the lane decides it, with one load round trip for the whole
 // wave, and turns it into the atom's own entry (which every consumer skips) if
 // it lies outside.  Further undecided candidates of the lane are decided at
-// once.  Candidates are staged indices q0 + k (k < n); x holds the bits of
-// the candidates to leave out (the atom itself and its bonded partners),
-// slot_of maps a staged index to its slot, and used[q] marks atom q as in a
-// row.
+// once.  Candidates are staged indices q0 + k (k < n, n >= 1) with coarse
+// positions jp[k] = (x | y << 16, z); x holds the bits of the candidates to
+// leave out (the atom itself and its bonded partners; bits from kBuildChunk
+// up are ignored), bw = chi - clo is the width of the undecided band, slot_of
+// maps a staged index to its slot, and used[q] marks atom q as in a row.
 template <bool W16, class SlotOf>
-__device__ __forceinline__ void build_chunk_q(const unsigned (&jxy)[kBuildChunk], const unsigned (&jz)[kBuildChunk],
-                                              unsigned pxy, unsigned pz, int clo, int chi,
-                                              const FMD_GLOBAL float4* gpos, int q0, int n, float4 pi, float L,
-                                              float invL, float rcs2, unsigned x, FMD_GLOBAL int* out,
+__device__ __forceinline__ void build_chunk_q(const uint2 (&jp)[kBuildChunk], unsigned pxy, unsigned pz, int clo,
+                                              unsigned bw, const FMD_GLOBAL float4* gpos, int q0, int n, float4 pi,
+                                              float L, float invL, float rcs2, unsigned x, FMD_GLOBAL int* out,
                                               FMD_GLOBAL unsigned short* out16, int kcap, int& cnt, unsigned char* used,
                                               int& pend, SlotOf slot_of) {
-    // t = q - clo (q >= 0 and clo in [0, 2^31 - 1): no overflow) is negative
-    // for a candidate certainly inside.  Its sign bit is shifted into m from
-    // the last candidate down (alignbit of (m, t) by 31 is m << 1 | t >> 31),
-    // so bit k of m is candidate k; the smallest t as an unsigned number is
-    // the nearest candidate not certainly inside, and only when it lies in the
-    // band [0, chi - clo] are the band's candidates found one by one.
+    // t = q - clo is negative for a candidate certainly inside.  Its sign bit
+    // is shifted into m from the last candidate down (alignbit of (m, t) by 31
+    // is m << 1 | t >> 31), so bit k of m is candidate k; the smallest t as an
+    // unsigned number is the nearest candidate not certainly inside, and only
+    // when it lies in the band [0, bw] are the band's candidates found one by
+    // one.
     unsigned m = 0u, umin = ~0u;
     int t[kBuildChunk];
 #pragma unroll
     for (int k = kBuildChunk - 1; k >= 0; k--) {
-        t[k] = coarse_q(pxy, pz, jxy[k], jz[k]) - clo;
+        t[k] = coarse_t(pxy, pz, jp[k].x, jp[k].y, -clo);
         m = __builtin_amdgcn_alignbit(m, (unsigned)t[k], 31);
         umin = min(umin, (unsigned)t[k]);
     }
-    const unsigned valid = n >= kBuildChunk ? (1u << kBuildChunk) - 1u : (1u << n) - 1u;  // k < n
+    const unsigned valid = (1u << min(n, kBuildChunk)) - 1u;  // k < n
     m &= valid & ~x;
-    if (umin <= (unsigned)(chi - clo)) {
+    if (umin <= bw) {
         unsigned band = 0u;
 #pragma unroll
-        for (int k = 0; k < kBuildChunk; k++) band |= ((unsigned)t[k] <= (unsigned)(chi - clo)) ? (1u << k) : 0u;
+        for (int k = 0; k < kBuildChunk; k++) band |= ((unsigned)t[k] <= bw) ? (1u << k) : 0u;
         band &= valid & ~x;
         if (band && pend < 0) {  // defer the first: inside for now, its row position recorded
             const int k = __builtin_ctz(band);
@@ -838,17 +858,20 @@ __device__ __forceinline__ void build_chunk_q(const unsigned (&jxy)[kBuildChunk]
             if ((band >> k) & 1u)
                 m |= (dist2_ref(pi, gpos[slot_of(q0 + k)], L, invL) < rcs2) ? (1u << k) : 0u;
     }
-    // one pass per set bit, lowest first: a wave runs as many passes as its
-    // fullest lane has pairs in the chunk
+    // the set bits in order, one pass per bit (a wave runs as many passes as
+    // its fullest lane has pairs in the chunk); entries past the row capacity
+    // are counted, not stored
+    const int p = __popc(m);
+    if (cnt + p > kcap) m = lowest_bits(m, kcap - cnt);
+    unsigned off = (unsigned)cnt * kTile;
+    cnt += p;
     while (m) {
         const int k = __builtin_ctz(m);
         m &= m - 1u;
-        if (cnt < kcap) {
-            if (W16) out16[cnt * kTile] = (unsigned short)(q0 + k);
-            else out[cnt * kTile] = q0 + k;
-            used[q0 + k] = 1;
-        }
-        cnt++;
+        if (W16) out16[off] = (unsigned short)(q0 + k);
+        else out[off] = q0 + k;
+        used[q0 + k] = 1;
+        off += kTile;
     }
 }
 
@@ -878,9 +901,9 @@ __device__ inline int block_excl_scan(int x, int* s_wsum, int* total) {
 // At most 128 VGPRs (four waves per SIMD, four workgroups per CU with the
 // LDS below): the rebuilding replicas' groups fit in fewer rounds.
 __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_eu(4, 8))) k_build(REBUILD_ARGS) {
-    // staged union, coarse fixed point: x and y in one word, z in another (one
-    // chunk of padding: a cell's last chunk reads whole)
-    __shared__ unsigned s_pxy[kBuildStage + kBuildChunk], s_pz[kBuildStage + kBuildChunk];
+    // staged union, coarse fixed point: x | y << 16 and z of each atom side by
+    // side (one chunk of padding: a cell's last chunk reads whole)
+    __shared__ uint2 s_pj[kBuildStage + kBuildChunk];
     __shared__ unsigned short s_seg[kBuildStage];  // staged atom -> union cell
     __shared__ unsigned char s_used[kBuildStage];  // staged atom is in a row of the group
     __shared__ int4 s_cell[kMaxUnion];             // (first slot, count, LDS offset, key)
@@ -890,8 +913,9 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
     __shared__ int s_wsum[kGroupTiles];
     __shared__ int s_c0[3], s_nu, s_nstage;
     __shared__ int s_len[kGroupSlots];              // padded row length of each atom of the group
-    unsigned short* s_cidx = (unsigned short*)s_pxy;  // after the scan: compact index of a used atom
-    const int wv = threadIdx.x >> 6, lane = lane_id(), tid = threadIdx.x;
+    unsigned short* s_cidx = (unsigned short*)s_pj;  // after the scan: compact index of a used atom
+    // the wave index as a scalar: each wave's tile and replica fields are uniform
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = lane_id(), tid = threadIdx.x;
     for (int gb = blockIdx.x, ng = *nrt / kGroupTiles; gb < ng; gb += gridDim.x) {
         const Tile t = tiles[rlist[gb * kGroupTiles + wv]];
         const RepDesc& d = descs[t.rep];
@@ -1023,16 +1047,17 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
         bool staged = W <= kMaxWin && nx >= 3;  // uniform over the group
         const int4 pf = valid ? (int4)gview(d.ipos)[i] : make_int4(0, 0, 0, 0);
         const unsigned cxy = coarse_xy(pf), cz16 = coarse_z(pf);
-        int clo, chi, xreach = 0;
+        int clo, chi, rb2;
         {
             // radius in coarse units (L / 2^16), widened by the band the exact
             // FP32 test may differ by and by sqrt(3) units of truncation
             const double rc = (double)d.rcs * 65536.0 / (double)L;
             const double band = 3e-5 + 1e-6 * (double)L / (double)d.rcs;  // >> the two forms' difference
-            const double rin = rc * sqrt(1.0 - band) - 1.7321 - 0.01, rout = rc * sqrt(1.0 + band) + 1.7321 + 0.01;
+            const double rb = rc * sqrt(1.0 + band);  // above the true distance of any pair the FP32 test accepts
+            const double rin = rc * sqrt(1.0 - band) - 1.7321 - 0.01, rout = rb + 1.7321 + 0.01;
             clo = rin > 0.0 ? (int)floor(rin * rin) : 0;
-            xreach = (int)ceil(rout);  // a pair within the list radius has |coarse dx| below this
             chi = (int)ceil(rout * rout);
+            rb2 = (int)fmin(ceil(rb * rb) + 1.0, 2147483647.0);
         }
         int nu = 0, nstage = 0;
         if (staged) {
@@ -1130,10 +1155,7 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     const int q = q0 + k * kGroupSlots + tid;
-                    if (q < nstage) {
-                        s_pxy[q] = coarse_xy(v4[k]);
-                        s_pz[q] = coarse_z(v4[k]);
-                    }
+                    if (q < nstage) s_pj[q] = make_uint2(coarse_xy(v4[k]), coarse_z(v4[k]));
                 }
             }
             __syncthreads();
@@ -1144,96 +1166,120 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
                 const int us = s_wmap[win_of(13)];
                 if (us < nu) qself = s_cell[us].z + (i - s_cell[us].x);
             }
-            // exclusions as staged indices: the atom itself and its bonded
-            // partners (a partner outside the 27 cells is no candidate), sorted
-            // ascending (none = INT_MAX); each range consumes them in order
+            // exclusions as staged indices: the atom itself and its first four
+            // bonded partners (none, or a partner outside the 27 cells, which is
+            // no candidate: INT_MAX)
             int xs[5] = {qself, INT_MAX, INT_MAX, INT_MAX, INT_MAX};
 #pragma unroll
             for (int b = 0; b < 4; b++) {
                 const int q = ex[b] >= 0 ? partner_q(ex[b]) : -1;
                 xs[b + 1] = q >= 0 ? q : INT_MAX;
             }
-            auto cswap = [](int& a, int& b) { const int lo = min(a, b), hi = max(a, b); a = lo; b = hi; };
-            cswap(xs[0], xs[1]); cswap(xs[3], xs[4]); cswap(xs[2], xs[4]); cswap(xs[2], xs[3]); cswap(xs[1], xs[4]);
-            cswap(xs[0], xs[3]); cswap(xs[0], xs[2]); cswap(xs[1], xs[3]); cswap(xs[1], xs[2]);
+            int nxu = 1;  // entries from nxu on are INT_MAX in every lane of the wave
+#pragma unroll
+            for (int b = 1; b < 5; b++) nxu = __ballot(xs[b] != INT_MAX) ? b + 1 : nxu;
             auto slot_of = [&](int q) __attribute__((always_inline)) {
                 const int4 ce = s_cell[s_seg[q]];
                 return ce.x + q - ce.z;
             };
-            auto scan_range = [&](int z, int e) __attribute__((always_inline)) {
-                // the first exclusion at or past z, and its successors
-                int xp = (xs[0] < z) + (xs[1] < z) + (xs[2] < z) + (xs[3] < z) + (xs[4] < z);
-                auto xat = [&](int p) {
-                    return p == 0 ? xs[0] : (p == 1 ? xs[1] : (p == 2 ? xs[2] : (p == 3 ? xs[3] : (p == 4 ? xs[4] : INT_MAX))));
-                };
-                int xn = xat(xp);
-                for (int q0 = z; q0 < e; q0 += kBuildChunk) {
-                    // a whole chunk from LDS (entries past the range are masked
-                    // out, and the arrays are padded by a chunk)
-                    unsigned jxy[kBuildChunk], jz[kBuildChunk];
+            // Lower bounds (coarse units) of the distance along y and z from
+            // this atom to any atom of the cell below and above its own: its
+            // offset from its cell's bounds floor(c 2^16 / nx), less 2 units
+            // (the truncation of the atom's coordinate and of the bound, and
+            // the rounding of the float cell coordinate).  With 5 or more cells
+            // per axis these offsets, taken as signed 16-bit differences, do
+            // not wrap.
+            int yb[2] = {0, 0}, zb[2] = {0, 0};
+            const bool trim = nx >= 5;
+            if (trim) {
+                auto cb = [&](int c) { return (int)(((unsigned)c << 16) / (unsigned)nx); };
+                auto off16 = [](int a, int b) { return max(0, (int)(short)(unsigned short)(a - b) - 2); };
+                const int yc = (int)(cxy >> 16), zc = (int)cz16;
+                yb[0] = off16(yc, cb(cy)); yb[1] = off16(cb(cy + 1), yc);
+                zb[0] = off16(zc, cb(cz)); zb[1] = off16(cb(cz + 1), zc);
+            }
+            // x offset of staged atom q from this atom (coarse units, signed)
+            auto dxc = [&](int q) { return (int)(short)(unsigned short)(s_pj[q].x - cxy); };
+            // first q in [lo, hi) with dxc(q) > v: a cell's atoms are in
+            // ascending x, and so are the three cells of a row in x order
+            auto first_above = [&](int lo, int hi, int v) {
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (dxc(mid) <= v) lo = mid + 1;
+                    else hi = mid;
+                }
+                return lo;
+            };
+            auto scan_rows = [&](auto w16) __attribute__((always_inline)) {
+                constexpr bool W16 = decltype(w16)::value;
+                auto scan_range = [&](int z, int e) __attribute__((always_inline)) {
+                    for (int q0 = z; q0 < e; q0 += kBuildChunk) {
+                        // a whole chunk from LDS (entries past the range are
+                        // masked out, and the array is padded by a chunk)
+                        uint2 jp[kBuildChunk];
 #pragma unroll
-                    for (int k = 0; k < kBuildChunk; k++) {
-                        jxy[k] = s_pxy[q0 + k];
-                        jz[k] = s_pz[q0 + k];
+                        for (int k = 0; k < kBuildChunk; k++) jp[k] = s_pj[q0 + k];
+                        // bit xs[b] - q0 for each exclusion; one outside the
+                        // chunk lands on a bit from 8 to 31, which is dropped
+                        auto xbit = [&](int v) { return 1u << min((unsigned)(v - q0), 31u); };
+                        unsigned x = xbit(xs[0]) | xbit(xs[1]) | xbit(xs[2]);
+                        if (__builtin_expect(nxu > 3, 0)) x |= xbit(xs[3]) | xbit(xs[4]);
+                        for (int b = 4; b < nb && b < d.bcap; b++) {  // partners past the first four
+                            const unsigned o = (unsigned)(partner_q(bt[b * kTile] & kSlotMask) - q0);
+                            x |= (o < (unsigned)kBuildChunk) ? (1u << o) : 0u;
+                        }
+                        build_chunk_q<W16>(jp, cxy, cz16, clo, (unsigned)(chi - clo), gpos, q0, e - q0, pi, L, invL,
+                                           rcs2, x, out, out16, kcap, cnt, s_used, pend, slot_of);
                     }
-                    unsigned x = 0u;
-                    while ((unsigned)(xn - q0) < (unsigned)kBuildChunk) {
-                        x |= 1u << (xn - q0);
-                        xn = xat(++xp);
+                };
+                // the 27 cells in (dz, dy, dx) order; the three cells of a
+                // (dz, dy) row are consecutive union cells (hence one contiguous
+                // staged range) unless the window wraps that axis
+                #pragma unroll 1
+                for (int r = 0; r < 9; r++) {
+                    const int ua = s_wmap[win_of(3 * r)], ub = s_wmap[win_of(3 * r + 1)],
+                              uc = s_wmap[win_of(3 * r + 2)];
+                    if (ua >= nu || ub >= nu || uc >= nu) {  // every neighbor cell was marked; guard regardless
+                        err |= ERR_INTERNAL;
+                        continue;
                     }
-                    for (int b = 4; b < nb && b < d.bcap; b++) {  // partners past the first four (not in KG chains)
-                        const unsigned o = (unsigned)(partner_q(bt[b * kTile] & kSlotMask) - q0);
-                        x |= (o < (unsigned)kBuildChunk) ? (1u << o) : 0u;
+                    const int4 A = s_cell[ua], B = s_cell[ub], Cc = s_cell[uc];
+                    const bool one = ub == ua + 1 && uc == ub + 1;
+                    // Only atoms within xr of this atom along x can be in the
+                    // list: the distance along y and z to the row's cells leaves
+                    // at most sqrt(rb2 - dy^2 - dz^2) along x, and the coarse x
+                    // difference is within one unit of the true one.  A row with
+                    // nothing left is skipped whole.
+                    int xr = 0;
+                    if (trim) {
+                        const int dyb = (r % 3 == 1) ? 0 : yb[r % 3 >> 1], dzb = (r / 3 == 1) ? 0 : zb[r / 3 >> 1];
+                        const int rem = rb2 - dyb * dyb - dzb * dzb;
+                        if (rem < 0) continue;
+                        xr = (int)ceilf(sqrtf((float)rem)) + 2;
                     }
-                    if (d.idx16)
-                        build_chunk_q<true>(jxy, jz, cxy, cz16, clo, chi, gpos, q0, e - q0, pi, L, invL, rcs2, x, out,
-                                            out16, kcap, cnt, s_used, pend, slot_of);
-                    else
-                        build_chunk_q<false>(jxy, jz, cxy, cz16, clo, chi, gpos, q0, e - q0, pi, L, invL, rcs2, x,
-                                             out, out16, kcap, cnt, s_used, pend, slot_of);
+                    if (one) {
+                        int z = A.z, e = Cc.z + Cc.y;
+                        if (trim) {
+                            z = first_above(z, e, -xr - 1);
+                            e = first_above(z, e, xr);
+                        }
+                        scan_range(z, e);
+                    } else {
+                        #pragma unroll 1
+                        for (int t = 0; t < 3; t++) {
+                            const int4 ce = t == 0 ? A : (t == 1 ? B : Cc);
+                            int z = ce.z, e = ce.z + ce.y;
+                            if (trim) {
+                                z = first_above(z, e, -xr - 1);
+                                e = first_above(z, e, xr);
+                            }
+                            scan_range(z, e);
+                        }
+                    }
                 }
             };
-            // the 27 cells in (dz, dy, dx) order; the three cells of a (dz, dy)
-            // row are consecutive union cells (hence one contiguous staged
-            // range) unless the window wraps that axis
-            #pragma unroll 1
-            for (int r = 0; r < 9; r++) {
-                const int ua = s_wmap[win_of(3 * r)], ub = s_wmap[win_of(3 * r + 1)], uc = s_wmap[win_of(3 * r + 2)];
-                if (ua >= nu || ub >= nu || uc >= nu) {  // every neighbor cell was marked; guard regardless
-                    err |= ERR_INTERNAL;
-                    continue;
-                }
-                const int4 A = s_cell[ua], B = s_cell[ub], Cc = s_cell[uc];
-                const bool one = ub == ua + 1 && uc == ub + 1;
-                // a cell's atoms are in ascending x: in the -x neighbor only
-                // those from x_i - reach on, in the +x neighbor those up to
-                // x_i + reach (coarse units; with 5 or more cells per axis the
-                // signed 16-bit difference does not wrap)
-                int za = A.z, ec = Cc.z + Cc.y;
-                if (nx >= 5) {
-                    auto dxc = [&](int q) { return (int)(short)(unsigned short)((s_pxy[q] & 0xFFFFu) - (cxy & 0xFFFFu)); };
-                    int lo = A.z, hi = A.z + A.y;  // first with dxc >= -reach
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if (dxc(mid) < -xreach) lo = mid + 1;
-                        else hi = mid;
-                    }
-                    za = lo;
-                    lo = Cc.z; hi = Cc.z + Cc.y;  // first with dxc > reach
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if (dxc(mid) <= xreach) lo = mid + 1;
-                        else hi = mid;
-                    }
-                    ec = lo;
-                }
-                #pragma unroll 1
-                for (int t = 0; t < (one ? 1 : 3); t++) {
-                    const int z = t == 0 ? za : (t == 1 ? B.z : Cc.z);
-                    const int e = one ? ec : (t == 0 ? A.z + A.y : (t == 1 ? B.z + B.y : ec));
-                    scan_range(z, e);
-                }
-            }
+            if (d.idx16) scan_rows(BoolC<true>{});
+            else scan_rows(BoolC<false>{});
             // the deferred candidate: one load for the wave; outside -> the atom's
             // own entry (its staged index), which the force pass and the pair
             // readout skip
@@ -1301,7 +1347,7 @@ __global__ void __launch_bounds__(kGroupSlots) __attribute__((amdgpu_waves_per_e
                     if (q2 >= 0) s_used[q2] = 1;
                 }
             }
-            // the scan's last reads of s_pxy precede the s_cidx writes below
+            // the scan's last reads of s_pj precede the s_cidx writes below
             pmiss = __syncthreads_or(pmiss);
             const int per = (nstage + kGroupSlots - 1) / kGroupSlots;
             const int q0 = min(tid * per, nstage), q1 = min(q0 + per, nstage);
