@@ -16,6 +16,8 @@ namespace fmd {
 constexpr int kNumClasses = 3;
 constexpr int kRgScratchChains = 1 << 16;  // chains the allocation-free Rg read holds
 constexpr int kGraphSteps = 20;
+// batches of at least this many replicas build lists beside the force pass
+constexpr int kForkReplicas = 8;
 
 int size_class(int natoms);
 
@@ -162,6 +164,8 @@ class Engine {
     std::string err_;
     hipStream_t stream_ = nullptr;
     hipStream_t side_ = nullptr;  // system creation beside the main stream's launches
+    hipStream_t aux_ = nullptr;   // a step's list rebuilds beside its force pass
+    hipEvent_t ev_fork_ = nullptr, ev_sorted_ = nullptr, ev_join_ = nullptr;
     double* scratch_ = nullptr;   // this device's scratch array (device address), or null
     bool defer_free_ = false;
     uint64_t budget_ = 0, in_use_ = 0;

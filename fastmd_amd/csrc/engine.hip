@@ -44,6 +44,10 @@ Engine::Engine(int device, uint64_t budget) : device_(device) {
     if (hip_check(hipSetDevice(device), "hipSetDevice")) return;
     if (hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate")) return;
     if (hip_check(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "hipStreamCreate")) return;
+    if (hip_check(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking), "hipStreamCreate")) return;
+    if (hip_check(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming), "hipEventCreate")) return;
+    if (hip_check(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming), "hipEventCreate")) return;
+    if (hip_check(hipEventCreateWithFlags(&ev_sorted_, hipEventDisableTiming), "hipEventCreate")) return;
     size_t fr = 0, tot = 0;
     if (hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo")) return;
     budget_ = budget ? budget : (uint64_t)(0.9 * (double)fr);
@@ -66,6 +70,7 @@ Engine::Engine(int device, uint64_t budget) : device_(device) {
 Engine::~Engine() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (side_) (void)hipStreamSynchronize(side_);
+    if (aux_) (void)hipStreamSynchronize(aux_);
     for (auto& kv : batches_) {
         free_batch(kv.second);
     }
@@ -78,6 +83,10 @@ Engine::~Engine() {
     if (d_descs_) (void)hipFree(d_descs_);
     if (stream_) (void)hipStreamDestroy(stream_);
     if (side_) (void)hipStreamDestroy(side_);
+    if (aux_) (void)hipStreamDestroy(aux_);
+    if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+    if (ev_join_) (void)hipEventDestroy(ev_join_);
+    if (ev_sorted_) (void)hipEventDestroy(ev_sorted_);
 }
 
 int32_t Engine::fail(int32_t code, const std::string& msg) {
@@ -762,20 +771,45 @@ Launch Engine::launch_of(const Batch& b, hipStream_t s) const {
     return Launch{d_descs_, b.d_tiles, b.ntl, b.d_reps, b.nr, b.d_rep_t0, b.d_rlist, b.d_nrt, b.d_chunks, b.nch, s};
 }
 
-// One MD step for every replica of the batch.
+// One MD step for every replica of the batch.  With many replicas, those that
+// rebuild their lists this step (about one in twenty) sort their atoms, build
+// their lists and compute their forces on the auxiliary stream, while the
+// force pass of all the others runs on the main stream from the end of that
+// sort: the list build's workgroups (the larger) are queued first, so they are
+// not starved, and the force pass fills the CUs they leave and free.  The
+// branches touch disjoint replicas, and a captured graph keeps them
+// concurrent.  A batch of few replicas gains little beside the cost of the
+// cross-stream waits and runs its steps in one stream.
 void Engine::launch_step_seq(Batch& b) {
     Launch l = launch_of(b);
     launch_pre(l);
     launch_decide(l);
-    launch_rebuild(l, 0, 1);
-    // the force pass over each size class's tile range with that class's
-    // tuned workgroup shape (adjacent classes of one shape in one launch)
-    for (int c = 0; c < kNumClasses;) {
-        int e = c + 1;
-        while (e < kNumClasses && tuned_[0][e] == tuned_[0][c]) e++;
-        const int t0 = b.cls_t0[c], nt = b.cls_t0[e - 1] + b.cls_nt[e - 1] - t0;
-        launch_force(l, t0, nt, tuned_[0][c], FMD_TERM_ALL, 1, FM_STEP);
-        c = e;
+    auto staying = [&](int sel) {
+        // each size class's tile range with that class's tuned workgroup
+        // shape (adjacent classes of one shape in one launch)
+        for (int c = 0; c < kNumClasses;) {
+            int e = c + 1;
+            while (e < kNumClasses && tuned_[0][e] == tuned_[0][c]) e++;
+            const int t0 = b.cls_t0[c], nt = b.cls_t0[e - 1] + b.cls_nt[e - 1] - t0;
+            launch_force(l, t0, nt, tuned_[0][c], FMD_TERM_ALL, 1, FM_STEP, sel);
+            c = e;
+        }
+    };
+    if (b.nr >= kForkReplicas) {
+        const Launch la = launch_of(b, aux_);
+        (void)hipEventRecord(ev_fork_, stream_);
+        (void)hipStreamWaitEvent(aux_, ev_fork_, 0);
+        launch_rebuild(la, 0, 0);
+        (void)hipEventRecord(ev_sorted_, aux_);
+        launch_build(la);
+        launch_force_listed(la, 1, FM_STEP);
+        (void)hipEventRecord(ev_join_, aux_);
+        (void)hipStreamWaitEvent(stream_, ev_sorted_, 0);
+        staying(FS_STAYING);
+        (void)hipStreamWaitEvent(stream_, ev_join_, 0);
+    } else {
+        launch_rebuild(l, 0, 1);
+        staying(FS_ALL);
     }
     launch_boundary(l, BND_STEP);
 }

@@ -29,12 +29,20 @@ void launch_decide(const Launch& l);
 // The rebuild phases (list of rebuilding tiles, Morton cell sort and, if
 // `build`, the Verlet lists); `always` ignores the active/rebuild flags.
 void launch_rebuild(const Launch& l, int always, int build);
+// The Verlet lists of the tiles the last rebuild phases listed.
+void launch_build(const Launch& l);
 // Force kernel over tiles [t0, t0+nt) of the list with the given block size.
 // Force kernel modes: FM_STEP skips inactive replicas and records PE and
 // virial on the steps that want them; FM_OBS always runs and records them;
 // FM_FAST always runs and records forces only (a plain step's work).
 enum ForceMode { FM_STEP = 0, FM_OBS = 1, FM_FAST = 2 };
-void launch_force(const Launch& l, int t0, int nt, int block, int terms, int kick, int mode);
+// Tile selection: every tile of the range, or only those of replicas that
+// keep their lists this step (FS_STAYING).
+enum ForceSel { FS_ALL = 0, FS_STAYING = 1 };
+void launch_force(const Launch& l, int t0, int nt, int block, int terms, int kick, int mode, int sel = FS_ALL);
+// All terms over the tiles of the replicas that rebuilt their lists this step
+// (the rebuild phases' tile list), a workgroup per group of tiles.
+void launch_force_listed(const Launch& l, int kick, int mode);
 void launch_boundary(const Launch& l, int mode);
 void launch_control(const Launch& l, int op, int64_t arg);
 // Radius of gyration of each of a replica's chains.

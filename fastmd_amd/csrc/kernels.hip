@@ -1680,247 +1680,264 @@ __device__ __forceinline__ float angle_atan2(float y, float x) {
 // (src/integrate/langevin.cu:109-130).  Writes force[i] = (F, pe_i) and 13
 // FP64 partials per wave: pe, virial[6], kinetic tensor[6] (after the kick).
 // ---------------------------------------------------------------------------
-template <int TERMS>
+template <int TERMS, bool LIST = false>
 __global__ void __launch_bounds__(kMaxBlockK) __attribute__((amdgpu_waves_per_eu(6, 8)))
-k_force(const RepDesc* __restrict__ descs,
-                                               const Tile* __restrict__ tiles, int t0, int ntl,
-                                               int kick, int mode) {
+k_force(const RepDesc* __restrict__ descs, const Tile* __restrict__ tiles, int t0, int ntl, int kick, int mode,
+        int sel, const int* __restrict__ rlist, const int* __restrict__ nrt) {
     constexpr int terms = TERMS;
     __shared__ int4 s_pos[(TERMS & 1) ? kGroupStage : 1];  // the group's neighbor set
-    // t0 and ntl are whole groups and a workgroup (1, 2 or 4 waves) lies in one
-    // group, so the returns below are uniform over the workgroup
-    const int w = t0 + xcd_wave_in_grid();
-    if (w >= t0 + ntl) return;
-    const Tile t = tiles[w];
-    const RepDesc& d = descs[t.rep];
-    RepState* st = d.st;
-    if (mode == FM_STEP && !st->active) return;
-    const int lane = lane_id();
-    // this lane computes the atom whose row the list build placed in its column
-    const int grp = t.lt / kGroupTiles;
-    const int i = grp * kGroupSlots + d.gperm[t.lt * kTile + lane];
-    // stage the group's neighbor set (coalesced slot list, one 16-byte position each)
-    const int nst = (terms & 1) ? d.gcount[grp] : -1;
-    if (nst >= 0) {
-        const int nth = blockDim.x, last = nst - 1;
-        const FMD_GLOBAL int4* gp = gview(d.ipos);
-        auto stage = [&](auto gs) {  // four slots, then four positions, in flight at once
-            for (int q = threadIdx.x; q < nst; q += 4 * nth) {
-                const int qb = q + nth, qc = q + 2 * nth, qd = q + 3 * nth;
-                const int sa = gs[q], sb = gs[min(qb, last)], sc = gs[min(qc, last)], sd = gs[min(qd, last)];
-                const int4 va = gp[sa], vb = gp[sb], vc = gp[sc], vd = gp[sd];
-                s_pos[q] = va;
-                if (qb < nst) s_pos[qb] = vb;
-                if (qc < nst) s_pos[qc] = vc;
-                if (qd < nst) s_pos[qd] = vd;
-            }
-        };
-        if (d.idx16) stage((const FMD_GLOBAL unsigned short*)gview(d.gstage) + (size_t)grp * kGroupStage);
-        else stage(gview(d.gstage) + (size_t)grp * kGroupStage);
-        __syncthreads();
-    }
-    const int4* sp = nst >= 0 ? s_pos : nullptr;
-    float fx = 0.f, fy = 0.f, fz = 0.f, pe = 0.f;
-    float w0 = 0.f, w1 = 0.f, w2 = 0.f, w3 = 0.f, w4 = 0.f, w5 = 0.f;
-    // Langevin steps that record nothing need none of the FP64 partials (no
-    // thermostat reads the kinetic energy; the host reevaluates unobserved
-    // steps on request), and neither does a forces-only launch
-    const bool partials = mode == FM_OBS || (mode == FM_STEP && (d.ensemble != 0 || st->want_obs));
-    double k0 = 0, k1 = 0, k2 = 0, k3 = 0, k4 = 0, k5 = 0;
-    double vkey = 0.0, fkey = 0.0;  // max_key of |v|^2 and |F|^2 (src/analysis/thermo.cu:26-75)
-    int err = 0;
-    if (i < d.natoms) {
-        const int4 pi = d.ipos[i];
-        const int ti = pi.w;
-        const float L = st->Lp;
-        const float ulen = L * 2.3283064365386963e-10f;  // fixed-point step, L * 2^-32
-        const bool obs = (mode == FM_OBS) || (mode == FM_STEP && st->want_obs);
-        if (terms & 1) {
-            LJSum a;
-            if (sp) {  // the array itself: its LDS address is a constant of the kernel
-                if (d.idx16) a = lj_variants<true, true>(d, t, lane, i, pi, L, obs, s_pos);
-                else a = lj_variants<false, true>(d, t, lane, i, pi, L, obs, s_pos);
-            } else {
-                if (d.idx16) a = lj_variants<true, false>(d, t, lane, i, pi, L, obs, sp);
-                else a = lj_variants<false, false>(d, t, lane, i, pi, L, obs, sp);
-            }
-            fx = a.fx; fy = a.fy; fz = a.fz; pe = a.pe;
-            w0 = a.w0; w1 = a.w1; w2 = a.w2; w3 = a.w3; w4 = a.w4; w5 = a.w5;
-            if (d.ntables) {
-                const LJSum b = d.idx16 ? table_loop<true>(d, t.lt, lane, i, pi, L, sp)
-                                        : table_loop<false>(d, t.lt, lane, i, pi, L, sp);
-                fx += b.fx; fy += b.fy; fz += b.fz; pe += b.pe;
-                w0 += b.w0; w1 += b.w1; w2 += b.w2; w3 += b.w3; w4 += b.w4; w5 += b.w5;
-            }
+    // One tile per wave.  t0 and ntl are whole groups and a workgroup (1, 2 or
+    // 4 waves) lies in one group, so the returns below are uniform over the
+    // workgroup.
+    auto tile = [&](int w) __attribute__((always_inline)) {
+        const Tile t = tiles[w];
+        const RepDesc& d = descs[t.rep];
+        RepState* st = d.st;
+        if (mode == FM_STEP && !st->active) return;
+        if (sel == FS_STAYING && st->rebuild) return;
+        const int lane = lane_id();
+        // this lane computes the atom whose row the list build placed in its column
+        const int grp = t.lt / kGroupTiles;
+        const int i = grp * kGroupSlots + d.gperm[t.lt * kTile + lane];
+        // stage the group's neighbor set (coalesced slot list, one 16-byte position each)
+        const int nst = (terms & 1) ? d.gcount[grp] : -1;
+        if (nst >= 0) {
+            const int nth = blockDim.x, last = nst - 1;
+            const FMD_GLOBAL int4* gp = gview(d.ipos);
+            auto stage = [&](auto gs) {  // four slots, then four positions, in flight at once
+                for (int q = threadIdx.x; q < nst; q += 4 * nth) {
+                    const int qb = q + nth, qc = q + 2 * nth, qd = q + 3 * nth;
+                    const int sa = gs[q], sb = gs[min(qb, last)], sc = gs[min(qc, last)], sd = gs[min(qd, last)];
+                    const int4 va = gp[sa], vb = gp[sb], vc = gp[sc], vd = gp[sd];
+                    s_pos[q] = va;
+                    if (qb < nst) s_pos[qb] = vb;
+                    if (qc < nst) s_pos[qc] = vc;
+                    if (qd < nst) s_pos[qd] = vd;
+                }
+            };
+            if (d.idx16) stage((const FMD_GLOBAL unsigned short*)gview(d.gstage) + (size_t)grp * kGroupStage);
+            else stage(gview(d.gstage) + (size_t)grp * kGroupStage);
+            __syncthreads();
         }
-        const int hdr = (terms & 6) ? d.bhdr[i] : 0;
-        // a bonded entry of a staged group is the partner's LDS byte offset in
-        // the group's neighbor set (read from the set, or through its slot
-        // list when this variant did not stage it); otherwise a slot
-        const bool bstaged = (terms & 6) && (sp || d.gcount[grp] >= 0);
-        const FMD_GLOBAL unsigned short* gs16 =
-            (const FMD_GLOBAL unsigned short*)gview(d.gstage) + (size_t)grp * kGroupStage;
-        const FMD_GLOBAL int* gs32 = gview(d.gstage) + (size_t)grp * kGroupStage;
-        auto partner = [&](int e) __attribute__((always_inline)) -> int4 {
-            const int v = e & kSlotMask;
-            if (sp) return *(const int4*)((const char*)s_pos + v);
-            if (bstaged) return d.ipos[d.idx16 ? (int)gs16[v >> 4] : gs32[v >> 4]];
-            return d.ipos[v];
-        };
-        // The bonded terms in the order bonds, then angles, each over its rows;
-        // the entries and partner positions of the first kBondAhead bonds and
-        // kAngleAhead angles (all of a Kremer-Grest bead's) are requested
-        // together, so their latencies overlap instead of forming one chain per
-        // term.
-        const int nb = (terms & 2) ? (hdr & 0xFF) : 0;
-        const int na = (terms & 4) ? ((hdr >> 8) & 0xFF) : 0;
-        const int* bt = d.bnd + (size_t)(i / kTile) * d.bcap * kTile + (i % kTile);
-        const int2* at = d.ang + (size_t)(i / kTile) * d.acap * kTile + (i % kTile);
-        auto bond_term = [&](int e, int4 pj) __attribute__((always_inline)) {
-            const float4 bp = d.bparm[(unsigned)e >> kSlotBits];
-            const float dx = fix_diff(pi.x, pj.x, ulen);
-            const float dy = fix_diff(pi.y, pj.y, ulen);
-            const float dz = fix_diff(pi.z, pj.z, ulen);
-            float r2 = sq3(dx, dy, dz);
-            const float R02 = bp.y * bp.y;
-            // a bond at or beyond R0 is a failure of the run, except during the
-            // capped soft push-off, where the clamp below keeps it finite
-            if (!(r2 < R02) && !(d.fcap > 0.0f)) err |= ERR_STRETCH;
-            r2 = fminf(fmaxf(r2, 1e-6f), R02 * 0.9999f);  // src/force/fene.cu:33
-            // divisions as hardware reciprocals (1 ulp), as the reference's
-            // fast-math build does; PE and virial only where observed
-            float ff = -bp.x * R02 * __builtin_amdgcn_rcpf(R02 - r2);
-            const float sig2 = bp.w * bp.w;
-            const bool wca = r2 < sig2 * 1.2599210498948732f;
-            float sr6 = 0.0f, sr12 = 0.0f;
-            if (wca) {
-                const float r2inv = __builtin_amdgcn_rcpf(r2);
-                const float sr2 = sig2 * r2inv;
-                sr6 = sr2 * sr2 * sr2;
-                sr12 = sr6 * sr6;
-                ff += 24.0f * bp.z * r2inv * (2.0f * sr12 - sr6);
+        const int4* sp = nst >= 0 ? s_pos : nullptr;
+        float fx = 0.f, fy = 0.f, fz = 0.f, pe = 0.f;
+        float w0 = 0.f, w1 = 0.f, w2 = 0.f, w3 = 0.f, w4 = 0.f, w5 = 0.f;
+        // Langevin steps that record nothing need none of the FP64 partials (no
+        // thermostat reads the kinetic energy; the host reevaluates unobserved
+        // steps on request), and neither does a forces-only launch
+        const bool partials = mode == FM_OBS || (mode == FM_STEP && (d.ensemble != 0 || st->want_obs));
+        double k0 = 0, k1 = 0, k2 = 0, k3 = 0, k4 = 0, k5 = 0;
+        double vkey = 0.0, fkey = 0.0;  // max_key of |v|^2 and |F|^2 (src/analysis/thermo.cu:26-75)
+        int err = 0;
+        if (i < d.natoms) {
+            const int4 pi = d.ipos[i];
+            const int ti = pi.w;
+            const float L = st->Lp;
+            const float ulen = L * 2.3283064365386963e-10f;  // fixed-point step, L * 2^-32
+            const bool obs = (mode == FM_OBS) || (mode == FM_STEP && st->want_obs);
+            if (terms & 1) {
+                LJSum a;
+                if (sp) {  // the array itself: its LDS address is a constant of the kernel
+                    if (d.idx16) a = lj_variants<true, true>(d, t, lane, i, pi, L, obs, s_pos);
+                    else a = lj_variants<false, true>(d, t, lane, i, pi, L, obs, s_pos);
+                } else {
+                    if (d.idx16) a = lj_variants<true, false>(d, t, lane, i, pi, L, obs, sp);
+                    else a = lj_variants<false, false>(d, t, lane, i, pi, L, obs, sp);
+                }
+                fx = a.fx; fy = a.fy; fz = a.fz; pe = a.pe;
+                w0 = a.w0; w1 = a.w1; w2 = a.w2; w3 = a.w3; w4 = a.w4; w5 = a.w5;
+                if (d.ntables) {
+                    const LJSum b = d.idx16 ? table_loop<true>(d, t.lt, lane, i, pi, L, sp)
+                                            : table_loop<false>(d, t.lt, lane, i, pi, L, sp);
+                    fx += b.fx; fy += b.fy; fz += b.fz; pe += b.pe;
+                    w0 += b.w0; w1 += b.w1; w2 += b.w2; w3 += b.w3; w4 += b.w4; w5 += b.w5;
+                }
             }
-            fx += ff * dx; fy += ff * dy; fz += ff * dz;
-            if (obs) {
-                float bpe = -0.5f * bp.x * R02 * logf(1.0f - r2 / R02);
-                if (wca) bpe += 4.0f * bp.z * (sr12 - sr6) + bp.z;
-                const float hx = 0.5f * ff * dx, hy = 0.5f * ff * dy, hz = 0.5f * ff * dz;
-                pe += 0.5f * bpe;
-                w0 += hx * dx; w1 += hx * dy; w2 += hx * dz;
-                w3 += hy * dy; w4 += hy * dz; w5 += hz * dz;
+            const int hdr = (terms & 6) ? d.bhdr[i] : 0;
+            // a bonded entry of a staged group is the partner's LDS byte offset in
+            // the group's neighbor set (read from the set, or through its slot
+            // list when this variant did not stage it); otherwise a slot
+            const bool bstaged = (terms & 6) && (sp || d.gcount[grp] >= 0);
+            const FMD_GLOBAL unsigned short* gs16 =
+                (const FMD_GLOBAL unsigned short*)gview(d.gstage) + (size_t)grp * kGroupStage;
+            const FMD_GLOBAL int* gs32 = gview(d.gstage) + (size_t)grp * kGroupStage;
+            auto partner = [&](int e) __attribute__((always_inline)) -> int4 {
+                const int v = e & kSlotMask;
+                if (sp) return *(const int4*)((const char*)s_pos + v);
+                if (bstaged) return d.ipos[d.idx16 ? (int)gs16[v >> 4] : gs32[v >> 4]];
+                return d.ipos[v];
+            };
+            // The bonded terms in the order bonds, then angles, each over its rows;
+            // the entries and partner positions of the first kBondAhead bonds and
+            // kAngleAhead angles (all of a Kremer-Grest bead's) are requested
+            // together, so their latencies overlap instead of forming one chain per
+            // term.
+            const int nb = (terms & 2) ? (hdr & 0xFF) : 0;
+            const int na = (terms & 4) ? ((hdr >> 8) & 0xFF) : 0;
+            const int* bt = d.bnd + (size_t)(i / kTile) * d.bcap * kTile + (i % kTile);
+            const int2* at = d.ang + (size_t)(i / kTile) * d.acap * kTile + (i % kTile);
+            auto bond_term = [&](int e, int4 pj) __attribute__((always_inline)) {
+                const float4 bp = d.bparm[(unsigned)e >> kSlotBits];
+                const float dx = fix_diff(pi.x, pj.x, ulen);
+                const float dy = fix_diff(pi.y, pj.y, ulen);
+                const float dz = fix_diff(pi.z, pj.z, ulen);
+                float r2 = sq3(dx, dy, dz);
+                const float R02 = bp.y * bp.y;
+                // a bond at or beyond R0 is a failure of the run, except during the
+                // capped soft push-off, where the clamp below keeps it finite
+                if (!(r2 < R02) && !(d.fcap > 0.0f)) err |= ERR_STRETCH;
+                r2 = fminf(fmaxf(r2, 1e-6f), R02 * 0.9999f);  // src/force/fene.cu:33
+                // divisions as hardware reciprocals (1 ulp), as the reference's
+                // fast-math build does; PE and virial only where observed
+                float ff = -bp.x * R02 * __builtin_amdgcn_rcpf(R02 - r2);
+                const float sig2 = bp.w * bp.w;
+                const bool wca = r2 < sig2 * 1.2599210498948732f;
+                float sr6 = 0.0f, sr12 = 0.0f;
+                if (wca) {
+                    const float r2inv = __builtin_amdgcn_rcpf(r2);
+                    const float sr2 = sig2 * r2inv;
+                    sr6 = sr2 * sr2 * sr2;
+                    sr12 = sr6 * sr6;
+                    ff += 24.0f * bp.z * r2inv * (2.0f * sr12 - sr6);
+                }
+                fx += ff * dx; fy += ff * dy; fz += ff * dz;
+                if (obs) {
+                    float bpe = -0.5f * bp.x * R02 * logf(1.0f - r2 / R02);
+                    if (wca) bpe += 4.0f * bp.z * (sr12 - sr6) + bp.z;
+                    const float hx = 0.5f * ff * dx, hy = 0.5f * ff * dy, hz = 0.5f * ff * dz;
+                    pe += 0.5f * bpe;
+                    w0 += hx * dx; w1 += hx * dy; w2 += hx * dz;
+                    w3 += hy * dy; w4 += hy * dz; w5 += hz * dz;
+                }
+            };
+            auto angle_term = [&](int2 e, int4 q1, int4 q2) __attribute__((always_inline)) {
+                const int role = (unsigned)e.x >> kSlotBits;
+                const float2 ap = d.aparm[(unsigned)e.y >> kSlotBits];
+                // angle atoms (A, B=center, C) from this atom's role
+                // (assignments, not conditionals of two lvalues: a select of
+                // addresses would keep pi in scratch memory)
+                int4 pa = q1, pb = q1, pc = q2;
+                if (role == 0) pa = pi;
+                if (role == 1) pb = pi;
+                else if (role == 2) pb = q2;
+                if (role == 2) pc = pi;
+                const float ax = fix_diff(pa.x, pb.x, ulen);
+                const float ay = fix_diff(pa.y, pb.y, ulen);
+                const float az = fix_diff(pa.z, pb.z, ulen);
+                const float bx = fix_diff(pc.x, pb.x, ulen);
+                const float by = fix_diff(pc.y, pb.y, ulen);
+                const float bz = fix_diff(pc.z, pb.z, ulen);
+                const float ra2 = ax * ax + ay * ay + az * az;
+                const float rb2 = bx * bx + by * by + bz * bz;
+                const float cxp = ay * bz - az * by, cyp = az * bx - ax * bz, czp = ax * by - ay * bx;
+                const float cr = sqrtf(cxp * cxp + cyp * cyp + czp * czp);
+                const float dot = ax * bx + ay * by + az * bz;
+                const float theta = angle_atan2(cr, dot);
+                const float inv_ab = rsqrtf(ra2) * rsqrtf(rb2);
+                const float c = dot * inv_ab;
+                const float s = fmaxf(cr * inv_ab, 1e-6f);
+                const float dth = theta - ap.y;
+                const float pref = 2.0f * ap.x * dth * __builtin_amdgcn_rcpf(s);
+                const float ia2 = __builtin_amdgcn_rcpf(ra2), ib2 = __builtin_amdgcn_rcpf(rb2);
+                const float fix = pref * (bx * inv_ab - c * ax * ia2);
+                const float fiy = pref * (by * inv_ab - c * ay * ia2);
+                const float fiz = pref * (bz * inv_ab - c * az * ia2);
+                const float fkx = pref * (ax * inv_ab - c * bx * ib2);
+                const float fky = pref * (ay * inv_ab - c * by * ib2);
+                const float fkz = pref * (az * inv_ab - c * bz * ib2);
+                if (role == 0) { fx += fix; fy += fiy; fz += fiz; }
+                else if (role == 2) { fx += fkx; fy += fky; fz += fkz; }
+                else { fx -= fix + fkx; fy -= fiy + fky; fz -= fiz + fkz; }
+                if (obs) {
+                    const float third = 1.0f / 3.0f;
+                    pe += ap.x * dth * dth * third;
+                    w0 += (fix * ax + fkx * bx) * third;
+                    w1 += (fix * ay + fkx * by) * third;
+                    w2 += (fix * az + fkx * bz) * third;
+                    w3 += (fiy * ay + fky * by) * third;
+                    w4 += (fiy * az + fky * bz) * third;
+                    w5 += (fiz * az + fkz * bz) * third;
+                }
+            };
+            constexpr int kBondAhead = 2, kAngleAhead = 3;
+            const int eb0 = (nb > 0) ? bt[0] : 0, eb1 = (nb > 1) ? bt[kTile] : 0;
+            const int2 ea0 = (na > 0) ? at[0] : make_int2(0, 0), ea1 = (na > 1) ? at[kTile] : make_int2(0, 0),
+                       ea2 = (na > 2) ? at[2 * kTile] : make_int2(0, 0);
+            int4 pb0 = pi, pb1 = pi, qa0 = pi, qc0 = pi, qa1 = pi, qc1 = pi, qa2 = pi, qc2 = pi;
+            if (nb > 0) pb0 = partner(eb0);
+            if (nb > 1) pb1 = partner(eb1);
+            if (na > 0) { qa0 = partner(ea0.x); qc0 = partner(ea0.y); }
+            if (na > 1) { qa1 = partner(ea1.x); qc1 = partner(ea1.y); }
+            if (na > 2) { qa2 = partner(ea2.x); qc2 = partner(ea2.y); }
+            if (nb > 0) bond_term(eb0, pb0);
+            if (nb > 1) bond_term(eb1, pb1);
+            for (int b = kBondAhead; b < nb; b++) {
+                const int e = bt[b * kTile];
+                bond_term(e, partner(e));
             }
-        };
-        auto angle_term = [&](int2 e, int4 q1, int4 q2) __attribute__((always_inline)) {
-            const int role = (unsigned)e.x >> kSlotBits;
-            const float2 ap = d.aparm[(unsigned)e.y >> kSlotBits];
-            // angle atoms (A, B=center, C) from this atom's role
-            // (assignments, not conditionals of two lvalues: a select of
-            // addresses would keep pi in scratch memory)
-            int4 pa = q1, pb = q1, pc = q2;
-            if (role == 0) pa = pi;
-            if (role == 1) pb = pi;
-            else if (role == 2) pb = q2;
-            if (role == 2) pc = pi;
-            const float ax = fix_diff(pa.x, pb.x, ulen);
-            const float ay = fix_diff(pa.y, pb.y, ulen);
-            const float az = fix_diff(pa.z, pb.z, ulen);
-            const float bx = fix_diff(pc.x, pb.x, ulen);
-            const float by = fix_diff(pc.y, pb.y, ulen);
-            const float bz = fix_diff(pc.z, pb.z, ulen);
-            const float ra2 = ax * ax + ay * ay + az * az;
-            const float rb2 = bx * bx + by * by + bz * bz;
-            const float cxp = ay * bz - az * by, cyp = az * bx - ax * bz, czp = ax * by - ay * bx;
-            const float cr = sqrtf(cxp * cxp + cyp * cyp + czp * czp);
-            const float dot = ax * bx + ay * by + az * bz;
-            const float theta = angle_atan2(cr, dot);
-            const float inv_ab = rsqrtf(ra2) * rsqrtf(rb2);
-            const float c = dot * inv_ab;
-            const float s = fmaxf(cr * inv_ab, 1e-6f);
-            const float dth = theta - ap.y;
-            const float pref = 2.0f * ap.x * dth * __builtin_amdgcn_rcpf(s);
-            const float ia2 = __builtin_amdgcn_rcpf(ra2), ib2 = __builtin_amdgcn_rcpf(rb2);
-            const float fix = pref * (bx * inv_ab - c * ax * ia2);
-            const float fiy = pref * (by * inv_ab - c * ay * ia2);
-            const float fiz = pref * (bz * inv_ab - c * az * ia2);
-            const float fkx = pref * (ax * inv_ab - c * bx * ib2);
-            const float fky = pref * (ay * inv_ab - c * by * ib2);
-            const float fkz = pref * (az * inv_ab - c * bz * ib2);
-            if (role == 0) { fx += fix; fy += fiy; fz += fiz; }
-            else if (role == 2) { fx += fkx; fy += fky; fz += fkz; }
-            else { fx -= fix + fkx; fy -= fiy + fky; fz -= fiz + fkz; }
-            if (obs) {
-                const float third = 1.0f / 3.0f;
-                pe += ap.x * dth * dth * third;
-                w0 += (fix * ax + fkx * bx) * third;
-                w1 += (fix * ay + fkx * by) * third;
-                w2 += (fix * az + fkx * bz) * third;
-                w3 += (fiy * ay + fky * by) * third;
-                w4 += (fiy * az + fky * bz) * third;
-                w5 += (fiz * az + fkz * bz) * third;
+            if (na > 0) angle_term(ea0, qa0, qc0);
+            if (na > 1) angle_term(ea1, qa1, qc1);
+            if (na > 2) angle_term(ea2, qa2, qc2);
+            for (int a = kAngleAhead; a < na; a++) {
+                const int2 e = at[a * kTile];
+                angle_term(e, partner(e.x), partner(e.y));
             }
-        };
-        constexpr int kBondAhead = 2, kAngleAhead = 3;
-        const int eb0 = (nb > 0) ? bt[0] : 0, eb1 = (nb > 1) ? bt[kTile] : 0;
-        const int2 ea0 = (na > 0) ? at[0] : make_int2(0, 0), ea1 = (na > 1) ? at[kTile] : make_int2(0, 0),
-                   ea2 = (na > 2) ? at[2 * kTile] : make_int2(0, 0);
-        int4 pb0 = pi, pb1 = pi, qa0 = pi, qc0 = pi, qa1 = pi, qc1 = pi, qa2 = pi, qc2 = pi;
-        if (nb > 0) pb0 = partner(eb0);
-        if (nb > 1) pb1 = partner(eb1);
-        if (na > 0) { qa0 = partner(ea0.x); qc0 = partner(ea0.y); }
-        if (na > 1) { qa1 = partner(ea1.x); qc1 = partner(ea1.y); }
-        if (na > 2) { qa2 = partner(ea2.x); qc2 = partner(ea2.y); }
-        if (nb > 0) bond_term(eb0, pb0);
-        if (nb > 1) bond_term(eb1, pb1);
-        for (int b = kBondAhead; b < nb; b++) {
-            const int e = bt[b * kTile];
-            bond_term(e, partner(e));
+            // Langevin steps followed by another step of the run also give that
+            // step's pre-force half kick (the same force, the same operations as
+            // k_pre), so the force need not be stored and read back; the kinetic
+            // partials are taken between the two kicks
+            const bool ahead = kick && mode == FM_STEP && d.ensemble == 0 && st->step + 1 < st->stop;
+            if (!ahead) d.force[i] = make_float4(fx, fy, fz, pe);
+            float4 v = d.vel[i];
+            const float m = d.mass[ti];
+            const float hdtm = d.half_dt / m;
+            if (kick) { v.x = fmaf(hdtm, fx, v.x); v.y = fmaf(hdtm, fy, v.y); v.z = fmaf(hdtm, fz, v.z); }
+            if (partials) {
+                const double md = m, vx = v.x, vy = v.y, vz = v.z;
+                k0 = md * vx * vx; k1 = md * vx * vy; k2 = md * vx * vz;
+                k3 = md * vy * vy; k4 = md * vy * vz; k5 = md * vz * vz;
+                const int g = d.gid[i];
+                vkey = max_key(v.x * v.x + v.y * v.y + v.z * v.z, g);
+                fkey = max_key(fx * fx + fy * fy + fz * fz, g);
+            }
+            if (ahead) {  // bit for bit the kick of k_pre (explicit FMAs there too)
+                v.x = fmaf(hdtm, fx, v.x); v.y = fmaf(hdtm, fy, v.y); v.z = fmaf(hdtm, fz, v.z);
+                if (d.vlim > 0.0f) v = limit_speed(v, d.vlim);
+            }
+            if (kick) d.vel[i] = v;
         }
-        if (na > 0) angle_term(ea0, qa0, qc0);
-        if (na > 1) angle_term(ea1, qa1, qc1);
-        if (na > 2) angle_term(ea2, qa2, qc2);
-        for (int a = kAngleAhead; a < na; a++) {
-            const int2 e = at[a * kTile];
-            angle_term(e, partner(e.x), partner(e.y));
-        }
-        // Langevin steps followed by another step of the run also give that
-        // step's pre-force half kick (the same force, the same operations as
-        // k_pre), so the force need not be stored and read back; the kinetic
-        // partials are taken between the two kicks
-        const bool ahead = kick && mode == FM_STEP && d.ensemble == 0 && st->step + 1 < st->stop;
-        if (!ahead) d.force[i] = make_float4(fx, fy, fz, pe);
-        float4 v = d.vel[i];
-        const float m = d.mass[ti];
-        const float hdtm = d.half_dt / m;
-        if (kick) { v.x = fmaf(hdtm, fx, v.x); v.y = fmaf(hdtm, fy, v.y); v.z = fmaf(hdtm, fz, v.z); }
         if (partials) {
-            const double md = m, vx = v.x, vy = v.y, vz = v.z;
-            k0 = md * vx * vx; k1 = md * vx * vy; k2 = md * vx * vz;
-            k3 = md * vy * vy; k4 = md * vy * vz; k5 = md * vz * vz;
-            const int g = d.gid[i];
-            vkey = max_key(v.x * v.x + v.y * v.y + v.z * v.z, g);
-            fkey = max_key(fx * fx + fy * fy + fz * fz, g);
-        }
-        if (ahead) {  // bit for bit the kick of k_pre (explicit FMAs there too)
-            v.x = fmaf(hdtm, fx, v.x); v.y = fmaf(hdtm, fy, v.y); v.z = fmaf(hdtm, fz, v.z);
-            if (d.vlim > 0.0f) v = limit_speed(v, d.vlim);
-        }
-        if (kick) d.vel[i] = v;
-    }
-    if (partials) {
-        double part[kNumSum] = {(double)pe, (double)w0, (double)w1, (double)w2, (double)w3, (double)w4,
-                                (double)w5, k0, k1, k2, k3, k4, k5};
+            double part[kNumSum] = {(double)pe, (double)w0, (double)w1, (double)w2, (double)w3, (double)w4,
+                                    (double)w5, k0, k1, k2, k3, k4, k5};
 #pragma unroll
-        for (int c = 0; c < kNumSum; c++) part[c] = wave_sum_last(part[c]);
-        vkey = wave_max_last(vkey);
-        fkey = wave_max_last(fkey);
-        if (lane == 63) {
+            for (int c = 0; c < kNumSum; c++) part[c] = wave_sum_last(part[c]);
+            vkey = wave_max_last(vkey);
+            fkey = wave_max_last(fkey);
+            if (lane == 63) {
 #pragma unroll
-            for (int c = 0; c < kNumSum; c++) d.wpart[(size_t)t.lt * kNumPart + c] = part[c];
-            d.wpart[(size_t)t.lt * kNumPart + kNumSum] = vkey;
-            d.wpart[(size_t)t.lt * kNumPart + kNumSum + 1] = fkey;
+                for (int c = 0; c < kNumSum; c++) d.wpart[(size_t)t.lt * kNumPart + c] = part[c];
+                d.wpart[(size_t)t.lt * kNumPart + kNumSum] = vkey;
+                d.wpart[(size_t)t.lt * kNumPart + kNumSum + 1] = fkey;
+            }
         }
+        if (err) atomicOr(&st->err, err);  // failures only
+    };
+    if constexpr (LIST) {
+        // the listed tiles (the rebuild phases' list): each workgroup (four
+        // waves: one group) takes listed groups in turn, the workgroups of one
+        // XCD (a multiple of kXcds in the grid) from one contiguous eighth of
+        // the list, so a replica's positions are gathered through one L2
+        const int ng = *nrt / kGroupTiles, per = (ng + kXcds - 1) / kXcds, x = blockIdx.x % kXcds;
+        const int g1 = min((x + 1) * per, ng), gs = gridDim.x / kXcds;
+        for (int g = x * per + blockIdx.x / kXcds; g < g1; g += gs) {
+            tile(rlist[g * kGroupTiles + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6)]);
+            __syncthreads();  // the next group's staging reuses s_pos
+        }
+    } else {
+        // the grid's wave w takes tile t0 + w
+        const int w = t0 + xcd_wave_in_grid();
+        if (w < t0 + ntl) tile(w);
     }
-    if (err) atomicOr(&st->err, err);  // failures only
 }
 
 // First reduction level: one workgroup sums the FP64 partials of kChunkTiles
@@ -2209,7 +2226,11 @@ __global__ void k_morton_keys(const RepDesc* __restrict__ dp, int natoms, uint32
 // ---------------------------------------------------------------------------
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 static constexpr int kStrideBlocks = 4096;  // grid cap of the grid-stride rebuild kernels
-static constexpr int kBuildBlocks = 8192;   // grid cap of the list-build workgroups (one group each)
+// grid cap of the list-build workgroups (a group per pass): the four per CU
+// that fit at once on 256 CUs, so none waits to start only to find no group
+static constexpr int kBuildBlocks = 1024;
+static constexpr int kForceListBlocks = 1280;  // workgroups of a listed force pass (five per CU; a multiple of kXcds)
+static_assert(kForceListBlocks % kXcds == 0, "the listed force pass splits its grid evenly over the XCDs");
 
 void launch_pre(const Launch& l) {
     if (l.ntl <= 0) return;
@@ -2257,27 +2278,39 @@ void launch_rebuild(const Launch& l, int always, int build) {
     k_scatter<<<g, 256, 0, l.stream>>>(l.descs, l.tiles, l.rlist, l.nrt);
     k_rank<<<g, 256, 0, l.stream>>>(l.descs, l.tiles, l.rlist, l.nrt);
     k_gather<<<g, 256, 0, l.stream>>>(l.descs, l.tiles, l.rlist, l.nrt);
-    if (build) {
-        int gb = cdiv(l.ntl, kGroupTiles);
-        if (gb > kBuildBlocks) gb = kBuildBlocks;
-        k_build<<<gb, kGroupSlots, 0, l.stream>>>(l.descs, l.tiles, l.rlist, l.nrt);
-    }
+    if (build) launch_build(l);
 }
-void launch_force(const Launch& l, int t0, int nt, int block, int terms, int kick, int mode) {
+void launch_build(const Launch& l) {
+    if (l.ntl <= 0) return;
+    int gb = cdiv(l.ntl, kGroupTiles);
+    if (gb > kBuildBlocks) gb = kBuildBlocks;
+    k_build<<<gb, kGroupSlots, 0, l.stream>>>(l.descs, l.tiles, l.rlist, l.nrt);
+}
+static void force_kernel(const Launch& l, int g, int block, int t0, int nt, int terms, int kick, int mode,
+                         int sel) {
+    const size_t lds = 0;
+#define FMD_FORCE_ARGS l.descs, l.tiles, t0, nt, kick, mode, sel, l.rlist, l.nrt
+    switch (terms & 7) {
+        case 1: k_force<1><<<g, block, lds, l.stream>>>(FMD_FORCE_ARGS); break;
+        case 2: k_force<2><<<g, block, lds, l.stream>>>(FMD_FORCE_ARGS); break;
+        case 3: k_force<3><<<g, block, lds, l.stream>>>(FMD_FORCE_ARGS); break;
+        case 4: k_force<4><<<g, block, lds, l.stream>>>(FMD_FORCE_ARGS); break;
+        case 5: k_force<5><<<g, block, lds, l.stream>>>(FMD_FORCE_ARGS); break;
+        case 6: k_force<6><<<g, block, lds, l.stream>>>(FMD_FORCE_ARGS); break;
+        case 7: k_force<7><<<g, block, lds, l.stream>>>(FMD_FORCE_ARGS); break;
+        default: k_force<0><<<g, block, lds, l.stream>>>(FMD_FORCE_ARGS); break;
+    }
+#undef FMD_FORCE_ARGS
+}
+void launch_force(const Launch& l, int t0, int nt, int block, int terms, int kick, int mode, int sel) {
     if (nt <= 0) return;
     block = valid_block(block);
-    const int g = cdiv(nt, block / 64);
-    const size_t lds = 0;
-    switch (terms & 7) {
-        case 1: k_force<1><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, mode); break;
-        case 2: k_force<2><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, mode); break;
-        case 3: k_force<3><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, mode); break;
-        case 4: k_force<4><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, mode); break;
-        case 5: k_force<5><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, mode); break;
-        case 6: k_force<6><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, mode); break;
-        case 7: k_force<7><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, mode); break;
-        default: k_force<0><<<g, block, lds, l.stream>>>(l.descs, l.tiles, t0, nt, kick, mode); break;
-    }
+    force_kernel(l, cdiv(nt, block / 64), block, t0, nt, terms, kick, mode, sel);
+}
+void launch_force_listed(const Launch& l, int kick, int mode) {
+    if (l.ntl <= 0) return;
+    k_force<7, true><<<kForceListBlocks, kGroupSlots, 0, l.stream>>>(l.descs, l.tiles, 0, 0, kick, mode, FS_ALL,
+                                                                      l.rlist, l.nrt);
 }
 void launch_boundary(const Launch& l, int mode) {
     if (l.nr <= 0) return;
