@@ -16,8 +16,12 @@ namespace fmd {
 constexpr int kNumClasses = 3;
 constexpr int kRgScratchChains = 1 << 16;  // chains the allocation-free Rg read holds
 constexpr int kGraphSteps = 20;
-// batches of at least this many replicas build lists beside the force pass
+// batches of at least this many replicas, with their list builds spread over
+// at least kForkPhases of the kRebuildEvery steps, build lists beside the
+// force pass
 constexpr int kForkReplicas = 8;
+constexpr int kForkPhases = kRebuildEvery / 2;
+constexpr int64_t kForkRecheck = 50 * kRebuildEvery;  // steps between two choices by a waiting caller
 
 int size_class(int natoms);
 
@@ -46,6 +50,10 @@ struct Batch {
     Tile* d_chunks = nullptr; // (replica, chunk of kChunkTiles tiles) list
     int nch = 0;
     int4* d_status = nullptr; // per replica (err, active, step lo, step hi), d_reps order
+    int* d_phase = nullptr;   // per replica, steps since its last list build (d_reps order)
+    bool fork = false;        // the step builds lists beside the force pass (launch_step_seq)
+    bool fork_known = false;
+    int64_t steps_since_fork = 0;  // steps launched since that choice
     std::vector<int> ord;     // host copy of d_reps
     void* block = nullptr;    // the arrays above, carved from one device block
     size_t block_bytes = 0;
@@ -103,7 +111,7 @@ class Engine {
     // The same on the side stream, completed before it returns (for systems
     // created there while the main stream is busy).
     int32_t begin_run_side(const std::vector<int>& ids, int64_t nsteps);
-    int32_t launch_steps(const std::vector<int>& ids, int64_t k, bool use_graph);
+    int32_t launch_steps(const std::vector<int>& ids, int64_t k, bool use_graph, bool may_sync = false);
     int32_t sync();
     int32_t check_errors(const std::vector<int>& ids, std::vector<int>* failed);
     int32_t replica_error(int32_t sys);
@@ -155,6 +163,7 @@ class Engine {
     Batch* batch_for(const std::vector<int>& ids);
     void drop_batches_with(int id);
     void launch_step_seq(Batch& b);
+    void decide_fork(Batch& b, bool may_sync);
     Launch launch_of(const Batch& b) const;
     Launch launch_of(const Batch& b, hipStream_t s) const;
     int32_t read_state(Replica& r, RepState& s);

@@ -733,7 +733,8 @@ Batch* Engine::batch_for(const std::vector<int>& ids_in) {
     const size_t o_tiles = 0, o_reps = o_tiles + al(tiles.size() * sizeof(Tile)),
                  o_t0 = o_reps + al(ord.size() * 4), o_chunks = o_t0 + al(t0s.size() * 4),
                  o_nrt = o_chunks + al(chunks.size() * sizeof(Tile)), o_rlist = o_nrt + al(4),
-                 o_status = o_rlist + al(tiles.size() * 4), total = o_status + al(ord.size() * 16);
+                 o_status = o_rlist + al(tiles.size() * 4), o_phase = o_status + al(ord.size() * 16),
+                 total = o_phase + al(ord.size() * 4);
     std::vector<char> host(o_rlist, 0);
     std::memcpy(host.data() + o_tiles, tiles.data(), tiles.size() * sizeof(Tile));
     std::memcpy(host.data() + o_reps, ord.data(), ord.size() * 4);
@@ -754,6 +755,7 @@ Batch* Engine::batch_for(const std::vector<int>& ids_in) {
     b->d_nrt = (int*)(blk + o_nrt);
     b->d_rlist = (int*)(blk + o_rlist);
     b->d_status = (int4*)(blk + o_status);
+    b->d_phase = (int*)(blk + o_phase);
     b->ord = ord;
     batches_[ids] = b;
     return b;
@@ -769,6 +771,40 @@ void Engine::free_batch(Batch* b) {
 Launch Engine::launch_of(const Batch& b) const { return launch_of(b, stream_); }
 Launch Engine::launch_of(const Batch& b, hipStream_t s) const {
     return Launch{d_descs_, b.d_tiles, b.ntl, b.d_reps, b.nr, b.d_rep_t0, b.d_rlist, b.d_nrt, b.d_chunks, b.nch, s};
+}
+
+// Whether the batch's steps build lists beside the force pass: with enough
+// replicas whose list builds are spread over enough steps of the rebuild
+// period that most steps have some, read from the device (a wait for the
+// stream) when the caller waits for the steps anyway; a batch in lock step
+// (replicas created together rebuild together every kRebuildEvery steps)
+// would pay for the cross-stream waits on every step.  A caller that keeps
+// the stream busy (the server) forks any batch of enough replicas.
+void Engine::decide_fork(Batch& b, bool may_sync) {
+    b.fork_known = true;
+    b.steps_since_fork = 0;
+    // FMD_STEP_FORK=1 or 0 sets the structure (batches formed from then on)
+    if (const char* v = std::getenv("FMD_STEP_FORK"); v && (v[0] == '0' || v[0] == '1')) {
+        b.fork = v[0] == '1';
+        return;
+    }
+    b.fork = b.nr >= kForkReplicas;
+    if (!b.fork || !may_sync) return;
+    std::vector<int> ph(b.nr);
+    launch_phase(launch_of(b), b.d_phase);
+    if (hipMemcpyAsync(ph.data(), b.d_phase, ph.size() * 4, hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+        hipStreamSynchronize(stream_) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+    }
+    bool seen[kRebuildEvery] = {};
+    int distinct = 0;
+    for (int v : ph)
+        if (v >= 0 && !seen[v % kRebuildEvery]) {
+            seen[v % kRebuildEvery] = true;
+            distinct++;
+        }
+    b.fork = distinct >= kForkPhases;
 }
 
 // One MD step for every replica of the batch.  With many replicas, those that
@@ -795,7 +831,7 @@ void Engine::launch_step_seq(Batch& b) {
             c = e;
         }
     };
-    if (b.nr >= kForkReplicas) {
+    if (b.fork) {
         const Launch la = launch_of(b, aux_);
         (void)hipEventRecord(ev_fork_, stream_);
         (void)hipStreamWaitEvent(aux_, ev_fork_, 0);
@@ -834,7 +870,7 @@ int32_t Engine::begin_run_side(const std::vector<int>& ids, int64_t nsteps) {
     return rc;
 }
 
-int32_t Engine::launch_steps(const std::vector<int>& ids, int64_t k, bool use_graph) {
+int32_t Engine::launch_steps(const std::vector<int>& ids, int64_t k, bool use_graph, bool may_sync) {
     Batch* b = batch_for(ids);
     if (!b) return FMD_ERR_ARG;
     // FMD_NO_GRAPH=1 issues the same kernel sequence without graph capture
@@ -844,9 +880,19 @@ int32_t Engine::launch_steps(const std::vector<int>& ids, int64_t k, bool use_gr
         return v && v[0] == '1';
     }();
     if (no_graph) use_graph = false;
+    // the step structure: chosen with the batch, again with each capture, and
+    // again by a waiting caller every kForkRecheck steps (the replicas' list
+    // builds may have spread out or fallen into step since)
+    if (!b->fork_known || (may_sync && b->steps_since_fork >= kForkRecheck)) {
+        const bool was = b->fork;
+        decide_fork(*b, may_sync);
+        if (b->graph && b->fork != was) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+    }
+    b->steps_since_fork += k;
     if (use_graph) {
         if (!b->graph || b->tune_version != tune_version_) {
             if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+            decide_fork(*b, may_sync);
             hipGraph_t g;
             int32_t rc = hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "BeginCapture");
             if (rc) return rc;
@@ -945,7 +991,7 @@ int32_t Engine::step(const int32_t* ids, int32_t n, int64_t nsteps, bool use_gra
         if (!rep(id)) return fail(FMD_ERR_ARG, "no such system " + std::to_string(id));
     if (nsteps == 0) return FMD_OK;
     if (int32_t rc = begin_run(v, nsteps)) return rc;
-    if (int32_t rc = launch_steps(v, nsteps, use_graph)) return rc;
+    if (int32_t rc = launch_steps(v, nsteps, use_graph, true)) return rc;
     if (int32_t rc = sync()) return rc;
     return check_errors(v, nullptr);
 }

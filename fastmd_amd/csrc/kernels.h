@@ -57,6 +57,8 @@ void launch_nh_chain_run(RepState* st, const double* ke, int n, int carry, doubl
                          double half_dt, double* scales, hipStream_t s);
 // Per replica of the list: (err, active, step lo, step hi).
 void launch_status(const Launch& l, int4* out);
+// Per replica of the list: steps since its last list build (-1: inactive).
+void launch_phase(const Launch& l, int* out);
 // Reference cell ids / Morton keys by global id for one replica.
 void launch_ref_cells(const RepDesc* d, int natoms, int* out, hipStream_t s);
 void launch_morton_keys(const RepDesc* d, int natoms, uint32_t* out, hipStream_t s);

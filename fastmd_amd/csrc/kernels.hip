@@ -449,6 +449,16 @@ __global__ void k_status(const RepDesc* __restrict__ descs, const int* __restric
     out[k] = make_int4(st->err, st->active, (int)(uint32_t)s, (int)(uint32_t)(s >> 32));
 }
 
+// Steps since each listed replica's last list build (-1: inactive), for the
+// host's choice of step structure.
+__global__ void k_phase(const RepDesc* __restrict__ descs, const int* __restrict__ reps, int nr,
+                        int* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nr) return;
+    const RepState* st = descs[reps[k]].st;
+    out[k] = st->active ? st->since_rebuild : -1;
+}
+
 // Host-requested state changes.
 __global__ void k_control(const RepDesc* __restrict__ descs, const int* __restrict__ reps, int nr,
                           int op, int64_t arg) {
@@ -2226,9 +2236,7 @@ __global__ void k_morton_keys(const RepDesc* __restrict__ dp, int natoms, uint32
 // ---------------------------------------------------------------------------
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 static constexpr int kStrideBlocks = 4096;  // grid cap of the grid-stride rebuild kernels
-// grid cap of the list-build workgroups (a group per pass): the four per CU
-// that fit at once on 256 CUs, so none waits to start only to find no group
-static constexpr int kBuildBlocks = 1024;
+static constexpr int kBuildBlocks = 8192;   // grid cap of the list-build workgroups (one group each)
 static constexpr int kForceListBlocks = 1280;  // workgroups of a listed force pass (five per CU; a multiple of kXcds)
 static_assert(kForceListBlocks % kXcds == 0, "the listed force pass splits its grid evenly over the XCDs");
 
@@ -2258,6 +2266,10 @@ void launch_nh_chain_run(RepState* st, const double* ke, int n, int carry, doubl
 void launch_status(const Launch& l, int4* out) {
     if (l.nr <= 0) return;
     k_status<<<cdiv(l.nr, 256), 256, 0, l.stream>>>(l.descs, l.reps, l.nr, out);
+}
+void launch_phase(const Launch& l, int* out) {
+    if (l.nr <= 0) return;
+    k_phase<<<cdiv(l.nr, 256), 256, 0, l.stream>>>(l.descs, l.reps, l.nr, out);
 }
 void launch_control(const Launch& l, int op, int64_t arg) {
     if (l.nr <= 0) return;

@@ -8,6 +8,8 @@ forces, potential energy and virial against the FP64 oracle (1e-5), the
 exact Verlet pair set of a fresh build, that the list the step path carried
 still holds every pair inside rc (the skin trigger), and bit identity of a
 replica alone and in a batch or the server."""
+import os
+
 import numpy as np
 import pytest
 
@@ -104,9 +106,9 @@ def _relaxed_descs(fmd, engine, specs, ensembles=None):
 def test_uneven_replicas_three_classes(fmd, engine, orc, capsys):
     """Replicas of the uneven workload (seed 12345) spanning the size classes
     (< 8K, 8K-32K, >= 32K beads, one of 32K-64K): 100 batched steps, every
-    replica checked against the oracle and bit-identical alone.  Nine replicas
-    relaxed apart rebuild their lists at different steps, so the batch builds
-    some replicas' lists beside the others' force pass."""
+    replica checked against the oracle and bit-identical alone, and the batch
+    bit-identical when it builds the rebuilding replicas' lists beside the
+    others' force pass (relaxed apart, they rebuild at different steps)."""
     from fastmd_amd import workloads
     specs = workloads.uneven_replica_specs(256)
     sizes = [s["nchains"] * s["chain_len"] for s in specs]
@@ -131,6 +133,17 @@ def test_uneven_replicas_three_classes(fmd, engine, orc, capsys):
         for x, y in zip(engine.state(a), states[k]):
             assert np.array_equal(x, y)
         engine.remove(a)
+    os.environ["FMD_STEP_FORK"] = "1"
+    try:
+        sids = [engine.add(x) for x in descs]
+        engine.step(sids, 100)
+        for s, ref in zip(sids, states):
+            for x, y in zip(engine.state(s), ref):
+                assert np.array_equal(x, y)
+        for s in sids:
+            engine.remove(s)
+    finally:
+        del os.environ["FMD_STEP_FORK"]
 
 
 def test_served_job_stream(fmd, engine, orc):
