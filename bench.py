@@ -457,13 +457,13 @@ def main():
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
-                     "kernel": "k_force<1> (LJ over the Verlet list)", "ms_per_launch": ms_lj,
+                     "kernel": "k_force<1, false> (LJ over the Verlet list)", "ms_per_launch": ms_lj,
                      "alg_bytes_per_launch": bytes_lj, "stored_bytes_per_launch": stored_lj,
                      "listed_pairs": pairs,
                      "alg_bytes_rule": "36 B per bead + 4 B per listed pair (rows stored as 16-bit slots)",
-                     "limiter": ("VALU issue: a VALU instruction issues in about 86% of the cycles per CU, "
-                                 "94% counting the quarter-rate reciprocal (23 instructions per pair); "
-                                 "HBM traffic is 1.11 GB per launch" if traffic else None),
+                     "limiter": ("VALU issue: a VALU instruction issues in about 96% of the cycles per CU "
+                                 "(22 instructions per listed pair); HBM traffic is "
+                                 f"{traffic / 1e9:.2f} GB per launch" if traffic else None),
                      "limiter_source": (os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None)},
         "cpu_baseline": cpu,
         "detail": {"plain_bead_steps_per_s": plain, "batched_over_plain": batched / plain,

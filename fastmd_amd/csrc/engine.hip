@@ -778,8 +778,10 @@ Launch Engine::launch_of(const Batch& b, hipStream_t s) const {
 // period that most steps have some, read from the device (a wait for the
 // stream) when the caller waits for the steps anyway; a batch in lock step
 // (replicas created together rebuild together every kRebuildEvery steps)
-// would pay for the cross-stream waits on every step.  A caller that keeps
-// the stream busy (the server) forks any batch of enough replicas.
+// would pay for the cross-stream waits on every step.  The server, which
+// keeps the stream busy and whose resident set changes from launch to launch
+// (mostly launched without a graph), keeps the one-stream step: measured on
+// its job stream, the waits cost it more than the overlap gave.
 void Engine::decide_fork(Batch& b, bool may_sync) {
     b.fork_known = true;
     b.steps_since_fork = 0;
@@ -788,8 +790,8 @@ void Engine::decide_fork(Batch& b, bool may_sync) {
         b.fork = v[0] == '1';
         return;
     }
-    b.fork = b.nr >= kForkReplicas;
-    if (!b.fork || !may_sync) return;
+    b.fork = false;
+    if (b.nr < kForkReplicas || !may_sync) return;
     std::vector<int> ph(b.nr);
     launch_phase(launch_of(b), b.d_phase);
     if (hipMemcpyAsync(ph.data(), b.d_phase, ph.size() * 4, hipMemcpyDeviceToHost, stream_) != hipSuccess ||
