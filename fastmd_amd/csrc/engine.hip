@@ -190,7 +190,7 @@ static Sizing sizing(const fmd_system_desc* d, int bcap, int acap, int kmult = 1
     s.ncell_cap = 1 << (3 * cb);
     s.thermo_cap = d->thermo_freq > 0 ? 4096 : 0;
     uint64_t b = 0;
-    b += (uint64_t)s.nslots * (9 * 16 + 7 * 4);
+    b += (uint64_t)s.nslots * (9 * 16 + 8 * 4);
     b += (uint64_t)s.natoms * 4;
     b += (uint64_t)s.ncell_cap * 8 + 4;
     b += (uint64_t)s.ntiles * 64 * ((uint64_t)s.kcap * 4 + (uint64_t)s.bcap * 4 + (uint64_t)s.acap * 8);
@@ -444,6 +444,7 @@ int32_t Engine::create_impl(const fmd_system_desc* d, int32_t* sys, int kmult, b
     D.tmp_img = (int4*)carve(ns * 16);
     D.cell_of = (int*)carve(ns * 4);
     D.rank = (int*)carve(ns * 4);
+    D.sortx = (int*)carve(ns * 4);
     D.order = (int*)carve(ns * 4);
     D.order2 = (int*)carve(ns * 4);
     D.slot_of = (int*)carve(na * 4);
@@ -1273,7 +1274,7 @@ int32_t Engine::get_rg_noalloc(int32_t sys, double* rg, double* per_chain, int32
 // ---------------------------------------------------------------------------
 namespace {
 constexpr uint32_t kSnapMagic = 0x534D4446u;  // "FDMS"
-constexpr uint32_t kSnapVersion = 2;
+constexpr uint32_t kSnapVersion = 3;
 struct SnapHeader {
     uint32_t magic, version, desc_bytes, state_bytes;
     uint64_t used;
@@ -1286,7 +1287,7 @@ struct SnapHeader {
 template <class F>
 void for_each_ptr(RepDesc& d, F&& f) {
     f(d.pos); f(d.vel); f(d.force); f(d.pos_ref); f(d.ipos); f(d.img); f(d.gid);
-    f(d.tmp_pos); f(d.tmp_vel); f(d.tmp_img); f(d.cell_of); f(d.rank); f(d.order); f(d.order2);
+    f(d.tmp_pos); f(d.tmp_vel); f(d.tmp_img); f(d.cell_of); f(d.rank); f(d.sortx); f(d.order); f(d.order2);
     f(d.slot_of); f(d.cell_cnt); f(d.cell_start); f(d.nnbr); f(d.nbr); f(d.gcount); f(d.tlen); f(d.gstage); f(d.gperm); f(d.bhdr); f(d.bnd); f(d.ang);
     f(d.t_boff); f(d.t_blst); f(d.t_aoff); f(d.t_alst); f(d.t_choff);
     f(d.lj); f(d.bparm); f(d.aparm); f(d.mass);

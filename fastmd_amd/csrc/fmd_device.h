@@ -172,6 +172,7 @@ struct RepDesc {
     int4* tmp_img;
     int* cell_of;
     int* rank;
+    int* sortx;            // [nslots] x grid coordinate relative to the atom's cell (the within-cell sort key)
     int* order;
     int* order2;
     int* slot_of;          // [natoms] global id -> slot

@@ -26,8 +26,9 @@ enum ControlOp { CTL_RUN = 0, CTL_FORCE_REBUILD = 1 };
 
 void launch_pre(const Launch& l);
 void launch_decide(const Launch& l);
-// The rebuild phases (list of rebuilding tiles, Morton cell sort and, if
-// `build`, the Verlet lists); `always` ignores the active/rebuild flags.
+// The rebuild phases (Morton cell sort and, if `build`, the Verlet lists) of
+// the tiles in the rebuild tile list: every tile if `always` (listed here),
+// else those k_decide listed this step (the replicas it found rebuilding).
 void launch_rebuild(const Launch& l, int always, int build);
 // The Verlet lists of the tiles the last rebuild phases listed.
 void launch_build(const Launch& l);

@@ -270,7 +270,8 @@ __device__ inline float4 limit_speed(float4 v, float vlim) {
 //  src/neighbor/skin_trigger.cuh:5-16)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_pre(const RepDesc* __restrict__ descs,
-                                             const Tile* __restrict__ tiles, int ntl) {
+                                             const Tile* __restrict__ tiles, int ntl, int* __restrict__ nrt) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *nrt = 0;  // k_decide lists this step's rebuilding tiles
     const int w = wave_in_grid();
     if (w >= ntl) return;
     const Tile t = tiles[w];
@@ -346,9 +347,12 @@ __global__ void __launch_bounds__(256) k_pre(const RepDesc* __restrict__ descs,
 // per-replica rebuild decision (src/main.cu:486-496, skin_trigger.cuh:18-33)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(1024) k_decide(const RepDesc* __restrict__ descs,
-                                                 const int* __restrict__ reps, int nr) {
+                                                 const int* __restrict__ reps, int nr,
+                                                 const int* __restrict__ rep_t0, int* __restrict__ rlist,
+                                                 int* __restrict__ nrt) {
     __shared__ float s_max[16];
     __shared__ int s_bad[16];
+    __shared__ int s_base;
     const RepDesc& d = descs[reps[blockIdx.x]];
     RepState* st = d.st;
     if (!st->active) return;
@@ -366,36 +370,45 @@ __global__ void __launch_bounds__(1024) k_decide(const RepDesc* __restrict__ des
         s_bad[threadIdx.x >> 6] = bad;
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
-        m = fmaxf(m, s_max[w]);
-        bad |= s_bad[w];
+    if (threadIdx.x == 0) {
+        s_base = -1;
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+            m = fmaxf(m, s_max[w]);
+            bad |= s_bad[w];
+        }
+        m = fmaxf(m, s_max[0]);
+        bad |= s_bad[0];
+        if (bad) st->err |= ERR_NAN;
+        if (st->err) {
+            st->active = 0;
+            st->rebuild = 0;
+        } else {
+            // k_pre of this step placed the positions in the box st->L
+            st->Lp = st->L;
+            st->invLp = st->invL;
+            set_grid_consts(st, d.rc2);
+            const float md = fmaxf(__int_as_float(st->maxdisp_bits), m);  // largest since the last rebuild
+            const int since = st->since_rebuild + 1;
+            const int rb = (md > d.half_skin2) || (since >= kRebuildEvery);
+            st->rebuild = rb;
+            if (rb) {
+                st->since_rebuild = 0;
+                st->maxdisp_bits = 0;
+                st->nrebuilds++;
+                set_grid(d, st);
+                s_base = atomicAdd(nrt, d.ntiles);  // this replica's run of the rebuild tile list
+            } else {
+                st->since_rebuild = since;
+                st->maxdisp_bits = __float_as_int(md);
+            }
+        }
     }
-    m = fmaxf(m, s_max[0]);
-    bad |= s_bad[0];
-    if (bad) st->err |= ERR_NAN;
-    if (st->err) {
-        st->active = 0;
-        st->rebuild = 0;
-        return;
-    }
-    // k_pre of this step placed the positions in the box st->L
-    st->Lp = st->L;
-    st->invLp = st->invL;
-    set_grid_consts(st, d.rc2);
-    const float md = fmaxf(__int_as_float(st->maxdisp_bits), m);  // largest since the last rebuild
-    const int since = st->since_rebuild + 1;
-    const int rb = (md > d.half_skin2) || (since >= kRebuildEvery);
-    st->rebuild = rb;
-    if (rb) {
-        st->since_rebuild = 0;
-        st->maxdisp_bits = 0;
-        st->nrebuilds++;
-        set_grid(d, st);
-    } else {
-        st->since_rebuild = since;
-        st->maxdisp_bits = __float_as_int(md);
-    }
+    __syncthreads();
+    // the replica's tiles in the rebuild tile list (whole groups, in any
+    // order of replicas: every consumer treats each group alone)
+    const int base = s_base;
+    if (base >= 0)
+        for (int k = threadIdx.x; k < d.ntiles; k += blockDim.x) rlist[base + k] = rep_t0[blockIdx.x] + k;
 }
 
 // Radius of gyration of each chain (src/analysis/rg.cu:3-69): unwrapped
@@ -562,8 +575,14 @@ __global__ void __launch_bounds__(256) k_bin(REBUILD_ARGS) {
             const float4 r = d.pos[i];
             const int nx = st->nx;
             const float ics = st->inv_cs;
-            key = cell_key(cell_coord(r.x, ics, nx), cell_coord(r.y, ics, nx), cell_coord(r.z, ics, nx));
+            const int cx = cell_coord(r.x, ics, nx);
+            key = cell_key(cx, cell_coord(r.y, ics, nx), cell_coord(r.z, ics, nx));
             d.cell_of[i] = (int)key;
+            // the within-cell sort key (k_rank): the x grid coordinate
+            // floor(x 2^32 / L) modulo 2^32, relative to the cell's lower bound
+            // floor(cx 2^32 / nx)
+            const unsigned lo = (unsigned)(((unsigned long long)cx << 32) / (unsigned)nx);
+            d.sortx[i] = (int)(fix_coord(r.x, 4294967296.0 / (double)st->Lp) - lo);
         }
         unsigned long long todo = __ballot(valid);
         while (todo) {  // wave-uniform: one pass per distinct key
@@ -651,9 +670,10 @@ __global__ void __launch_bounds__(256) k_scatter(REBUILD_ARGS) {
 // independent of the atomic arrival order of k_bin, in which the list build
 // finds the atoms of a row of cells within reach along x by binary search.
 // x is the grid coordinate floor(x 2^32 / L) taken modulo 2^32 relative to the
-// cell's lower bound floor(cx 2^32 / nx): the list build compares grid
-// coordinates modulo 2^32, and a position a rounding step outside [0, L) (at
-// -0 or L itself) stands at the cell's end where the grid coordinate puts it.
+// cell's lower bound floor(cx 2^32 / nx) (k_bin's sortx): the list build
+// compares grid coordinates modulo 2^32, and a position a rounding step
+// outside [0, L) (at -0 or L itself) stands at the cell's end where the grid
+// coordinate puts it.
 __global__ void __launch_bounds__(256) k_rank(REBUILD_ARGS) {
     REBUILD_LOOP {
         REBUILD_PROLOGUE
@@ -662,15 +682,11 @@ __global__ void __launch_bounds__(256) k_rank(REBUILD_ARGS) {
             const int key = d.cell_of[src];
             const int s = d.cell_start[key], e = d.cell_start[key + 1];
             const int g = d.tmp_img[src].w;
-            const int nx = st->nx;
-            const double sc = 4294967296.0 / (double)st->Lp;
-            const float x = d.tmp_pos[src].x;
-            const unsigned lo = (unsigned)(((unsigned long long)cell_coord(x, st->inv_cs, nx) << 32) / (unsigned)nx);
-            const int rx = (int)(fix_coord(x, sc) - lo);
+            const int rx = d.sortx[src];
             int r = 0;
             for (int q = s; q < e; q++) {
                 const int o = d.order[q];
-                const int ro = (int)(fix_coord(d.tmp_pos[o].x, sc) - lo);
+                const int ro = d.sortx[o];
                 r += (ro < rx) || (ro == rx && d.tmp_img[o].w < g);
             }
             d.order2[s + r] = src;
@@ -2235,18 +2251,20 @@ __global__ void k_morton_keys(const RepDesc* __restrict__ dp, int natoms, uint32
 // launchers
 // ---------------------------------------------------------------------------
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
-static constexpr int kStrideBlocks = 4096;  // grid cap of the grid-stride rebuild kernels
+// grid cap of the grid-stride rebuild kernels: a step's rebuild list (about
+// 4,000 tiles) in one pass, without thousands of waves that find no tile
+static constexpr int kStrideBlocks = 1024;
 static constexpr int kBuildBlocks = 8192;   // grid cap of the list-build workgroups (one group each)
 static constexpr int kForceListBlocks = 1280;  // workgroups of a listed force pass (five per CU; a multiple of kXcds)
 static_assert(kForceListBlocks % kXcds == 0, "the listed force pass splits its grid evenly over the XCDs");
 
 void launch_pre(const Launch& l) {
     if (l.ntl <= 0) return;
-    k_pre<<<cdiv(l.ntl, 4), 256, 0, l.stream>>>(l.descs, l.tiles, l.ntl);
+    k_pre<<<cdiv(l.ntl, 4), 256, 0, l.stream>>>(l.descs, l.tiles, l.ntl, l.nrt);
 }
 void launch_decide(const Launch& l) {
     if (l.nr <= 0) return;
-    k_decide<<<l.nr, 1024, 0, l.stream>>>(l.descs, l.reps, l.nr);
+    k_decide<<<l.nr, 1024, 0, l.stream>>>(l.descs, l.reps, l.nr, l.rep_t0, l.rlist, l.nrt);
 }
 void launch_rg(const RepDesc* d, int nchains, double* out, hipStream_t s) {
     if (nchains <= 0) return;
@@ -2282,7 +2300,8 @@ static inline int valid_block(int b) { return (b == 64 || b == 128 || b == 256) 
 
 void launch_rebuild(const Launch& l, int always, int build) {
     if (l.ntl <= 0) return;
-    k_rebuild_list<<<1, 1024, 0, l.stream>>>(l.descs, l.reps, l.rep_t0, l.nr, always, l.rlist, l.nrt);
+    // a step's rebuild tile list is k_decide's; otherwise list the tiles here
+    if (always) k_rebuild_list<<<1, 1024, 0, l.stream>>>(l.descs, l.reps, l.rep_t0, l.nr, always, l.rlist, l.nrt);
     int g = cdiv(l.ntl, 4);
     if (g > kStrideBlocks) g = kStrideBlocks;
     k_bin<<<g, 256, 0, l.stream>>>(l.descs, l.tiles, l.rlist, l.nrt);
