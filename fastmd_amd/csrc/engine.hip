@@ -813,24 +813,28 @@ void Engine::decide_fork(Batch& b, bool may_sync) {
 // One MD step for every replica of the batch.  With many replicas, those that
 // rebuild their lists this step (about one in twenty) sort their atoms, build
 // their lists and compute their forces on the auxiliary stream, while the
-// force pass of all the others runs on the main stream from the end of that
-// sort: the list build's workgroups (the larger) are queued first, so they are
-// not starved, and the force pass fills the CUs they leave and free.  The
-// branches touch disjoint replicas, and a captured graph keeps them
-// concurrent.  A batch of few replicas gains little beside the cost of the
-// cross-stream waits and runs its steps in one stream.
+// force pass of all the others runs on the main stream: a share of it
+// (kEarlyForce) beside the sort, whose kernels are small and fit beside it,
+// and the rest from the end of the sort, after the list build's workgroups
+// (the larger) are queued, so they are not starved; the force pass fills the
+// CUs they leave and free.  The branches touch disjoint replicas, and a
+// captured graph keeps them concurrent.  A batch of few replicas gains little
+// beside the cost of the cross-stream waits and runs its steps in one stream.
 void Engine::launch_step_seq(Batch& b) {
     Launch l = launch_of(b);
     launch_pre(l);
     launch_decide(l);
-    auto staying = [&](int sel) {
-        // each size class's tile range with that class's tuned workgroup
-        // shape (adjacent classes of one shape in one launch)
+    // the part [f0, f1) of each size class's tile range (whole groups) with
+    // that class's tuned workgroup shape (adjacent classes of one shape in
+    // one launch)
+    auto staying = [&](int sel, double f0, double f1) {
         for (int c = 0; c < kNumClasses;) {
             int e = c + 1;
             while (e < kNumClasses && tuned_[0][e] == tuned_[0][c]) e++;
             const int t0 = b.cls_t0[c], nt = b.cls_t0[e - 1] + b.cls_nt[e - 1] - t0;
-            launch_force(l, t0, nt, tuned_[0][c], FMD_TERM_ALL, 1, FM_STEP, sel);
+            const int a = (int)(f0 * (nt / kGroupTiles)) * kGroupTiles;
+            const int z = f1 >= 1.0 ? nt : (int)(f1 * (nt / kGroupTiles)) * kGroupTiles;
+            launch_force(l, t0 + a, z - a, tuned_[0][c], FMD_TERM_ALL, 1, FM_STEP, sel);
             c = e;
         }
     };
@@ -843,12 +847,13 @@ void Engine::launch_step_seq(Batch& b) {
         launch_build(la);
         launch_force_listed(la, 1, FM_STEP);
         (void)hipEventRecord(ev_join_, aux_);
+        staying(FS_STAYING, 0.0, kEarlyForce);
         (void)hipStreamWaitEvent(stream_, ev_sorted_, 0);
-        staying(FS_STAYING);
+        staying(FS_STAYING, kEarlyForce, 1.0);
         (void)hipStreamWaitEvent(stream_, ev_join_, 0);
     } else {
         launch_rebuild(l, 0, 1);
-        staying(FS_ALL);
+        staying(FS_ALL, 0.0, 1.0);
     }
     launch_boundary(l, BND_STEP);
 }
