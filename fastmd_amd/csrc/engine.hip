@@ -910,6 +910,9 @@ int32_t Engine::launch_steps(const std::vector<int>& ids, int64_t k, bool use_gr
             rc = hip_check(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0), "GraphInstantiate");
             (void)hipGraphDestroy(g);
             if (rc) return rc;
+            // the graph's one-time setup now, not in its first launch
+            rc = hip_check(hipGraphUpload(b->graph, stream_), "GraphUpload");
+            if (rc) return rc;
             b->graph_steps = kGraphSteps;
             b->tune_version = tune_version_;
         }
