@@ -1,5 +1,7 @@
 """Batched multi-replica stepping: each replica's result is bit-identical
 alone and in the mix, with the plain one-system path and through the server."""
+import os
+
 import numpy as np
 import pytest
 
@@ -42,6 +44,26 @@ def test_alone_equals_batched(fmd, trio):
         assert np.array_equal(p, st[0]) and np.array_equal(v, st[1]) and np.array_equal(im, st[2])
         assert e.drain_thermo(sid)[0] == th
     e.close()
+
+
+def test_forked_step_equals_alone(fmd, trio):
+    """With FMD_STEP_FORK=1 the replicas that rebuild in a step sort, build their
+    lists and compute their forces on a second stream beside the other
+    replicas' force pass: Langevin, NVT-NH and NPT-NH replicas still equal
+    their alone runs (one-stream step) bit for bit, thermo rows included."""
+    alone = [_run_alone(fmd, d, 137) for d in trio]
+    os.environ["FMD_STEP_FORK"] = "1"
+    try:
+        e = fmd.Engine(0)
+        sids = [e.add(d) for d in trio]
+        e.step(sids, 137)
+        for sid, (st, th) in zip(sids, alone):
+            p, v, im = e.state(sid)
+            assert np.array_equal(p, st[0]) and np.array_equal(v, st[1]) and np.array_equal(im, st[2])
+            assert e.drain_thermo(sid)[0] == th
+        e.close()
+    finally:
+        del os.environ["FMD_STEP_FORK"]
 
 
 def test_plain_equals_graph(fmd, trio):
