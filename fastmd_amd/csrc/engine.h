@@ -27,6 +27,12 @@ constexpr int64_t kForkRecheck = 50 * kRebuildEvery;  // steps between two choic
 // 0.3, 0.5, 0.6, 0.7 and 0.85 gave 0.737, 0.730, 0.716, 0.723, 0.738 and
 // 0.774 ms per step, 0.740 without this part)
 constexpr double kEarlyForce = 0.5;
+// workgroups of the list build in that step, the GPU's slots for it (256 CUs x
+// 4): the rebuilding replicas hold about a twentieth of the groups, and
+// workgroups past the list only take slots beside the force pass (measured on
+// 255 staggered replicas against 8,192: 4,096 -3.1 us, 2,048 -4.7 to -6.5 us,
+// 1,024 -5.8 us per step)
+constexpr int kForkBuildBlocks = 1024;
 
 int size_class(int natoms);
 

@@ -844,7 +844,7 @@ void Engine::launch_step_seq(Batch& b) {
         (void)hipStreamWaitEvent(aux_, ev_fork_, 0);
         launch_rebuild(la, 0, 0);
         (void)hipEventRecord(ev_sorted_, aux_);
-        launch_build(la);
+        launch_build(la, kForkBuildBlocks);
         launch_force_listed(la, 1, FM_STEP);
         (void)hipEventRecord(ev_join_, aux_);
         staying(FS_STAYING, 0.0, kEarlyForce);

@@ -30,8 +30,9 @@ void launch_decide(const Launch& l);
 // the tiles in the rebuild tile list: every tile if `always` (listed here),
 // else those k_decide listed this step (the replicas it found rebuilding).
 void launch_rebuild(const Launch& l, int always, int build);
-// The Verlet lists of the tiles the last rebuild phases listed.
-void launch_build(const Launch& l);
+// The Verlet lists of the tiles the last rebuild phases listed, by at most
+// max_blocks workgroups (0: the default cap), each taking listed groups in turn.
+void launch_build(const Launch& l, int max_blocks = 0);
 // Force kernel over tiles [t0, t0+nt) of the list with the given block size.
 // Force kernel modes: FM_STEP skips inactive replicas and records PE and
 // virial on the steps that want them; FM_OBS always runs and records them;

@@ -2311,10 +2311,11 @@ void launch_rebuild(const Launch& l, int always, int build) {
     k_gather<<<g, 256, 0, l.stream>>>(l.descs, l.tiles, l.rlist, l.nrt);
     if (build) launch_build(l);
 }
-void launch_build(const Launch& l) {
+void launch_build(const Launch& l, int max_blocks) {
     if (l.ntl <= 0) return;
     int gb = cdiv(l.ntl, kGroupTiles);
-    if (gb > kBuildBlocks) gb = kBuildBlocks;
+    const int cap = max_blocks > 0 ? max_blocks : kBuildBlocks;
+    if (gb > cap) gb = cap;
     k_build<<<gb, kGroupSlots, 0, l.stream>>>(l.descs, l.tiles, l.rlist, l.nrt);
 }
 static void force_kernel(const Launch& l, int g, int block, int t0, int nt, int terms, int kick, int mode,
